@@ -1,0 +1,165 @@
+/* nuppn_hip.h - C ABI of the MI355X (gfx950) network-integration path.
+ *
+ * Plain pointers and sizes only.  Unless a name ends in _dev, array arguments are host memory
+ * owned by the caller and are only read or written during the call.  All indices are 0-based
+ * except in the two SuperLU entry points, which keep the reference's Fortran conventions.
+ *
+ * Every entry point replaces a reference interface, cited beside it (paths relative to the
+ * NuPPN checkout).  Errors come back as codes, never as a process exit: the reference's
+ * integration codes (utils/source/errors.F90:5-16) are positive, library errors negative.
+ */
+#ifndef NUPPN_HIP_H
+#define NUPPN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* integration error codes, utils/source/errors.F90:5-16 */
+#define NUPPN_OK            0
+#define NUPPN_SLV_CNVRG     1
+#define NUPPN_MAX_SUBSTEPS  2
+#define NUPPN_BAD_STEP      3
+#define NUPPN_NEG_ABUNDS    4
+#define NUPPN_HUGE_ABUNDS   5
+#define NUPPN_SUM_X         6
+#define NUPPN_GOT_NAN       7
+#define NUPPN_SMALL_DT      11
+/* library errors */
+#define NUPPN_ERR_ARG          (-1)   /* bad argument */
+#define NUPPN_ERR_HIP          (-2)   /* HIP runtime error / no device / library not loadable */
+#define NUPPN_ERR_PATTERN      (-3)   /* a matrix entry became exactly 0 or -1: the reference would
+                                         drop it and re-order; not reproduced by the static path */
+#define NUPPN_ERR_SINGULAR     (-4)   /* exactly zero pivot (slu_solve returns info > 0 there) */
+#define NUPPN_ERR_UNSUPPORTED  (-5)   /* configuration outside the static-network path */
+
+/* Reaction network (physics/source/rates.F90:13-16, ppn_physics.F90:746-815 conventions).
+ * k[m*nre + l], m = 0..7 holds k1..k8 of reaction l: k1,k3,k5,k7 are 0-based species indices,
+ * k2,k4,k6,k8 counts.  rtype: 0 two-body (rate x rho), 1 photodisintegration, 2 decay.
+ * coef[m*nre + l] = REACLIB a0..a6.  aw/zw: mass number and charge of each species.  igamma:
+ * the gamma pseudo-species (molar abundance 1, not a considered isotope). */
+typedef struct {
+    int nsp;
+    int nre;
+    const int *k;
+    const int *rtype;
+    const double *coef;
+    const double *aw;
+    const double *zw;
+    int igamma;
+} nuppn_network;
+
+/* solver knobs, solver/source/solver_knobs.F90:36-81 (same names and meaning) */
+typedef struct {
+    int ittd;
+    int irdn;
+    int max_sub_steps;
+    int integration_method;   /* 0 backward Euler, 1 Bader-Deuflhard */
+    int interpolate_temp;
+    int interpolate_dens;
+    double dgrd;
+    double grdthreshold;
+    double cyminformal;       /* the static-network path needs cyminformal < 0 */
+    double min_rate_ov_dt;
+    double min_t9_for_slv;
+    double delta_state_max;
+} nuppn_knobs;
+
+/* per-zone diagnostics (solver/source/solver_diagnostics.F90:7-17 plus work counters) */
+typedef struct {
+    int nsubt;
+    int titers;
+    int iter;
+    int nvar1;
+    int nsucc;
+    int nfail;
+    long long n_newton;
+    long long n_factor;
+    long long n_solve;
+} nuppn_zone_stats;
+
+typedef struct nuppn_ctx nuppn_ctx;
+
+/* readsolverinput defaults (solver_knobs.F90:43-57) with the static network (cyminformal = -1) */
+void nuppn_default_knobs(nuppn_knobs *k);
+
+/* Network set-up: makemap (solver/source/jac_rhs.F90:461-540), the static reduced pattern of
+ * I - dt J, the inverse (pull) maps for dX/dt and J, the COLAMD column order with etree
+ * post-order (SuperLU get_perm_c.c:405 + sp_preorder.c:140-150) and device upload.
+ * Replaces solver_init + makemap + the network broadcast (jac_rhs.F90:565-580). */
+nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, int *status);
+void nuppn_ctx_destroy(nuppn_ctx *ctx);
+
+/* Column order used by the factorisation, SuperLU convention (perm_c[i] = position of reduced
+ * column i).  set: caller-supplied order, like SuperLU's MY_PERMC; get: the order in use. */
+int nuppn_set_column_order(nuppn_ctx *ctx, const int *perm_c);
+int nuppn_get_column_order(nuppn_ctx *ctx, int *perm_c);
+
+/* Host-only (no GPU needed): the column order SuperLU's dgssv would use for an n x n CSC
+ * pattern - get_perm_c(COLAMD) then sp_preorder's etree post-order (SuperLU_5.0/SRC/
+ * get_perm_c.c:405, sp_preorder.c:140-150).  perm_c [n] (SuperLU convention), etree [n] or
+ * NULL (post-ordered parents, root parent = n). */
+int nuppn_column_order(int n, const int *colptr, const int *rowind, int *perm_c, int *etree);
+
+/* Reduced system pattern (CSC of I - dt J over the masked species, reduce_jacobian +
+ * csrcsc, jac_rhs.F90:134-213): sizes, then arrays of n+1 / nnz / n / nsp entries. */
+int nuppn_reduced_size(nuppn_ctx *ctx, int *n, int *nnz);
+int nuppn_reduced_pattern(nuppn_ctx *ctx, int *colptr, int *rowind, int *red2sp);
+
+/* REACLIB rates for nz thermodynamic states (reaclib_calculate_rates, physics/source/
+ * reaclib.F90:431-582, with evaluate_rates.F90:46,172 clips).  v: [nz*nre], zone-major. */
+int nuppn_rates_eval(nuppn_ctx *ctx, int nz, const double *t9, const double *rho,
+                     const double *ye, double *v);
+
+/* dX/dt (calculate_dxdt, jac_rhs.F90:217-278) for one state; x, dxdt: [nsp], v: [nre]. */
+int nuppn_dxdt(nuppn_ctx *ctx, const double *x, const double *v, double *dxdt);
+
+/* Values of the reduced matrix I - dt J in the nuppn_reduced_pattern order
+ * (calculate_jacobian + val*(-dt) + reduce_jacobian, backward_euler.F90:133-137). */
+int nuppn_jacobian_reduced(nuppn_ctx *ctx, double dt, const double *x, const double *v,
+                           double *csc_val);
+
+/* GEPP of a reduced-pattern matrix on the device (SuperLU dgstrf pivot policy, dpivotL.c) and
+ * solve of nrhs right-hand sides b [nrhs*n]; perm_r [n] receives the row pivots (SuperLU
+ * convention), nnz_lu [2] the sizes of L (without unit diagonal) and U (with diagonal). */
+int nuppn_lu_solve_reduced(nuppn_ctx *ctx, const double *csc_val, int nrhs, double *b,
+                           int *perm_r, int *nnz_lu);
+
+/* One zone: evaluate_all_rates at the cycle mid-point + integrate_network
+ * (frames/ppn/source/ppn.F90:289-301, solver/source/solver.F90:53-238).  yps: [nsp] in/out,
+ * ye in/out, *ierr: integration code.  Returns NUPPN_OK or a library error. */
+int nuppn_integrate_zone(nuppn_ctx *ctx, double *yps, double t9_0, double t9_1, double rho_0,
+                         double rho_1, double *ye, double dtwant, int *ierr,
+                         nuppn_zone_stats *stats);
+
+/* All zones of an mppnp cycle in one launch (replaces the worker loop of
+ * frames/mppnp/source/mppnp.F90:713-753).  yps: [nz*nsp], zone-major (Fortran yps(nsp,nz));
+ * t9_0/t9_1/rho_0/rho_1/dtwant/ye: [nz]; ierr: [nz]; stats: [nz] or NULL. */
+int nuppn_integrate_batch(nuppn_ctx *ctx, int nz, double *yps, const double *t9_0,
+                          const double *t9_1, const double *rho_0, const double *rho_1,
+                          double *ye, const double *dtwant, int *ierr, nuppn_zone_stats *stats);
+
+/* Same with device-resident arrays, enqueued on `stream` (a hipStream_t, NULL = default). */
+int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double *t9_0,
+                              const double *t9_1, const double *rho_0, const double *rho_1,
+                              double *ye, const double *dtwant, int *ierr,
+                              nuppn_zone_stats *stats, void *stream);
+
+/* Kernel timing of the last batch call (HIP events on the launch stream), milliseconds:
+ * [0] rate kernel, [1] integration kernel. */
+int nuppn_last_timing(nuppn_ctx *ctx, float *ms2);
+
+/* Link-level drop-ins with the reference signatures and semantics.
+ * slu_solve: solver/source/slu_f90/slu_f90.c:3-54 (Fortran interface slu_f90/slu.f90:6-11);
+ *   colind/rowptr are shifted to 0-based in place when shiftindex != 0, as the reference does.
+ * c_fortran_dgssv_: solver/source/SuperLU_5.0/FORTRAN/c_fortran_dgssv.c:35-190
+ *   (iopt 1 factor -> handle, 2 solve, 3 free; 1-based CSC). */
+void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int colind[],
+               int rowptr[], double rhs[], int *ierr);
+void c_fortran_dgssv_(int *iopt, int *n, int *nnz, int *nrhs, double *values, int *rowind,
+                      int *colptr, double *b, int *ldb, long long *f_factors, int *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,400 @@
+// Device building blocks shared by the kernels: wavefront helpers, REACLIB rates, the pull-form
+// dX/dt and Jacobian assembly, and the left-looking GEPP with SuperLU's pivot policy.
+//
+// One wavefront (64 lanes) owns one zone.  Every floating-point result that the reference
+// computes sequentially is produced here with the same operations in the same order per
+// element (the translation unit is compiled with -ffp-contract=off), so results are equal bit
+// for bit to a sequential evaluation given equal inputs; only the libm calls of the rate
+// formula (exp, log, cbrt) may differ in the last place from the host's.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nuppn {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    return (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
+}
+
+__device__ __forceinline__ int readlane_i(int v, int l) { return __shfl(v, l, WAVE); }
+
+__device__ __forceinline__ double readlane_d(double v, int l) { return __shfl(v, l, WAVE); }
+
+// compiler-level ordering point between dependent LDS phases of one wavefront
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); __syncthreads(); }
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+// s = ((0 + a[0]) + a[1]) + ... : the sequential order of Fortran's SUM (every lane gets s)
+__device__ inline double seq_sum(const double *a, int n) {
+    double s = 0.0;
+    for (int base = 0; base < n; base += WAVE) {
+        int i = base + lane_id();
+        double v = (i < n) ? a[i] : 0.0;
+        int cnt = min(WAVE, n - base);
+        for (int t = 0; t < cnt; ++t) s = s + readlane_d(v, t);
+    }
+    return s;
+}
+
+// real ** integer as gfortran evaluates it (libgcc __powidf2: square-and-multiply)
+__device__ __forceinline__ double powi(double x, int m) {
+    unsigned int n = m < 0 ? -(unsigned int) m : (unsigned int) m;
+    double y = (n % 2) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+__device__ __constant__ const double kFact[5] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0};
+
+// T9 powers of reaclib_calculate_rates (physics/source/reaclib.F90:443,473-479)
+struct T9Pow {
+    double t9, t9inv, p13, p13inv, p53, lnt9;
+};
+
+__device__ __forceinline__ T9Pow t9_powers(double t9in) {
+    T9Pow p;
+    p.t9 = fmax(t9in, 0.01);
+    p.t9inv = 1.0 / p.t9;
+    p.p13 = pow(p.t9, 1.0 / 3.0);
+    p.p13inv = 1.0 / p.p13;
+    double p43 = p.t9 * p.p13;
+    p.p53 = p.p13 * p43;
+    p.lnt9 = log(p.t9);
+    return p;
+}
+
+// one rate: REACLIB sum in the reference's order, exp, density scaling by reaction type,
+// the 1e-99 floors of reaclib.F90:581 and evaluate_rates.F90:46,172
+__device__ __forceinline__ double reaclib_rate(const double *__restrict__ coef, int nre, int l,
+                                               int rtype, const T9Pow &p, double rho) {
+    double r0 = coef[l] + coef[nre + l] * p.t9inv;
+    r0 = r0 + coef[2 * nre + l] * p.p13inv;
+    r0 = r0 + coef[3 * nre + l] * p.p13;
+    r0 = r0 + coef[4 * nre + l] * p.t9;
+    r0 = r0 + coef[5 * nre + l] * p.p53;
+    r0 = r0 + coef[6 * nre + l] * p.lnt9;
+    double reacv = 0.0 + exp(r0);
+    double rv = (rtype == 0) ? reacv * rho : reacv;
+    rv = fmax(rv, 1.0e-99);
+    rv = fmax(rv * 1.0, 1.0e-99);
+    return fmax(rv, 1.0e-99);
+}
+
+// ------------------------------------------------------------------------------------------
+// Static network description on the device (built once on the host).
+struct DevNet {
+    int nsp, nre, n, nnzA, igamma;
+    const int *k;           // [8*nre]
+    const int *rtype;       // [nre]
+    const double *coef;     // [7*nre]
+    const double *aw;       // [nsp]
+    const double *zw;       // [nsp]
+    const int *red2sp;      // [n] reduced index -> species
+    // dX/dt pull map over reduced species: entries (l << 2 | role), role 0..3 = k1,k3,k5,k7
+    const int *dx_ptr;      // [n+1]
+    const int *dx_ent;
+    // Jacobian pull map over reduced CSC entries q: entries (l << 3 | m), m = statement order
+    // ice1..ice8 of calculate_jacobian (jac_rhs.F90:365-392)
+    const int *jq_ptr;      // [nnzA+1]
+    const int *jq_ent;
+    const int *A_colptr;    // [n+1] reduced CSC pattern
+    const int *A_rowind;    // [nnzA]
+    const unsigned char *A_isdiag;  // [nnzA]
+    const int *perm_c;      // [n] SuperLU convention
+    const int *iperm_c;     // [n]
+};
+
+// xn = X/A for considered isotopes (A > 0), xn(gamma) = 1 (jac_rhs.F90:235-240, 312-317)
+__device__ inline void molar(const DevNet &N, const double *x, double *xn) {
+    for (int i = lane_id(); i < N.nsp; i += WAVE)
+        xn[i] = (i != N.igamma && N.aw[i] > 0.0) ? x[i] / N.aw[i] : 0.0;
+    if (lane_id() == 0) xn[N.igamma] = 1.0;
+}
+
+// per-reaction quantities: vx (calculate_dxdt flux, jac_rhs.F90:245) and the two Jacobian stems
+// (jac_rhs.F90:355, 359)
+__device__ inline void reaction_terms(const DevNet &N, const double *xn, const double *v,
+                                      double *vx, double *st1, double *st2) {
+    const int nre = N.nre;
+    for (int l = lane_id(); l < nre; l += WAVE) {
+        const int in1 = N.k[l], n1 = N.k[nre + l], in2 = N.k[2 * nre + l], n2 = N.k[3 * nre + l];
+        const double y1 = xn[in1], y2 = xn[in2], vl = v[l];
+        vx[l] = kFact[n1] * vl * powi(y1, n1) * y2;
+        st1[l] = (double) n1 * kFact[n1] * kFact[n2] * powi(y2, n2) * powi(y1, n1 - 1) * vl;
+        st2[l] = (double) n2 * kFact[n1] * kFact[n2] * powi(y1, n1) * powi(y2, n2 - 1) * vl;
+    }
+}
+
+// dX/dt of reduced species r from the per-reaction fluxes, accumulated in reaction order
+// (calculate_dxdt's scatter order), times A (jac_rhs.F90:250-272)
+__device__ __forceinline__ double dxdt_species(const DevNet &N, int r, const double *vx) {
+    const int nre = N.nre;
+    double acc = 0.0;
+    for (int e = N.dx_ptr[r]; e < N.dx_ptr[r + 1]; ++e) {
+        const int ent = N.dx_ent[e], l = ent >> 2, role = ent & 3;
+        const double f = vx[l];
+        double term;
+        if (role == 0) term = -N.k[nre + l] * f;            // -k2 vx
+        else if (role == 1) term = -f;                      // -vx
+        else if (role == 2) term = N.k[5 * nre + l] * f;    // k6 vx
+        else term = N.k[7 * nre + l] * f;                   // k8 vx
+        acc = acc + term;
+    }
+    return acc * N.aw[N.red2sp[r]];
+}
+
+// Jacobian slot value before scaling: 1e-99 seed + contributions in reaction order
+// (jac_rhs.F90:309, 365-392)
+__device__ __forceinline__ double jac_entry(const DevNet &N, int q, const double *st1,
+                                            const double *st2) {
+    const int nre = N.nre;
+    const double *an = N.aw;
+    double acc = 1.0e-99;
+    for (int e = N.jq_ptr[q]; e < N.jq_ptr[q + 1]; ++e) {
+        const int ent = N.jq_ent[e], l = ent >> 3, m = ent & 7;
+        const int in1 = N.k[l], n1 = N.k[nre + l], in2 = N.k[2 * nre + l], n2 = N.k[3 * nre + l];
+        const int out1 = N.k[4 * nre + l], no1 = N.k[5 * nre + l], out2 = N.k[6 * nre + l];
+        const int no2 = N.k[7 * nre + l];
+        const double s1 = st1[l], s2 = st2[l];
+        double t;
+        switch (m) {
+            case 0: t = (-n1 * s1) * an[in1] / an[in1]; break;
+            case 1: t = (-n2 * s1) * an[in2] / an[in1]; break;
+            case 2: t = (-n1 * s2) * an[in1] / an[in2]; break;
+            case 3: t = (no1 * s1) * an[out1] / an[in1]; break;
+            case 4: t = (no2 * s1) * an[out2] / an[in1]; break;
+            case 5: t = (-n2 * s2) * an[in2] / an[in2]; break;
+            case 6: t = (no1 * s2) * an[out1] / an[in2]; break;
+            default: t = (no2 * s2) * an[out2] / an[in2]; break;
+        }
+        acc = acc + t;
+    }
+    return acc;
+}
+
+// Reduced matrix values of I - scale*J (newton_iteration val*(-dt) + reduce_jacobian,
+// jac_rhs.F90:140-161): returns 0, or NUPPN_ERR_PATTERN when an entry would be dropped by the
+// reference (exact 0 or -1 before the identity) or the diagonal ends exactly 0.
+__device__ inline int assemble_matrix(const DevNet &N, double scale, const double *st1,
+                                      const double *st2, double *Aval) {
+    int bad = 0;
+    for (int q = lane_id(); q < N.nnzA; q += WAVE) {
+        double jv = jac_entry(N, q, st1, st2);
+        double val = jv * (-scale);
+        if (N.A_isdiag[q]) {
+            val = val + 1.0;
+            if (val == 0.0) bad = 1;
+        } else {
+            if (jv == 0.0 || fabs(val) == 0.0 || val == -1.0) bad = 1;
+        }
+        Aval[q] = val;
+    }
+    return ballot(bad) ? -3 : 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Left-looking GEPP on one wavefront.
+struct LUWork {           // LDS
+    double *x;            // [n] dense column / solve vector
+    unsigned char *flag;  // [n]
+    int *perm_r;          // [n]
+    int *pivrow;          // [n]
+    int *touched;         // [n]
+    int *Lp;              // [n+1]
+};
+
+struct LUStore {          // global, per zone slot
+    int *Li;
+    double *Lx;
+    int *Ui;
+    double *Ux;
+    int *Up;              // [n+1]
+};
+
+// Factor the n x n CSC matrix (Acp, Ari, Aval; columns taken in the order iperm_c).  Pivot
+// policy of SuperLU's dpivotL (SuperLU_5.0/SRC/dpivotL.c:114-156): the largest |value| among
+// the not-yet-pivoted rows of the updated column (first in discovery order on ties) unless the
+// original diagonal is nonzero and >= 1.0 * that maximum; L scaled by the reciprocal of the
+// pivot (dpivotL.c:178-180).  Column j is updated by the pivot columns k < j in ascending k,
+// so every entry receives its updates in the same order as a sequential left-looking sweep.
+// Returns 0 or j+1 for the first column without a nonzero pivot (factorisation stops there).
+__device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
+                                  const int *__restrict__ Ari, const double *__restrict__ Aval,
+                                  const int *__restrict__ iperm_c, LUWork w, LUStore s) {
+    const int lane = lane_id();
+    const uint64_t below = lanes_below();
+    for (int i = lane; i < n; i += WAVE) {
+        w.x[i] = 0.0;
+        w.flag[i] = 0;
+        w.perm_r[i] = -1;
+    }
+    if (lane == 0) { w.Lp[0] = 0; s.Up[0] = 0; }
+    wave_sync();
+    int nl = 0, nu = 0;
+    for (int j = 0; j < n; ++j) {
+        const int c = iperm_c[j];
+        const int a0 = Acp[c], a1 = Acp[c + 1];
+        int nt = 0;
+        for (int base = a0; base < a1; base += WAVE) {
+            const int p = base + lane;
+            const bool act = p < a1;
+            const int r = act ? Ari[p] : 0;
+            const uint64_t m = ballot(act);
+            if (act) {
+                w.x[r] = Aval[p];
+                w.flag[r] = 1;
+                w.touched[nt + popc(m & below)] = r;
+            }
+            nt += popc(m);
+        }
+        wave_sync();
+        // updates from earlier pivot columns, ascending k
+        for (int k0 = 0; k0 < j; k0 += WAVE) {
+            const int kl = k0 + lane;
+            const int prl = (kl < j) ? w.pivrow[kl] : 0;
+            int done = -1;
+            for (;;) {
+                const bool cand = (kl < j) && (lane > done) && w.flag[prl];
+                const uint64_t m = ballot(cand);
+                if (!m) break;
+                const int t = __ffsll((unsigned long long) m) - 1;
+                done = t;
+                const int k = k0 + t;
+                const int pr = readlane_i(prl, t);
+                const double xk = w.x[pr];
+                if (lane == 0) { s.Ui[nu] = k; s.Ux[nu] = xk; }
+                ++nu;
+                const int l0 = w.Lp[k], l1 = w.Lp[k + 1];
+                for (int base = l0; base < l1; base += WAVE) {
+                    const int p = base + lane;
+                    const bool act = p < l1;
+                    const int i = act ? s.Li[p] : 0;
+                    const double lv = act ? s.Lx[p] : 0.0;
+                    const bool isnew = act && !w.flag[i];
+                    const uint64_t nm = ballot(isnew);
+                    if (isnew) {
+                        w.touched[nt + popc(nm & below)] = i;
+                        w.flag[i] = 1;
+                        w.x[i] = 0.0;
+                    }
+                    nt += popc(nm);
+                    if (act) w.x[i] = w.x[i] - lv * xk;
+                    wave_sync();
+                }
+            }
+        }
+        // pivot search over the not-yet-pivoted rows, in discovery order
+        double best = 0.0;
+        int bestt = 0x7fffffff, firstt = 0x7fffffff, diagt = -1;
+        for (int t = lane; t < nt; t += WAVE) {
+            const int r = w.touched[t];
+            if (w.perm_r[r] < 0) {
+                const double a = fabs(w.x[r]);
+                if (a > best) { best = a; bestt = t; }
+                if (t < firstt) firstt = t;
+                if (r == c) diagt = t;
+            }
+        }
+        const double gmax = wave_max(best);
+        const int bt = wave_min_i((best == gmax && gmax > 0.0) ? bestt : 0x7fffffff);
+        firstt = wave_min_i(firstt);
+        diagt = wave_max_i(diagt);
+        if (firstt == 0x7fffffff || gmax == 0.0) return j + 1;
+        int pt = bt;
+        if (diagt >= 0) {
+            const double a = fabs(w.x[w.touched[diagt]]);
+            if (a != 0.0 && a >= 1.0 * gmax) pt = diagt;
+        }
+        const int piv = w.touched[pt];
+        const double pv = w.x[piv];
+        if (lane == 0) {
+            s.Ui[nu] = j;
+            s.Ux[nu] = pv;
+            s.Up[j + 1] = nu + 1;
+            w.perm_r[piv] = j;
+            w.pivrow[j] = piv;
+        }
+        ++nu;
+        wave_sync();
+        const double temp = 1.0 / pv;
+        for (int base = 0; base < nt; base += WAVE) {
+            const int t = base + lane;
+            const bool act = t < nt;
+            const int r = act ? w.touched[t] : 0;
+            const bool inl = act && w.perm_r[r] < 0;
+            const uint64_t m = ballot(inl);
+            if (inl) {
+                const int q = nl + popc(m & below);
+                s.Li[q] = r;
+                s.Lx[q] = w.x[r] * temp;
+            }
+            nl += popc(m);
+        }
+        if (lane == 0) w.Lp[j + 1] = nl;
+        for (int t = lane; t < nt; t += WAVE) {
+            const int r = w.touched[t];
+            w.flag[r] = 0;
+            w.x[r] = 0.0;
+        }
+        wave_sync();
+    }
+    return 0;
+}
+
+// Solve with the factors: b <- Pc U^-1 L^-1 Pr b (dgstrs NOTRANS order, SuperLU_5.0/SRC/
+// dgstrs.c:149-317).  b is global [n]; uses w.x as the work vector.
+__device__ inline void gepp_solve(int n, const int *__restrict__ perm_c, LUWork w, LUStore s,
+                                  double *b) {
+    const int lane = lane_id();
+    double *y = w.x;
+    for (int i = lane; i < n; i += WAVE) y[w.perm_r[i]] = b[i];
+    wave_sync();
+    for (int j = 0; j < n; ++j) {
+        const double yj = y[j];
+        const int l0 = w.Lp[j], l1 = w.Lp[j + 1];
+        for (int p = l0 + lane; p < l1; p += WAVE) {
+            const int i = w.perm_r[s.Li[p]];
+            y[i] = y[i] - s.Lx[p] * yj;
+        }
+        wave_sync();
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        const int u0 = s.Up[j], pd = s.Up[j + 1] - 1;
+        const double yj = y[j] / s.Ux[pd];
+        for (int p = u0 + lane; p < pd; p += WAVE) {
+            const int k = s.Ui[p];
+            y[k] = y[k] - s.Ux[p] * yj;
+        }
+        if (lane == 0) y[j] = yj;
+        wave_sync();
+    }
+    for (int i = lane; i < n; i += WAVE) b[i] = y[perm_c[i]];
+    wave_sync();
+}
+
+}  // namespace nuppn

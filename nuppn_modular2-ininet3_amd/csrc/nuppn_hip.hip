@@ -1,0 +1,833 @@
+// MI355X (gfx950) network-integration path: kernels, host set-up and the C ABI of
+// include/nuppn_hip.h.
+//
+// Kernels
+//   rates_kernel      REACLIB rates for (reactions x zones): a coalesced structure-of-arrays
+//                     stream, one lane per (reaction, zone), zones along gridDim.y.
+//   zone_init/zone_control/zone_newton kernels: integrate_network for a batch of zones as a
+//                     per-zone state machine (one wavefront per zone): the control kernel runs the
+//                     sub-step and Newton decisions, the Newton kernel one Newton iteration
+//                     (Jacobian, rhs, GEPP, solve) for every zone that needs one; the host loops
+//                     until no zone needs another iteration.
+//   lu_kernel         factor + solve of given reduced-pattern matrices (test entry and the
+//                     SuperLU drop-ins).
+//   dxdt_kernel / jac_kernel  single-state assembly (test entries).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/nuppn_hip.h"
+#include "device_common.hpp"
+#include "integrate.hpp"
+#include "zone.hpp"
+#include "symbolic.hpp"
+
+using namespace nuppn;
+
+#define HIP_OK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "nuppn_hip: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                  \
+            return NUPPN_ERR_HIP;                                                         \
+        }                                                                                 \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// kernels
+
+__global__ void __launch_bounds__(256) rates_kernel(const double *__restrict__ coef,
+                                                    const int *__restrict__ rtype, int nre, int nz,
+                                                    const double *__restrict__ t9_0,
+                                                    const double *__restrict__ t9_1,
+                                                    const double *__restrict__ rho_0,
+                                                    const double *__restrict__ rho_1,
+                                                    double *__restrict__ v) {
+    const int z = blockIdx.y;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nz || l >= nre) return;
+    // evaluate_all_rates at the cycle mid-point (ppn.F90:287-298, mppnp.F90:719)
+    const double t9 = 0.5 * (t9_0[z] + t9_1[z]);
+    const double rho = 0.5 * (rho_0[z] + rho_1[z]);
+    const T9Pow p = t9_powers(t9);
+    v[(size_t) z * nre + l] = reaclib_rate(coef, nre, l, rtype[l], p, rho);
+}
+
+
+
+__global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) { zone_init(a, blockIdx.x); }
+
+__global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) { zone_control(a, blockIdx.x); }
+
+// one Newton iteration for every zone in PH_NEWTON; each wavefront owns one scratch slot and
+// strides over the zones
+__global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const LUWork w = make_work(smem, a.N.n);
+    const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
+    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton(a, S, w, z);
+}
+
+__global__ void flag_reset_kernel(int *flag) { flag[0] = 0; }
+
+// single-state assembly entries (test hooks): mode 0 dX/dt, mode 1 reduced matrix values
+__global__ void __launch_bounds__(64) assemble_kernel(KArgs a, int mode, const double *x,
+                                                     const double *v, double dt, double *out,
+                                                     int *status) {
+    const DevNet &N = a.N;
+    const Slot S = make_slot(a.slot_base, a.off);
+    molar(N, x, S.xn);
+    wave_sync();
+    reaction_terms(N, S.xn, v, S.vx, S.st1, S.st2);
+    wave_sync();
+    if (mode == 0) {
+        for (int i = lane_id(); i < N.nsp; i += WAVE) out[i] = 0.0;
+        wave_sync();
+        for (int r = lane_id(); r < N.n; r += WAVE) out[N.red2sp[r]] = dxdt_species(N, r, S.vx);
+    } else {
+        const int e = assemble_matrix(N, dt, S.st1, S.st2, out);
+        if (lane_id() == 0) status[0] = e;
+    }
+}
+
+// factor + solve given matrices of a fixed pattern (test hook and SuperLU drop-ins).
+// Acp/Ari/perm_c/iperm_c: pattern; vals: [nrhs_sets? no: one matrix]; b: [nrhs*n] in/out
+__global__ void __launch_bounds__(64) lu_kernel(int n, const int *Acp, const int *Ari,
+                                               const double *vals, const int *perm_c,
+                                               const int *iperm_c, int nrhs, double *b,
+                                               LUStore st, int *perm_r_out, int *Lp_out,
+                                               int *info_out, int factor) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    LUWork w = make_work(smem, n);
+    int info = 0;
+    if (factor) {
+        info = gepp_factor(n, Acp, Ari, vals, iperm_c, w, st);
+        for (int i = lane_id(); i < n; i += WAVE) perm_r_out[i] = w.perm_r[i];
+        for (int i = lane_id(); i <= n; i += WAVE) Lp_out[i] = w.Lp[i];
+        wave_sync();
+    } else {
+        for (int i = lane_id(); i < n; i += WAVE) w.perm_r[i] = perm_r_out[i];
+        for (int i = lane_id(); i <= n; i += WAVE) w.Lp[i] = Lp_out[i];
+        wave_sync();
+    }
+    if (info == 0)
+        for (int r = 0; r < nrhs; ++r) gepp_solve(n, perm_c, w, st, b + (size_t) r * n);
+    if (lane_id() == 0) info_out[0] = info;
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+
+namespace {
+
+template <class T>
+int dev_upload(T **dptr, const T *src, size_t count) {
+    if (count == 0) count = 1;
+    HIP_OK(hipMalloc((void **) dptr, sizeof(T) * count));
+    if (src) HIP_OK(hipMemcpy(*dptr, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <class T>
+int dev_upload(T **dptr, const std::vector<T> &v) {
+    return dev_upload(dptr, v.data(), v.size());
+}
+
+void dev_free(void *p) {
+    if (p) (void) hipFree(p);
+}
+
+}  // namespace
+
+struct nuppn_ctx {
+    HostNetwork net;
+    Symbolic sym;
+    nuppn_knobs kn;
+    std::vector<int> iperm_c;
+    // device copies of the network and symbolic data
+    int *d_k = nullptr, *d_rtype = nullptr, *d_red2sp = nullptr, *d_dx_ptr = nullptr;
+    int *d_dx_ent = nullptr, *d_jq_ptr = nullptr, *d_jq_ent = nullptr, *d_Acp = nullptr;
+    int *d_Ari = nullptr, *d_perm_c = nullptr, *d_iperm_c = nullptr;
+    unsigned char *d_isdiag = nullptr;
+    double *d_coef = nullptr, *d_aw = nullptr, *d_zw = nullptr;
+    // persistent-kernel slots
+    SlotLayout layout{};
+    size_t lds = 0;
+    int max_slots = 0, nslots = 0;
+    char *d_slots = nullptr;
+    int *d_flag = nullptr;
+    int *h_flag = nullptr;   // pinned
+    // per-zone state for batches
+    int nz_state_cap = 0;
+    double *d_yps_old = nullptr, *d_y_old = nullptr, *d_y_cur = nullptr, *d_y_new = nullptr;
+    double *d_work = nullptr;
+    ZoneState *d_state = nullptr;
+    size_t work_stride = 0;
+    // staging for host-array calls
+    int nz_cap = 0, nz_vcap = 0;
+    double *d_yps = nullptr, *d_t9_0 = nullptr, *d_t9_1 = nullptr, *d_rho_0 = nullptr;
+    double *d_rho_1 = nullptr, *d_dt = nullptr, *d_ye = nullptr, *d_v = nullptr;
+    int *d_ierr = nullptr;
+    ZoneStats *d_stats = nullptr;
+    int *d_status = nullptr;
+    hipEvent_t ev[4] = {};
+    float last_ms[2] = {0.f, 0.f};
+    long long last_passes = 0;
+    std::mutex mu;
+};
+
+static DevNet dev_net(const nuppn_ctx *c) {
+    DevNet N;
+    N.nsp = c->net.nsp;
+    N.nre = c->net.nre;
+    N.n = c->sym.n;
+    N.nnzA = c->sym.nnzA;
+    N.igamma = c->net.igamma;
+    N.k = c->d_k;
+    N.rtype = c->d_rtype;
+    N.coef = c->d_coef;
+    N.aw = c->d_aw;
+    N.zw = c->d_zw;
+    N.red2sp = c->d_red2sp;
+    N.dx_ptr = c->d_dx_ptr;
+    N.dx_ent = c->d_dx_ent;
+    N.jq_ptr = c->d_jq_ptr;
+    N.jq_ent = c->d_jq_ent;
+    N.A_colptr = c->d_Acp;
+    N.A_rowind = c->d_Ari;
+    N.A_isdiag = c->d_isdiag;
+    N.perm_c = c->d_perm_c;
+    N.iperm_c = c->d_iperm_c;
+    return N;
+}
+
+static DevKnobs dev_knobs(const nuppn_knobs &k) {
+    DevKnobs K;
+    K.ittd = k.ittd;
+    K.max_sub_steps = k.max_sub_steps;
+    K.integration_method = k.integration_method;
+    K.interpolate_temp = k.interpolate_temp;
+    K.interpolate_dens = k.interpolate_dens;
+    K.dgrd = k.dgrd;
+    K.grdthreshold = k.grdthreshold;
+    K.min_t9_for_slv = k.min_t9_for_slv;
+    K.delta_state_max = k.delta_state_max;
+    return K;
+}
+
+static int ensure_slots(nuppn_ctx *c, int want) {
+    want = std::max(1, std::min(want, c->max_slots));
+    if (want <= c->nslots) return 0;
+    dev_free(c->d_slots);
+    c->d_slots = nullptr;
+    c->nslots = 0;
+    HIP_OK(hipMalloc((void **) &c->d_slots, c->layout.bytes * (size_t) want));
+    c->nslots = want;
+    return 0;
+}
+
+static int ensure_zone_buffers(nuppn_ctx *c, int nz) {
+    if (nz <= c->nz_cap) return 0;
+    for (void *p : {(void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1, (void *) c->d_rho_0,
+                    (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye, (void *) c->d_ierr,
+                    (void *) c->d_stats})
+        dev_free(p);
+    const size_t nsp = c->net.nsp;
+    HIP_OK(hipMalloc((void **) &c->d_yps, sizeof(double) * nsp * nz));
+    HIP_OK(hipMalloc((void **) &c->d_t9_0, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_t9_1, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_rho_0, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_rho_1, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_dt, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_ye, sizeof(double) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_ierr, sizeof(int) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_stats, sizeof(ZoneStats) * nz));
+    c->nz_cap = nz;
+    return 0;
+}
+
+static int ensure_rate_buffer(nuppn_ctx *c, int nz) {
+    if (nz <= c->nz_vcap) return 0;
+    dev_free(c->d_v);
+    HIP_OK(hipMalloc((void **) &c->d_v, sizeof(double) * (size_t) c->net.nre * nz));
+    c->nz_vcap = nz;
+    return 0;
+}
+
+static int ensure_zone_state(nuppn_ctx *c, int nz) {
+    if (nz <= c->nz_state_cap) return 0;
+    for (void *p : {(void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
+                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state})
+        dev_free(p);
+    const size_t nsp = c->net.nsp;
+    size_t pow2 = 1;
+    while (pow2 < nsp) pow2 <<= 1;
+    c->work_stride = nsp + pow2;
+    HIP_OK(hipMalloc((void **) &c->d_yps_old, sizeof(double) * nsp * nz));
+    HIP_OK(hipMalloc((void **) &c->d_y_old, sizeof(double) * nsp * nz));
+    HIP_OK(hipMalloc((void **) &c->d_y_cur, sizeof(double) * nsp * nz));
+    HIP_OK(hipMalloc((void **) &c->d_y_new, sizeof(double) * nsp * nz));
+    HIP_OK(hipMalloc((void **) &c->d_work, sizeof(double) * c->work_stride * nz));
+    HIP_OK(hipMalloc((void **) &c->d_state, sizeof(ZoneState) * nz));
+    c->nz_state_cap = nz;
+    return 0;
+}
+
+static int upload_order(nuppn_ctx *c) {
+    const int n = c->sym.n;
+    c->iperm_c.assign(n, 0);
+    for (int i = 0; i < n; ++i) c->iperm_c[c->sym.perm_c[i]] = i;
+    HIP_OK(hipMemcpy(c->d_perm_c, c->sym.perm_c.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_iperm_c, c->iperm_c.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static KArgs kargs(nuppn_ctx *c) {
+    KArgs a;
+    memset(&a, 0, sizeof(a));
+    a.N = dev_net(c);
+    a.K = dev_knobs(c->kn);
+    a.slot_base = c->d_slots;
+    a.slot_bytes = c->layout.bytes;
+    for (int i = 0; i < S_COUNT; ++i) a.off[i] = c->layout.off[i];
+    return a;
+}
+
+extern "C" {
+
+void nuppn_default_knobs(nuppn_knobs *k) {
+    k->ittd = 6;
+    k->irdn = 2;
+    k->max_sub_steps = 4000;
+    k->integration_method = 0;
+    k->interpolate_temp = 1;
+    k->interpolate_dens = 1;
+    k->dgrd = 1.0e-3;
+    k->grdthreshold = 1.0e-15;
+    k->cyminformal = -1.0;
+    k->min_rate_ov_dt = -1.0e99;
+    k->min_t9_for_slv = 0.01;
+    k->delta_state_max = 1.0e99;
+}
+
+int nuppn_column_order(int n, const int *colptr, const int *rowind, int *perm_c, int *etree) {
+    if (n <= 0 || !colptr || !rowind || !perm_c) return NUPPN_ERR_ARG;
+    std::vector<int> pc, et;
+    const int rc = superlu_column_order(n, colptr, rowind, pc, &et);
+    if (rc) return NUPPN_ERR_ARG;
+    std::copy(pc.begin(), pc.end(), perm_c);
+    if (etree) std::copy(et.begin(), et.end(), etree);
+    return NUPPN_OK;
+}
+
+nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, int *status) {
+    int dummy;
+    if (!status) status = &dummy;
+    *status = NUPPN_ERR_ARG;
+    if (!net || !net->k || !net->rtype || !net->coef || !net->aw || !net->zw) return nullptr;
+    nuppn_knobs kn;
+    if (knobs) kn = *knobs;
+    else nuppn_default_knobs(&kn);
+    if (kn.cyminformal >= 0.0 || kn.irdn != 2) {
+        // the device path uses the static reduced pattern (every species of every reaction);
+        // the reference's dynamic network (cyminformal >= 0) changes it per iteration
+        *status = NUPPN_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    if (kn.integration_method != 0) {
+        *status = NUPPN_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    kn.min_rate_ov_dt = -1.0e99;   // solver_knobs.F90:66-69 for cyminformal < 1e-99
+    auto c = std::make_unique<nuppn_ctx>();
+    c->kn = kn;
+    HostNetwork &h = c->net;
+    h.nsp = net->nsp;
+    h.nre = net->nre;
+    h.igamma = net->igamma;
+    h.k.assign(net->k, net->k + 8 * (size_t) net->nre);
+    h.rtype.assign(net->rtype, net->rtype + net->nre);
+    h.coef.assign(net->coef, net->coef + 7 * (size_t) net->nre);
+    h.aw.assign(net->aw, net->aw + net->nsp);
+    h.zw.assign(net->zw, net->zw + net->nsp);
+    if (build_symbolic(h, c->sym) != 0) return nullptr;
+    Symbolic &s = c->sym;
+    if (superlu_column_order(s.n, s.A_colptr.data(), s.A_rowind.data(), s.perm_c) != 0) return nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        *status = NUPPN_ERR_HIP;
+        return nullptr;
+    }
+    *status = NUPPN_ERR_HIP;
+    if (dev_upload(&c->d_k, h.k) || dev_upload(&c->d_rtype, h.rtype) ||
+        dev_upload(&c->d_coef, h.coef) || dev_upload(&c->d_aw, h.aw) ||
+        dev_upload(&c->d_zw, h.zw) || dev_upload(&c->d_red2sp, s.red2sp) ||
+        dev_upload(&c->d_dx_ptr, s.dx_ptr) || dev_upload(&c->d_dx_ent, s.dx_ent) ||
+        dev_upload(&c->d_jq_ptr, s.jq_ptr) || dev_upload(&c->d_jq_ent, s.jq_ent) ||
+        dev_upload(&c->d_Acp, s.A_colptr) || dev_upload(&c->d_Ari, s.A_rowind) ||
+        dev_upload(&c->d_isdiag, s.A_isdiag) ||
+        dev_upload(&c->d_perm_c, (const int *) nullptr, (size_t) s.n) ||
+        dev_upload(&c->d_iperm_c, (const int *) nullptr, (size_t) s.n) ||
+        dev_upload(&c->d_flag, (const int *) nullptr, 1) ||
+        dev_upload(&c->d_status, (const int *) nullptr, 4))
+        return nullptr;
+    if (upload_order(c.get())) return nullptr;
+    c->layout = slot_layout(h.nsp, h.nre, s.n, s.nnzA);
+    c->lds = lds_bytes(s.n);
+    if (c->lds > 160 * 1024) {
+        *status = NUPPN_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    if (c->lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *) zone_newton_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int) c->lds) != hipSuccess)
+        return nullptr;
+    if (c->lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) lu_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) c->lds);
+    int dev = 0, ncu = 0, per_cu = 0;
+    (void) hipGetDevice(&dev);
+    (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *) zone_newton_kernel, 64,
+                                                     c->lds) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    size_t free_b = 0, total_b = 0;
+    (void) hipMemGetInfo(&free_b, &total_b);
+    const size_t budget = free_b / 2;
+    int by_mem = (int) std::max<size_t>(1, budget / c->layout.bytes);
+    c->max_slots = std::max(1, std::min(ncu * per_cu, by_mem));
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipHostMalloc((void **) &c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return nullptr;
+    if (ensure_slots(c.get(), 1)) return nullptr;
+    *status = NUPPN_OK;
+    return c.release();
+}
+
+void nuppn_ctx_destroy(nuppn_ctx *c) {
+    if (!c) return;
+    (void) hipDeviceSynchronize();
+    for (void *p : {(void *) c->d_k, (void *) c->d_rtype, (void *) c->d_red2sp, (void *) c->d_dx_ptr,
+                    (void *) c->d_dx_ent, (void *) c->d_jq_ptr, (void *) c->d_jq_ent,
+                    (void *) c->d_Acp, (void *) c->d_Ari, (void *) c->d_perm_c,
+                    (void *) c->d_iperm_c, (void *) c->d_isdiag, (void *) c->d_coef,
+                    (void *) c->d_aw, (void *) c->d_zw, (void *) c->d_slots, (void *) c->d_flag,
+                    (void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
+                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state,
+                    (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
+                    (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
+                    (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
+                    (void *) c->d_status})
+        dev_free(p);
+    for (auto &e : c->ev)
+        if (e) (void) hipEventDestroy(e);
+    if (c->h_flag) (void) hipHostFree(c->h_flag);
+    delete c;
+}
+
+int nuppn_set_column_order(nuppn_ctx *c, const int *perm_c) {
+    if (!c || !perm_c) return NUPPN_ERR_ARG;
+    const int n = c->sym.n;
+    std::vector<char> seen(n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (perm_c[i] < 0 || perm_c[i] >= n || seen[perm_c[i]]) return NUPPN_ERR_ARG;
+        seen[perm_c[i]] = 1;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    c->sym.perm_c.assign(perm_c, perm_c + n);
+    return upload_order(c);
+}
+
+int nuppn_get_column_order(nuppn_ctx *c, int *perm_c) {
+    if (!c || !perm_c) return NUPPN_ERR_ARG;
+    std::copy(c->sym.perm_c.begin(), c->sym.perm_c.end(), perm_c);
+    return NUPPN_OK;
+}
+
+int nuppn_reduced_size(nuppn_ctx *c, int *n, int *nnz) {
+    if (!c || !n || !nnz) return NUPPN_ERR_ARG;
+    *n = c->sym.n;
+    *nnz = c->sym.nnzA;
+    return NUPPN_OK;
+}
+
+int nuppn_reduced_pattern(nuppn_ctx *c, int *colptr, int *rowind, int *red2sp) {
+    if (!c) return NUPPN_ERR_ARG;
+    if (colptr) std::copy(c->sym.A_colptr.begin(), c->sym.A_colptr.end(), colptr);
+    if (rowind) std::copy(c->sym.A_rowind.begin(), c->sym.A_rowind.end(), rowind);
+    if (red2sp) std::copy(c->sym.red2sp.begin(), c->sym.red2sp.end(), red2sp);
+    return NUPPN_OK;
+}
+
+int nuppn_rates_eval(nuppn_ctx *c, int nz, const double *t9, const double *rho, const double *ye,
+                     double *v) {
+    (void) ye;   // rho*ye scaling applies to '  ec' rates, which the network format excludes
+    if (!c || nz <= 0 || !t9 || !rho || !v) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (ensure_zone_buffers(c, nz) || ensure_rate_buffer(c, nz)) return NUPPN_ERR_HIP;
+    HIP_OK(hipMemcpy(c->d_t9_0, t9, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_0, rho, sizeof(double) * nz, hipMemcpyHostToDevice));
+    const int nre = c->net.nre;
+    dim3 grid((nre + 255) / 256, nz);
+    hipLaunchKernelGGL(rates_kernel, grid, dim3(256), 0, 0, c->d_coef, c->d_rtype, nre, nz,
+                       c->d_t9_0, c->d_t9_0, c->d_rho_0, c->d_rho_0, c->d_v);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(v, c->d_v, sizeof(double) * nre * (size_t) nz, hipMemcpyDeviceToHost));
+    return NUPPN_OK;
+}
+
+static int assemble_call(nuppn_ctx *c, int mode, const double *x, const double *v, double dt,
+                         double *out, size_t out_count) {
+    std::lock_guard<std::mutex> g(c->mu);
+    if (ensure_slots(c, 1)) return NUPPN_ERR_HIP;
+    double *d_x = nullptr, *d_v = nullptr, *d_out = nullptr;
+    if (dev_upload(&d_x, x, c->net.nsp) || dev_upload(&d_v, v, c->net.nre) ||
+        dev_upload(&d_out, (const double *) nullptr, out_count))
+        return NUPPN_ERR_HIP;
+    KArgs a = kargs(c);
+    hipLaunchKernelGGL(assemble_kernel, dim3(1), dim3(64), 0, 0, a, mode, d_x, d_v, dt, d_out,
+                       c->d_status);
+    int st = 0;
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(out, d_out, sizeof(double) * out_count, hipMemcpyDeviceToHost));
+    if (mode == 1) HIP_OK(hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    dev_free(d_x);
+    dev_free(d_v);
+    dev_free(d_out);
+    return st;
+}
+
+int nuppn_dxdt(nuppn_ctx *c, const double *x, const double *v, double *dxdt) {
+    if (!c || !x || !v || !dxdt) return NUPPN_ERR_ARG;
+    return assemble_call(c, 0, x, v, 0.0, dxdt, c->net.nsp);
+}
+
+int nuppn_jacobian_reduced(nuppn_ctx *c, double dt, const double *x, const double *v,
+                           double *csc_val) {
+    if (!c || !x || !v || !csc_val) return NUPPN_ERR_ARG;
+    return assemble_call(c, 1, x, v, dt, csc_val, c->sym.nnzA);
+}
+
+int nuppn_lu_solve_reduced(nuppn_ctx *c, const double *csc_val, int nrhs, double *b, int *perm_r,
+                           int *nnz_lu) {
+    if (!c || !csc_val || nrhs < 0 || (nrhs > 0 && !b)) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (ensure_slots(c, 1)) return NUPPN_ERR_HIP;
+    const int n = c->sym.n;
+    double *d_val = nullptr, *d_b = nullptr;
+    int *d_pr = nullptr, *d_lp = nullptr;
+    if (dev_upload(&d_val, csc_val, c->sym.nnzA) ||
+        dev_upload(&d_b, nrhs ? b : (const double *) nullptr, (size_t) n * std::max(nrhs, 1)) ||
+        dev_upload(&d_pr, (const int *) nullptr, n) ||
+        dev_upload(&d_lp, (const int *) nullptr, n + 1))
+        return NUPPN_ERR_HIP;
+    const Slot dummy{};
+    (void) dummy;
+    LUStore st;
+    char *base = c->d_slots;
+    st.Li = (int *) (base + c->layout.off[S_LI]);
+    st.Lx = (double *) (base + c->layout.off[S_LX]);
+    st.Ui = (int *) (base + c->layout.off[S_UI]);
+    st.Ux = (double *) (base + c->layout.off[S_UX]);
+    st.Up = (int *) (base + c->layout.off[S_UP]);
+    hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), c->lds, 0, n, c->d_Acp, c->d_Ari, d_val,
+                       c->d_perm_c, c->d_iperm_c, nrhs, d_b, st, d_pr, d_lp, c->d_status, 1);
+    HIP_OK(hipGetLastError());
+    int info = 0;
+    HIP_OK(hipMemcpy(&info, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (nrhs) HIP_OK(hipMemcpy(b, d_b, sizeof(double) * n * (size_t) nrhs, hipMemcpyDeviceToHost));
+    if (perm_r) HIP_OK(hipMemcpy(perm_r, d_pr, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (nnz_lu) {
+        int lpn = 0, upn = 0;
+        HIP_OK(hipMemcpy(&lpn, d_lp + n, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&upn, st.Up + n, sizeof(int), hipMemcpyDeviceToHost));
+        nnz_lu[0] = lpn;
+        nnz_lu[1] = upn;
+    }
+    dev_free(d_val);
+    dev_free(d_b);
+    dev_free(d_pr);
+    dev_free(d_lp);
+    return info ? NUPPN_ERR_SINGULAR : NUPPN_OK;
+}
+
+int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                              const double *t9_1, const double *rho_0, const double *rho_1,
+                              double *ye, const double *dtwant, int *ierr,
+                              nuppn_zone_stats *stats, void *stream) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    hipStream_t s = (hipStream_t) stream;
+    if (ensure_slots(c, nz) || ensure_rate_buffer(c, nz)) return NUPPN_ERR_HIP;
+    if (ensure_zone_state(c, nz)) return NUPPN_ERR_HIP;
+    const int nre = c->net.nre;
+    HIP_OK(hipEventRecord(c->ev[0], s));
+    dim3 grid((nre + 255) / 256, nz);
+    hipLaunchKernelGGL(rates_kernel, grid, dim3(256), 0, s, c->d_coef, c->d_rtype, nre, nz, t9_0,
+                       t9_1, rho_0, rho_1, c->d_v);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(c->ev[1], s));
+    KArgs a = kargs(c);
+    ZoneData &Z = a.Z;
+    Z.yps = yps;
+    Z.yps_old = c->d_yps_old;
+    Z.y_old = c->d_y_old;
+    Z.y_cur = c->d_y_cur;
+    Z.y_new = c->d_y_new;
+    Z.v = c->d_v;
+    Z.work = c->d_work;
+    Z.nsp_stride = (size_t) c->net.nsp;
+    Z.nre_stride = (size_t) nre;
+    Z.work_stride = c->work_stride;
+    Z.t9_0 = t9_0;
+    Z.t9_1 = t9_1;
+    Z.rho_0 = rho_0;
+    Z.rho_1 = rho_1;
+    Z.dtwant = dtwant;
+    Z.ye = ye;
+    Z.ierr = ierr;
+    Z.stats = (ZoneStats *) stats;
+    Z.state = c->d_state;
+    Z.flag = c->d_flag;
+    Z.nz = nz;
+    const int grid_n = std::min(nz, c->nslots);
+    HIP_OK(hipEventRecord(c->ev[2], s));
+    hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), 0, s, a);
+    HIP_OK(hipGetLastError());
+    // each pass advances every zone by at least one Newton iteration or finishes it
+    const long long max_pass = (long long) c->kn.max_sub_steps * (c->kn.ittd + 2) + 16;
+    long long pass = 0;
+    for (;; ++pass) {
+        hipLaunchKernelGGL(flag_reset_kernel, dim3(1), dim3(1), 0, s, c->d_flag);
+        hipLaunchKernelGGL(zone_control_kernel, dim3(nz), dim3(64), 0, s, a);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (c->h_flag[0] == 0) break;
+        if (pass >= max_pass) return NUPPN_ERR_HIP;
+        hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
+        HIP_OK(hipGetLastError());
+    }
+    c->last_passes = pass;
+    HIP_OK(hipEventRecord(c->ev[3], s));
+    return NUPPN_OK;
+}
+
+int nuppn_last_timing(nuppn_ctx *c, float *ms2) {
+    if (!c || !ms2) return NUPPN_ERR_ARG;
+    HIP_OK(hipEventSynchronize(c->ev[3]));
+    HIP_OK(hipEventElapsedTime(&ms2[0], c->ev[0], c->ev[1]));
+    HIP_OK(hipEventElapsedTime(&ms2[1], c->ev[2], c->ev[3]));
+    return NUPPN_OK;
+}
+
+int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                          const double *t9_1, const double *rho_0, const double *rho_1, double *ye,
+                          const double *dtwant, int *ierr, nuppn_zone_stats *stats) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
+    }
+    const size_t nsp = c->net.nsp;
+    HIP_OK(hipMemcpy(c->d_yps, yps, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_0, t9_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_1, t9_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_0, rho_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_ye, ye, sizeof(double) * nz, hipMemcpyHostToDevice));
+    int rc = nuppn_integrate_batch_dev(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
+                                       c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr,
+                                       (nuppn_zone_stats *) c->d_stats, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(yps, c->d_yps, sizeof(double) * nsp * nz, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ye, c->d_ye, sizeof(double) * nz, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ierr, c->d_ierr, sizeof(int) * nz, hipMemcpyDeviceToHost));
+    if (stats)
+        HIP_OK(hipMemcpy(stats, c->d_stats, sizeof(ZoneStats) * nz, hipMemcpyDeviceToHost));
+    return NUPPN_OK;
+}
+
+int nuppn_integrate_zone(nuppn_ctx *c, double *yps, double t9_0, double t9_1, double rho_0,
+                         double rho_1, double *ye, double dtwant, int *ierr,
+                         nuppn_zone_stats *stats) {
+    if (!c || !yps || !ye || !ierr) return NUPPN_ERR_ARG;
+    return nuppn_integrate_batch(c, 1, yps, &t9_0, &t9_1, &rho_0, &rho_1, ye, &dtwant, ierr,
+                                 stats);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// SuperLU link-level drop-ins: the column order is SuperLU's (COLAMD + etree post-order), the
+// numeric factorisation and the solves run on the device.
+
+namespace {
+
+struct DropinFactors {
+    int n = 0;
+    int *d_Acp = nullptr, *d_Ari = nullptr, *d_perm_c = nullptr, *d_iperm_c = nullptr;
+    int *d_perm_r = nullptr, *d_Lp = nullptr, *d_info = nullptr;
+    double *d_val = nullptr, *d_b = nullptr;
+    LUStore st{};
+    size_t lds = 0;
+    int nb_cap = 0;
+};
+
+void dropin_free(DropinFactors *f) {
+    if (!f) return;
+    for (void *p : {(void *) f->d_Acp, (void *) f->d_Ari, (void *) f->d_perm_c,
+                    (void *) f->d_iperm_c, (void *) f->d_perm_r, (void *) f->d_Lp,
+                    (void *) f->d_info, (void *) f->d_val, (void *) f->d_b, (void *) f->st.Li,
+                    (void *) f->st.Lx, (void *) f->st.Ui, (void *) f->st.Ux, (void *) f->st.Up})
+        dev_free(p);
+    delete f;
+}
+
+// factor a 0-based CSC matrix (rows sorted or not; COLAMD handles both); returns SuperLU info,
+// or a negative library error
+int dropin_factor(int n, const int *colptr, const int *rowind, const double *val,
+                  DropinFactors **out) {
+    std::vector<int> perm_c;
+    if (superlu_column_order(n, colptr, rowind, perm_c) != 0) return NUPPN_ERR_ARG;
+    std::vector<int> iperm(n);
+    for (int i = 0; i < n; ++i) iperm[perm_c[i]] = i;
+    auto *f = new DropinFactors;
+    f->n = n;
+    const int nnz = colptr[n];
+    const size_t capL = (size_t) n * (n - 1) / 2 + 1, capU = (size_t) n * (n + 1) / 2 + 1;
+    if (dev_upload(&f->d_Acp, colptr, n + 1) || dev_upload(&f->d_Ari, rowind, nnz) ||
+        dev_upload(&f->d_val, val, nnz) || dev_upload(&f->d_perm_c, perm_c) ||
+        dev_upload(&f->d_iperm_c, iperm) || dev_upload(&f->d_perm_r, (const int *) nullptr, n) ||
+        dev_upload(&f->d_Lp, (const int *) nullptr, n + 1) ||
+        dev_upload(&f->d_info, (const int *) nullptr, 1) ||
+        dev_upload(&f->st.Li, (const int *) nullptr, capL) ||
+        dev_upload(&f->st.Lx, (const double *) nullptr, capL) ||
+        dev_upload(&f->st.Ui, (const int *) nullptr, capU) ||
+        dev_upload(&f->st.Ux, (const double *) nullptr, capU) ||
+        dev_upload(&f->st.Up, (const int *) nullptr, n + 1)) {
+        dropin_free(f);
+        return NUPPN_ERR_HIP;
+    }
+    f->lds = lds_bytes(n);
+    if (f->lds > 64 * 1024)
+        (void) hipFuncSetAttribute((const void *) lu_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int) f->lds);
+    hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), f->lds, 0, n, f->d_Acp, f->d_Ari, f->d_val,
+                       f->d_perm_c, f->d_iperm_c, 0, (double *) nullptr, f->st, f->d_perm_r,
+                       f->d_Lp, f->d_info, 1);
+    int info = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(&info, f->d_info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        dropin_free(f);
+        return NUPPN_ERR_HIP;
+    }
+    *out = f;
+    return info;
+}
+
+int dropin_solve(DropinFactors *f, int nrhs, double *b, int ldb) {
+    const int n = f->n;
+    if (nrhs <= 0) return 0;
+    if (nrhs > f->nb_cap) {
+        dev_free(f->d_b);
+        f->d_b = nullptr;
+        if (dev_upload(&f->d_b, (const double *) nullptr, (size_t) n * nrhs)) return NUPPN_ERR_HIP;
+        f->nb_cap = nrhs;
+    }
+    for (int r = 0; r < nrhs; ++r)
+        if (hipMemcpy(f->d_b + (size_t) r * n, b + (size_t) r * ldb, sizeof(double) * n,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return NUPPN_ERR_HIP;
+    hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), f->lds, 0, n, f->d_Acp, f->d_Ari, f->d_val,
+                       f->d_perm_c, f->d_iperm_c, nrhs, f->d_b, f->st, f->d_perm_r, f->d_Lp,
+                       f->d_info, 0);
+    if (hipGetLastError() != hipSuccess) return NUPPN_ERR_HIP;
+    for (int r = 0; r < nrhs; ++r)
+        if (hipMemcpy(b + (size_t) r * ldb, f->d_b + (size_t) r * n, sizeof(double) * n,
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return NUPPN_ERR_HIP;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int colind[],
+               int rowptr[], double rhs[], int *ierr) {
+    if (!ierr) return;
+    if (n <= 0 || nnz < 0 || !val || !colind || !rowptr || !rhs) {
+        *ierr = -1;
+        return;
+    }
+    if (shiftindex) {   // in place, as slu_f90.c:11-15 (the Fortran caller restores them)
+        for (int i = 0; i <= n; ++i) rowptr[i] -= 1;
+        for (int i = 0; i < nnz; ++i) colind[i] -= 1;
+    }
+    std::vector<int> cp(n + 1, 0), ri(nnz);
+    std::vector<double> cv(nnz);
+    if (isrow) {   // dCompRow_to_CompCol (dutil.c:154): rows ascending within each column
+        for (int p = 0; p < nnz; ++p) cp[colind[p] + 1]++;
+        for (int i = 0; i < n; ++i) cp[i + 1] += cp[i];
+        std::vector<int> pos(cp.begin(), cp.end() - 1);
+        for (int r = 0; r < n; ++r)
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+                const int q = pos[colind[p]]++;
+                ri[q] = r;
+                cv[q] = val[p];
+            }
+    } else {
+        std::copy(rowptr, rowptr + n + 1, cp.begin());
+        std::copy(colind, colind + nnz, ri.begin());
+        std::copy(val, val + nnz, cv.begin());
+    }
+    DropinFactors *f = nullptr;
+    const int info = dropin_factor(n, cp.data(), ri.data(), cv.data(), &f);
+    if (info != 0) {
+        *ierr = info;
+        if (f) dropin_free(f);
+        return;
+    }
+    *ierr = dropin_solve(f, 1, rhs, n);
+    dropin_free(f);
+}
+
+void c_fortran_dgssv_(int *iopt, int *n, int *nnz, int *nrhs, double *values, int *rowind,
+                      int *colptr, double *b, int *ldb, long long *f_factors, int *info) {
+    if (!iopt || !info) return;
+    if (*iopt == 1) {   // factor (1-based CSC copied to 0-based, c_fortran_dgssv.c:90-93)
+        const int nn = *n, nz = *nnz;
+        std::vector<int> cp(nn + 1), ri(nz);
+        for (int i = 0; i <= nn; ++i) cp[i] = colptr[i] - 1;
+        for (int i = 0; i < nz; ++i) ri[i] = rowind[i] - 1;
+        DropinFactors *f = nullptr;
+        *info = dropin_factor(nn, cp.data(), ri.data(), values, &f);
+        *f_factors = (long long) (uintptr_t) f;
+    } else if (*iopt == 2) {
+        DropinFactors *f = (DropinFactors *) (uintptr_t) *f_factors;
+        *info = f ? dropin_solve(f, *nrhs, b, *ldb) : -1;
+    } else if (*iopt == 3) {
+        dropin_free((DropinFactors *) (uintptr_t) *f_factors);
+        *f_factors = 0;
+        *info = 0;
+    } else {
+        // the reference prints and exits(-1) (c_fortran_dgssv.c:186-188); errors stay codes here
+        *info = -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+// Host-side symbolic phase (see symbolic.hpp).
+#include "symbolic.hpp"
+
+#include <algorithm>
+#include <cstdint>
+#include <tuple>
+
+namespace nuppn {
+
+namespace {
+
+inline int find_in_row(const std::vector<int> &rowptr, const std::vector<int> &colind, int r,
+                       int c) {
+    auto b = colind.begin() + rowptr[r], e = colind.begin() + rowptr[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (it != e && *it == c) ? int(it - colind.begin()) : -1;
+}
+
+}  // namespace
+
+int build_symbolic(const HostNetwork &net, Symbolic &s) {
+    const int nsp = net.nsp, nre = net.nre, g = net.igamma;
+    if (nsp <= 1 || nre <= 0 || g < 0 || g >= nsp) return -1;
+    for (int m : {0, 2, 4, 6})
+        for (int l = 0; l < nre; ++l)
+            if (net.k[m * nre + l] < 0 || net.k[m * nre + l] >= nsp) return -1;
+
+    // --- makemap: 8 positions per reaction + diagonal of each considered isotope, numbered
+    //     row-major (jac_rhs.F90:481-511), slot lookup by binary search (getelm)
+    std::vector<int64_t> keys;
+    keys.reserve(8 * (size_t) nre + nsp);
+    auto pos_rc = [&](int l, int m, int &r, int &c) {
+        const int in1 = net.k[l], in2 = net.k[2 * nre + l];
+        const int out1 = net.k[4 * nre + l], out2 = net.k[6 * nre + l];
+        // dns2csr_map order: in1in1, in2in1, out1in1, out2in1, in1in2, in2in2, out1in2, out2in2
+        const int rr[8] = {in1, in2, out1, out2, in1, in2, out1, out2};
+        const int cc[8] = {in1, in1, in1, in1, in2, in2, in2, in2};
+        r = rr[m];
+        c = cc[m];
+    };
+    for (int l = 0; l < nre; ++l)
+        for (int m = 0; m < 8; ++m) {
+            int r, c;
+            pos_rc(l, m, r, c);
+            keys.push_back((int64_t) r * nsp + c);
+        }
+    for (int i = 0; i < nsp; ++i)
+        if (i != g) keys.push_back((int64_t) i * nsp + i);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    s.nnz_full = (int) keys.size();
+    s.full_rowptr.assign(nsp + 1, 0);
+    s.full_colind.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {
+        s.full_rowptr[keys[i] / nsp + 1]++;
+        s.full_colind[i] = (int) (keys[i] % nsp);
+    }
+    for (int i = 0; i < nsp; ++i) s.full_rowptr[i + 1] += s.full_rowptr[i];
+    s.map.resize(8 * (size_t) nre);
+    for (int l = 0; l < nre; ++l)
+        for (int m = 0; m < 8; ++m) {
+            int r, c;
+            pos_rc(l, m, r, c);
+            s.map[m * nre + l] = find_in_row(s.full_rowptr, s.full_colind, r, c);
+        }
+
+    // --- static mask: every species of every reaction (cyminformal < 0 marks all of them,
+    //     jac_rhs.F90:350-353), gamma cleared (line 442)
+    std::vector<unsigned char> mask(nsp, 0);
+    for (int l = 0; l < nre; ++l)
+        for (int m : {0, 2, 4, 6}) mask[net.k[m * nre + l]] = 1;
+    mask[g] = 0;
+    s.sp2red.assign(nsp, -1);
+    s.red2sp.clear();
+    for (int i = 0; i < nsp; ++i)
+        if (mask[i]) {
+            s.sp2red[i] = (int) s.red2sp.size();
+            s.red2sp.push_back(i);
+        }
+    s.n = (int) s.red2sp.size();
+
+    // --- reduced entries with their contributions.  Statement order m of calculate_jacobian
+    //     (jac_rhs.F90:365-392): ice1 (in1,in1), ice2 (in2,in1), ice3 (in1,in2), ice4
+    //     (out1,in1), ice5 (out2,in1), ice6 (in2,in2), ice7 (out1,in2), ice8 (out2,in2).
+    struct Contrib { int col, row, l, m; };
+    std::vector<Contrib> con;
+    con.reserve(8 * (size_t) nre);
+    for (int l = 0; l < nre; ++l) {
+        const int in1 = net.k[l], in2 = net.k[2 * nre + l];
+        const int out1 = net.k[4 * nre + l], out2 = net.k[6 * nre + l];
+        const int rr[8] = {in1, in2, in1, out1, out2, in2, out1, out2};
+        const int cc[8] = {in1, in1, in2, in1, in1, in2, in2, in2};
+        const bool ok[8] = {in1 != g,
+                            in1 != g && in2 != g,
+                            in1 != g && in2 != g,
+                            in1 != g && out1 != g,
+                            in1 != g && out2 != g,
+                            in2 != g,
+                            in2 != g && out1 != g,
+                            in2 != g && out2 != g};
+        for (int m = 0; m < 8; ++m)
+            if (ok[m]) con.push_back({s.sp2red[cc[m]], s.sp2red[rr[m]], l, m});
+    }
+    for (int r = 0; r < s.n; ++r) con.push_back({r, r, -1, -1});   // diagonal of masked species
+    std::sort(con.begin(), con.end(), [](const Contrib &a, const Contrib &b) {
+        return std::tie(a.col, a.row, a.l, a.m) < std::tie(b.col, b.row, b.l, b.m);
+    });
+    s.A_colptr.assign(s.n + 1, 0);
+    s.A_rowind.clear();
+    s.A_isdiag.clear();
+    s.jq_ptr.assign(1, 0);
+    s.jq_ent.clear();
+    for (size_t i = 0; i < con.size();) {
+        size_t j = i;
+        const int col = con[i].col, row = con[i].row;
+        while (j < con.size() && con[j].col == col && con[j].row == row) {
+            if (con[j].l >= 0) s.jq_ent.push_back((con[j].l << 3) | con[j].m);
+            ++j;
+        }
+        s.A_rowind.push_back(row);
+        s.A_isdiag.push_back(row == col ? 1 : 0);
+        s.A_colptr[col + 1]++;
+        s.jq_ptr.push_back((int) s.jq_ent.size());
+        i = j;
+    }
+    for (int c = 0; c < s.n; ++c) s.A_colptr[c + 1] += s.A_colptr[c];
+    s.nnzA = (int) s.A_rowind.size();
+
+    // --- dX/dt pull map: contributions of reaction l to species k1, k3, k5, k7 in that order
+    //     (jac_rhs.F90:252-255), reactions ascending
+    std::vector<std::vector<int>> dx(s.n);
+    for (int l = 0; l < nre; ++l) {
+        const int sp[4] = {net.k[l], net.k[2 * nre + l], net.k[4 * nre + l], net.k[6 * nre + l]};
+        for (int role = 0; role < 4; ++role)
+            if (sp[role] != g && mask[sp[role]]) dx[s.sp2red[sp[role]]].push_back((l << 2) | role);
+    }
+    s.dx_ptr.assign(1, 0);
+    s.dx_ent.clear();
+    for (int r = 0; r < s.n; ++r) {
+        s.dx_ent.insert(s.dx_ent.end(), dx[r].begin(), dx[r].end());
+        s.dx_ptr.push_back((int) s.dx_ent.size());
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Column elimination tree (Liu's algorithm on the row cliques of A'A replaced by stars at each
+// row's first column; disjoint sets with path halving) - SuperLU_5.0/SRC/sp_coletree.c
+
+namespace {
+inline int dset_find(std::vector<int> &pp, int i) {
+    int p = pp[i], gp = pp[p];
+    while (gp != p) {
+        pp[i] = gp;
+        i = gp;
+        p = pp[i];
+        gp = pp[p];
+    }
+    return p;
+}
+}  // namespace
+
+void column_etree(int nr, int nc, const int *colbeg, const int *colend, const int *rowind,
+                  std::vector<int> &parent) {
+    std::vector<int> root(nc), pp(nc), firstcol(nr, nc);
+    parent.assign(nc, nc);
+    for (int col = 0; col < nc; ++col)
+        for (int p = colbeg[col]; p < colend[col]; ++p)
+            firstcol[rowind[p]] = std::min(firstcol[rowind[p]], col);
+    for (int col = 0; col < nc; ++col) {
+        pp[col] = col;
+        int cset = col;
+        root[cset] = col;
+        parent[col] = nc;
+        for (int p = colbeg[col]; p < colend[col]; ++p) {
+            const int row = firstcol[rowind[p]];
+            if (row >= col) continue;
+            const int rset = dset_find(pp, row);
+            const int rroot = root[rset];
+            if (rroot != col) {
+                parent[rroot] = col;
+                pp[cset] = rset;
+                cset = rset;
+                root[cset] = col;
+            }
+        }
+    }
+}
+
+// post-order numbering of a forest given by parent[] (root parent = n), children visited in
+// ascending order; returns post[0..n] with post[n] = n
+std::vector<int> tree_postorder(int n, const std::vector<int> &parent) {
+    std::vector<int> first_kid(n + 1, -1), next_kid(n + 1, -1), post(n + 1, 0);
+    for (int v = n - 1; v >= 0; --v) {
+        const int dad = parent[v];
+        next_kid[v] = first_kid[dad];
+        first_kid[dad] = v;
+    }
+    // iterative DFS from the dummy root n
+    std::vector<int> stack;
+    stack.reserve(n + 1);
+    int postnum = 0;
+    std::vector<int> it(n + 1);
+    for (int v = 0; v <= n; ++v) it[v] = first_kid[v];
+    stack.push_back(n);
+    while (!stack.empty()) {
+        const int v = stack.back();
+        if (it[v] != -1) {
+            const int w = it[v];
+            it[v] = next_kid[w];
+            stack.push_back(w);
+        } else {
+            post[v] = postnum++;
+            stack.pop_back();
+        }
+    }
+    return post;
+}
+
+int superlu_column_order(int n, const int *colptr, const int *rowind, std::vector<int> &perm_c,
+                         std::vector<int> *etree_out) {
+    std::vector<int> p;
+    const int rc = colamd_order(n, n, colptr, rowind, p);
+    if (rc != 0) return rc;
+    // get_colamd: perm_c[p[i]] = i
+    perm_c.assign(n, 0);
+    for (int i = 0; i < n; ++i) perm_c[p[i]] = i;
+    // sp_preorder: AC = A Pc, column etree of AC, post-order, perm_c <- post[perm_c]
+    std::vector<int> colbeg(n), colend(n);
+    for (int i = 0; i < n; ++i) {
+        colbeg[perm_c[i]] = colptr[i];
+        colend[perm_c[i]] = colptr[i + 1];
+    }
+    std::vector<int> etree;
+    column_etree(n, n, colbeg.data(), colend.data(), rowind, etree);
+    const std::vector<int> post = tree_postorder(n, etree);
+    if (etree_out) {
+        etree_out->assign(n, 0);
+        for (int i = 0; i < n; ++i) (*etree_out)[post[i]] = post[etree[i]];
+    }
+    for (int i = 0; i < n; ++i) perm_c[i] = post[perm_c[i]];
+    return 0;
+}
+
+}  // namespace nuppn

@@ -1,0 +1,386 @@
+// Per-zone driver shared by the kernels: the per-slot scratch layout, the LDS carve-up and
+// integrate_network for one zone on one wavefront (solver/source/solver.F90:53-238).
+#pragma once
+#include "device_common.hpp"
+#include "integrate.hpp"
+
+namespace nuppn {
+
+// ------------------------------------------------------------------------------------------
+// per-slot scratch layout
+
+enum SlotArr {
+    S_YOLD, S_YCUR, S_YNEW, S_YPS, S_YPSOLD, S_XN, S_DXDT, S_VX, S_ST1, S_ST2, S_VBUF, S_AVAL,
+    S_RHS, S_SORT, S_JACCPY, S_DELTA, S_QCOL, S_YERR, S_YSCAL, S_YEXT, S_LI, S_LX, S_UI, S_UX,
+    S_UP, S_COUNT
+};
+
+struct SlotLayout {
+    size_t off[S_COUNT];
+    size_t bytes;
+};
+
+inline SlotLayout slot_layout(int nsp, int nre, int n, int nnzA) {
+    const size_t capL = (size_t) n * (n - 1) / 2 + 1, capU = (size_t) n * (n + 1) / 2 + 1;
+    size_t pow2 = 1;
+    while (pow2 < (size_t) nsp) pow2 <<= 1;
+    const size_t len[S_COUNT] = {
+        8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp,
+        8ull * nre, 8ull * nre, 8ull * nre, 8ull * nre, 8ull * nnzA, 8ull * n, 8ull * pow2,
+        8ull * nnzA, 8ull * nsp, 64ull * nsp, 64ull * nsp, 8ull * nsp, 8ull * nsp,
+        4ull * capL, 8ull * capL, 4ull * capU, 8ull * capU, 4ull * (n + 1)};
+    SlotLayout L;
+    size_t o = 0;
+    for (int i = 0; i < S_COUNT; ++i) {
+        L.off[i] = o;
+        o += (len[i] + 255) & ~(size_t) 255;
+    }
+    L.bytes = o;
+    return L;
+}
+
+// Per-zone state of the sub-step controller and the backward-Euler Newton loop, kept in
+// global memory between launches (solver.F90:65-107 locals, backward_euler.F90:31-53 locals).
+enum ZonePhase { PH_SUBSTEP = 0, PH_NEWTON = 1, PH_AFTER_NEWTON = 2, PH_DONE = 3 };
+
+struct ZoneState {
+    int phase, i, iter, ierr, lib_err, solved, pad0, pad1;
+    double dtdone, dttry, dtrcmd, my_t9_last, my_rho_last, grd, ye_entry, pad2;
+    ZoneStats st;
+};
+
+// per-zone arrays (zone-major, fixed strides) and the I/O of one batch
+struct ZoneData {
+    double *yps, *yps_old, *y_old, *y_cur, *y_new, *v, *work;
+    size_t nsp_stride, nre_stride, work_stride;
+    const double *t9_0, *t9_1, *rho_0, *rho_1, *dtwant;
+    double *ye;
+    int *ierr;
+    ZoneStats *stats;   // may be null
+    ZoneState *state;
+    int *flag;          // [0] set when some zone needs a Newton iteration
+    int nz;
+};
+
+struct KArgs {
+    DevNet N;
+    DevKnobs K;
+    ZoneData Z;
+    char *slot_base;
+    size_t slot_bytes;
+    size_t off[S_COUNT];
+};
+
+__device__ inline Slot make_slot(char *base, const size_t *off) {
+    Slot S;
+    S.y_old = (double *) (base + off[S_YOLD]);
+    S.y_cur = (double *) (base + off[S_YCUR]);
+    S.y_new = (double *) (base + off[S_YNEW]);
+    S.yps = (double *) (base + off[S_YPS]);
+    S.yps_old = (double *) (base + off[S_YPSOLD]);
+    S.xn = (double *) (base + off[S_XN]);
+    S.dxdt = (double *) (base + off[S_DXDT]);
+    S.vx = (double *) (base + off[S_VX]);
+    S.st1 = (double *) (base + off[S_ST1]);
+    S.st2 = (double *) (base + off[S_ST2]);
+    S.vbuf = (double *) (base + off[S_VBUF]);
+    S.Aval = (double *) (base + off[S_AVAL]);
+    S.rhs = (double *) (base + off[S_RHS]);
+    S.sortbuf = (double *) (base + off[S_SORT]);
+    S.jac_cpy = (double *) (base + off[S_JACCPY]);
+    S.delta = (double *) (base + off[S_DELTA]);
+    S.qcol = (double *) (base + off[S_QCOL]);
+    S.yerr = (double *) (base + off[S_YERR]);
+    S.yscal = (double *) (base + off[S_YSCAL]);
+    S.yext = (double *) (base + off[S_YEXT]);
+    S.lu.Li = (int *) (base + off[S_LI]);
+    S.lu.Lx = (double *) (base + off[S_LX]);
+    S.lu.Ui = (int *) (base + off[S_UI]);
+    S.lu.Ux = (double *) (base + off[S_UX]);
+    S.lu.Up = (int *) (base + off[S_UP]);
+    return S;
+}
+
+// LDS: x[n] doubles, perm_r / pivrow / touched [n] ints, Lp [n+1] ints, flag [n] bytes
+inline size_t lds_bytes(int n) {
+    size_t b = 8ull * n + 4ull * n * 3 + 4ull * (n + 1);
+    b = (b + 15) & ~(size_t) 15;
+    b += (size_t) n;
+    return (b + 255) & ~(size_t) 255;
+}
+
+__device__ inline LUWork make_work(char *smem, int n) {
+    LUWork w;
+    char *p = smem;
+    w.x = (double *) p;
+    p += 8ull * n;
+    w.perm_r = (int *) p;
+    p += 4ull * n;
+    w.pivrow = (int *) p;
+    p += 4ull * n;
+    w.touched = (int *) p;
+    p += 4ull * n;
+    w.Lp = (int *) p;
+    p += 4ull * (n + 1);
+    p = (char *) (((uintptr_t) p + 15) & ~(uintptr_t) 15);
+    w.flag = (unsigned char *) p;
+    return w;
+}
+
+// linear interpolant over [0, dtwant] (solver.F90:84-91, interp.F90:29-50)
+__device__ inline double interp2(double x1, double y0, double y1, double xw) {
+    double x = fmax(xw, 0.0);
+    x = fmin(x, x1);
+    return y0 + (y1 - y0) * (x - 0.0) / (x1 - 0.0);
+}
+
+
+// -------------------------------------------------------------------------------------------
+// Sub-step controller (solver.F90:53-238) and the backward-Euler decisions
+// (backward_euler.F90:29-109) as a per-zone state machine.  One wavefront per zone.
+
+__device__ inline double *zrow(double *base, size_t stride, int z) { return base + (size_t) z * stride; }
+
+// set-up of a zone at the start of a batch: yps_old = yps, entry Ye, phase SUBSTEP
+__device__ inline void zone_init(const KArgs &a, int z) {
+    const DevNet &N = a.N;
+    const ZoneData &Z = a.Z;
+    double *yps = zrow(Z.yps, Z.nsp_stride, z);
+    copy_vec(zrow(Z.yps_old, Z.nsp_stride, z), yps, N.nsp);
+    double *work = zrow(Z.work, Z.work_stride, z);
+    const double ye = calculate_ye(N, yps, work, work + Z.nsp_stride);
+    if (lane_id() == 0) {
+        ZoneState s = {};
+        s.phase = PH_SUBSTEP;
+        s.i = 0;
+        const double t9_0 = Z.t9_0[z], t9_1 = Z.t9_1[z], rho_0 = Z.rho_0[z], rho_1 = Z.rho_1[z];
+        s.dttry = Z.dtwant[z];
+        s.my_t9_last = 0.5 * (t9_0 + t9_1);
+        s.my_rho_last = 0.5 * (rho_0 + rho_1);
+        s.ye_entry = ye;
+        Z.state[z] = s;
+    }
+    wave_sync();
+}
+
+// final bookkeeping of a zone (solver.F90:229-237): clip, Ye, outputs
+__device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr, bool loop_done) {
+    const DevNet &N = a.N;
+    const ZoneData &Z = a.Z;
+    double *yps = zrow(Z.yps, Z.nsp_stride, z);
+    if (loop_done) {
+        for (int j = lane_id(); j < N.nsp; j += WAVE) yps[j] = fmax(yps[j], 1.0e-99);
+        wave_sync();
+    }
+    double ye = s.ye_entry;
+    if (s.solved) {
+        double *work = zrow(Z.work, Z.work_stride, z);
+        ye = calculate_ye(N, yps, work, work + Z.nsp_stride);
+    }
+    s.phase = PH_DONE;
+    s.ierr = ierr;
+    s.st.nvar1 = N.n;
+    if (lane_id() == 0) {
+        Z.ye[z] = ye;
+        Z.ierr[z] = s.lib_err ? s.lib_err : ierr;
+        if (Z.stats) Z.stats[z] = s.st;
+    }
+}
+
+// one pass of the controller for zone z: consume the result of a Newton iteration if there is
+// one, then advance sub-steps until the zone needs another Newton iteration or is done.
+__device__ inline void zone_control(const KArgs &a, int z) {
+    const DevNet &N = a.N;
+    const DevKnobs &K = a.K;
+    const ZoneData &Z = a.Z;
+    const int nsp = N.nsp;
+    ZoneState s = Z.state[z];
+    if (s.phase == PH_DONE || s.phase == PH_NEWTON) return;
+    double *yps = zrow(Z.yps, Z.nsp_stride, z);
+    double *yps_old = zrow(Z.yps_old, Z.nsp_stride, z);
+    double *y_old = zrow(Z.y_old, Z.nsp_stride, z);
+    double *y_cur = zrow(Z.y_cur, Z.nsp_stride, z);
+    double *y_new = zrow(Z.y_new, Z.nsp_stride, z);
+    double *v = zrow(Z.v, Z.nre_stride, z);
+    const double t9_0 = Z.t9_0[z], t9_1 = Z.t9_1[z], rho_0 = Z.rho_0[z], rho_1 = Z.rho_1[z];
+    const double dtwant = Z.dtwant[z];
+    const double rmin = 0.2, rmax = 2.0, dgrd = K.dgrd;
+    const double sum_check = K.grdthreshold * (nsp - 1) * dgrd;
+    int ierr = 0;
+    bool step_ended = false;   // a backward-Euler step just returned (ierr set)
+
+    if (s.phase == PH_AFTER_NEWTON) {
+        if (s.lib_err) {   // the reference stops here (zero in the matrix, singular factor)
+            zone_finish(a, z, s, 3, false);
+            if (lane_id() == 0) Z.state[z] = s;
+            return;
+        }
+        s.iter += 1;
+        s.st.titers++;
+        s.st.iter++;
+        s.st.n_newton++;
+        const double mneg = wave_maxneg(y_new, nsp);
+        const double sm = seq_sum(y_new, nsp);
+        if (s.grd <= dgrd && mneg < sum_check && fabs(1.0 - sm) <= SUM_TOL) {
+            copy_vec(yps, y_new, nsp);
+            s.dtrcmd = 2.0 * fmax(fmin(sqrt(dgrd / fmax(s.grd, SMALL_C)), rmax), rmin) * s.dttry;
+            ypsplus(yps, nsp);
+            ierr = 0;
+            step_ended = true;
+        } else {
+            copy_vec(y_cur, y_new, nsp);
+            if (s.iter > K.ittd) {
+                double err, tol;
+                if (mneg >= sum_check) { err = mneg; tol = sum_check; ierr = 4; }
+                else if (fabs(1.0 - sm) > SUM_TOL) { err = fabs(1.0 - sm); tol = SUM_TOL; ierr = 6; }
+                else { err = s.grd; tol = dgrd; ierr = 1; }
+                s.dtrcmd = fmax(fmin(sqrt(tol / fmax(err, SMALL_C)), rmax), rmin) * s.dttry;
+                step_ended = true;
+            } else {
+                s.phase = PH_NEWTON;
+                if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+                return;
+            }
+        }
+    }
+
+    for (;;) {
+        if (step_ended) {
+            step_ended = false;
+            // back in integrate_network after do_one_time_step (solver.F90:165-225)
+            if (s.dttry < SMALL_C) { zone_finish(a, z, s, 11, false); break; }
+            if (ierr == 0) {
+                s.dtdone = s.dtdone + s.dttry;
+                for (int j = lane_id(); j < nsp; j += WAVE) yps[j] = fmax(yps[j], 1.0e-99);
+                wave_sync();
+                copy_vec(yps_old, yps, nsp);
+                s.st.nsucc++;
+                if (s.dtdone >= dtwant) { s.st.nsubt = s.i; zone_finish(a, z, s, 0, true); break; }
+            } else {
+                copy_vec(yps, yps_old, nsp);
+                s.st.nfail++;
+            }
+            s.dttry = fmin(s.dtrcmd, dtwant - s.dtdone);
+            if (s.i == K.max_sub_steps) { s.st.nsubt = s.i; zone_finish(a, z, s, 2, false); break; }
+        }
+        // next sub-step (solver.F90:108-164)
+        s.i += 1;
+        if (s.i > K.max_sub_steps) {   // loop ran out through the halving branch
+            s.st.nsubt = s.i;
+            zone_finish(a, z, s, ierr, true);
+            break;
+        }
+        double my_t9, my_rho, my_t9_0 = 0.0, my_t9_1 = 0.0, my_rho_0 = 0.0, my_rho_1 = 0.0;
+        if (K.interpolate_temp) {
+            const double t_0 = s.dtdone, t_1 = fmin(s.dtdone + s.dttry, dtwant);
+            my_t9_0 = interp2(dtwant, t9_0, t9_1, t_0);
+            my_t9_1 = interp2(dtwant, t9_0, t9_1, t_1);
+            my_t9 = 0.5 * (my_t9_0 + my_t9_1);
+        } else {
+            my_t9 = 0.5 * (t9_0 + t9_1);
+        }
+        if (K.interpolate_dens) {
+            const double t_0 = s.dtdone, t_1 = fmin(s.dtdone + s.dttry, dtwant);
+            my_rho_0 = interp2(dtwant, rho_0, rho_1, t_0);
+            my_rho_1 = interp2(dtwant, rho_0, rho_1, t_1);
+            my_rho = 0.5 * (my_rho_0 + my_rho_1);
+        } else {
+            my_rho = 0.5 * (rho_0 + rho_1);
+        }
+        double diff = fabs(my_t9_0 - my_t9_1) / my_t9_0;
+        diff = fmax(diff, fabs(my_rho_0 - my_rho_1) / my_rho_0);
+        if (diff > K.delta_state_max && t9_0 > K.min_t9_for_slv && t9_1 > K.min_t9_for_slv) {
+            s.dttry = 0.5 * s.dttry;
+            continue;
+        }
+        if (my_t9 != s.my_t9_last || my_rho != s.my_rho_last) {
+            zone_rates(N, my_t9, my_rho, v);
+            s.my_rho_last = my_rho;
+            s.my_t9_last = my_t9;
+        }
+        ierr = 0;
+        if (my_t9 >= K.min_t9_for_slv) {
+            // backward_euler_newton set-up (backward_euler.F90:49-53)
+            copy_vec(y_old, yps, nsp);
+            copy_vec(y_cur, yps, nsp);
+            copy_vec(y_new, yps, nsp);
+            s.iter = 0;
+            s.grd = 0.0;
+            s.st.titers = 0;
+            s.st.iter = 0;
+            s.solved = 1;
+            if (0 > K.ittd) {
+                const double mneg = wave_maxneg(y_new, nsp);
+                const double sm = seq_sum(y_new, nsp);
+                double err, tol;
+                if (mneg >= sum_check) { err = mneg; tol = sum_check; ierr = 4; }
+                else if (fabs(1.0 - sm) > SUM_TOL) { err = fabs(1.0 - sm); tol = SUM_TOL; ierr = 6; }
+                else { err = s.grd; tol = dgrd; ierr = 1; }
+                s.dtrcmd = fmax(fmin(sqrt(tol / fmax(err, SMALL_C)), rmax), rmin) * s.dttry;
+                step_ended = true;
+                continue;
+            }
+            s.phase = PH_NEWTON;
+            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+            return;
+        }
+        // below min_t9_for_slv: no nucleosynthesis, the step counts as done (solver.F90:180-183)
+        step_ended = true;
+        ierr = 0;
+    }
+    if (lane_id() == 0) Z.state[z] = s;
+}
+
+// One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z, using the
+// per-wave scratch S and LDS work w.  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON.
+__device__ inline void zone_newton(const KArgs &a, const Slot &S, LUWork w, int z) {
+    const DevNet &N = a.N;
+    const DevKnobs &K = a.K;
+    const ZoneData &Z = a.Z;
+    ZoneState s = Z.state[z];
+    if (s.phase != PH_NEWTON) return;
+    const double *y_old = zrow(Z.y_old, Z.nsp_stride, z);
+    const double *y_cur = zrow(Z.y_cur, Z.nsp_stride, z);
+    double *y_new = zrow(Z.y_new, Z.nsp_stride, z);
+    const double *v = zrow(Z.v, Z.nre_stride, z);
+    const double dt = s.dttry;
+    molar(N, y_cur, S.xn);
+    wave_sync();
+    reaction_terms(N, S.xn, v, S.vx, S.st1, S.st2);
+    wave_sync();
+    int e = assemble_matrix(N, dt, S.st1, S.st2, S.Aval);
+    for (int r = lane_id(); r < N.n; r += WAVE) {
+        const int sp = N.red2sp[r];
+        const double d = dxdt_species(N, r, S.vx);
+        S.rhs[r] = y_old[sp] - y_cur[sp] + dt * d;
+    }
+    wave_sync();
+    if (!e) {
+        const int info = gepp_factor(N.n, N.A_colptr, N.A_rowind, S.Aval, N.iperm_c, w, S.lu);
+        s.st.n_factor++;
+        if (info) e = -4;
+    }
+    if (!e) {
+        gepp_solve(N.n, N.perm_c, w, S.lu, S.rhs);
+        s.st.n_solve++;
+        for (int r = lane_id(); r < N.n; r += WAVE) {
+            const int sp = N.red2sp[r];
+            y_new[sp] = y_cur[sp] + S.rhs[r];
+        }
+        wave_sync();
+        double g = 0.0;
+        for (int i = lane_id(); i < N.nsp; i += WAVE) {
+            const double yc = y_cur[i];
+            const double d = y_new[i] - yc;
+            const double ga = (yc > K.grdthreshold) ? fabs(d / yc) : 0.0;
+            g = fmax(g, ga);
+        }
+        s.grd = wave_max(g);
+    }
+    s.lib_err = e;
+    s.phase = PH_AFTER_NEWTON;
+    if (lane_id() == 0) Z.state[z] = s;
+    wave_sync();
+}
+
+}  // namespace nuppn

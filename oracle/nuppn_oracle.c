@@ -563,7 +563,7 @@ int orc_lu_factor(int n, const int *colptr, const int *rowind, const double *val
         }
         int piv;
         if (first < 0) { info = j + 1; piv = -1; }
-        else if (pivmax == 0.0) { info = info ? info : j + 1; piv = touched[first]; }
+        else if (pivmax == 0.0) { info = j + 1; piv = touched[first]; }
         else {
             double thresh = 1.0 * pivmax;
             if (diag >= 0) {
@@ -572,11 +572,9 @@ int orc_lu_factor(int n, const int *colptr, const int *rowind, const double *val
             }
             piv = touched[pivptr];
         }
-        if (piv < 0) {  /* structurally empty column */
-            for (int t = 0; t < nt; ++t) { flag[touched[t]] = 0; x[touched[t]] = 0.0; }
-            lu->Up[j + 1] = (int) nu; lu->Lp[j + 1] = lu->Lp[j];
-            lu->pivrow[j] = -1;
-            continue;
+        if (info) {  /* no nonzero pivot: the factorisation stops (the caller stops too) */
+            for (int k2 = j; k2 < n; ++k2) { lu->Up[k2 + 1] = (int) nu; lu->Lp[k2 + 1] = (int) lu->Lp[j]; }
+            break;
         }
         lu->perm_r[piv] = j;
         lu->pivrow[j] = piv;
@@ -814,7 +812,8 @@ static int newton_iteration(orc_ctx *c, const double *y_old, const double *y_cur
     return 0;
 }
 
-/* backward_euler_newton, backward_euler.F90:29-109 */
+/* backward_euler_newton, backward_euler.F90:29-109.  x ** HALF is evaluated as sqrt(x), the
+ * correctly rounded value of the same quantity (libm pow differs from it by < 1 ulp). */
 int orc_backward_euler(orc_ctx *c, double *yps, double dt, double *dtrcmd, orc_stats *st)
 {
     const int nsp = c->nsp;
@@ -838,7 +837,7 @@ int orc_backward_euler(orc_ctx *c, double *yps, double dt, double *dtrcmd, orc_s
             else if (fabs(1.0 - plain_sum(nsp, y_new)) > SUM_TOL) {
                 err = fabs(1.0 - plain_sum(nsp, y_new)); tol = SUM_TOL; ierr = SUM_X;
             } else { err = grd; tol = dgrd; ierr = SLV_CNVRG; }
-            *dtrcmd = fmax(fmin(pow(tol / fmax(err, SMALL_C), 0.5), rmax), rmin) * dt;
+            *dtrcmd = fmax(fmin(sqrt(tol / fmax(err, SMALL_C)), rmax), rmin) * dt;
             break;
         }
         int e = newton_iteration(c, y_old, y_cur, dt, &grd, y_new, st);
@@ -848,7 +847,7 @@ int orc_backward_euler(orc_ctx *c, double *yps, double dt, double *dtrcmd, orc_s
         if (grd <= dgrd && maxval_neg(nsp, y_new) < sum_check &&
             fabs(1.0 - plain_sum(nsp, y_new)) <= SUM_TOL) {
             memcpy(yps, y_new, sizeof(double) * nsp);
-            *dtrcmd = 2.0 * fmax(fmin(pow(dgrd / fmax(grd, SMALL_C), 0.5), rmax), rmin) * dt;
+            *dtrcmd = 2.0 * fmax(fmin(sqrt(dgrd / fmax(grd, SMALL_C)), rmax), rmin) * dt;
             orc_ypsplus(nsp, yps);
             break;
         }
@@ -1088,6 +1087,12 @@ static double interp2(double x0, double x1, double y0, double y1, double xw)
 int orc_integrate_zone(orc_ctx *c, double *yps, double t9_0, double t9_1, double rho_0,
                        double rho_1, double *ye, double dtwant, orc_stats *st)
 {
+    return orc_integrate_zone_v(c, NULL, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, st);
+}
+
+int orc_integrate_zone_v(orc_ctx *c, const double *v_mid, double *yps, double t9_0, double t9_1,
+                         double rho_0, double rho_1, double *ye, double dtwant, orc_stats *st)
+{
     const int nsp = c->nsp;
     orc_stats local;
     if (!st) st = &local;
@@ -1095,7 +1100,8 @@ int orc_integrate_zone(orc_ctx *c, double *yps, double t9_0, double t9_1, double
     /* caller side: calculate_ye + evaluate_all_rates at the cycle mid-point (ppn.F90:289-298) */
     double t9m = 0.5 * (t9_0 + t9_1), rhom = 0.5 * (rho_0 + rho_1);
     *ye = orc_calculate_ye(c, yps);
-    orc_rates(c, t9m, rhom, *ye, c->v);
+    if (v_mid) memcpy(c->v, v_mid, sizeof(double) * c->nre);
+    else orc_rates(c, t9m, rhom, *ye, c->v);
 
     /* integrate_network, solver.F90:53-238 */
     double dtdone = 0.0, dtrcmd = 0.0, ye_old = *ye;

@@ -99,6 +99,10 @@ int orc_bader_deuflhard(orc_ctx *c, double *yps, double dt, double *dtrcmd, orc_
  * integrate_network (solver/source/solver.F90:53-238) */
 int orc_integrate_zone(orc_ctx *c, double *yps, double t9_0, double t9_1, double rho_0,
                        double rho_1, double *ye, double dtwant, orc_stats *st);
+/* the same with the mid-point rate vector supplied by the caller (v_mid [nre]) instead of
+ * evaluated here: isolates the integrator from libm differences in the rate formula */
+int orc_integrate_zone_v(orc_ctx *c, const double *v_mid, double *yps, double t9_0, double t9_1,
+                         double rho_0, double rho_1, double *ye, double dtwant, orc_stats *st);
 
 /* sortedsum (utils.F90:481-501, heapsort indexx of sort.F90:73-125) and calculate_ye
  * (utils.F90:537-553), ypsplus (utils.F90:726-740) */

@@ -78,6 +78,9 @@ def lib():
     L.orc_integrate_zone.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.POINTER(C.c_double), C.c_double,
                                      C.POINTER(Stats)]
+    L.orc_integrate_zone_v.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.POINTER(C.c_double), C.c_double,
+                                       C.POINTER(Stats)]
     L.orc_sortedsum.restype = C.c_double
     L.orc_sortedsum.argtypes = [C.c_int, _dp]
     L.orc_calculate_ye.restype = C.c_double
@@ -210,12 +213,18 @@ class Oracle:
         ierr = lib().orc_bader_deuflhard(self.h, y, float(dt), C.byref(dtr), C.byref(st))
         return ierr, y, dtr.value, st
 
-    def integrate_zone(self, y, t9_0, t9_1, rho_0, rho_1, dtwant, ye=0.5):
+    def integrate_zone(self, y, t9_0, t9_1, rho_0, rho_1, dtwant, ye=0.5, v_mid=None):
         y = f64(y).copy()
         yeo = C.c_double(ye)
         st = Stats()
-        ierr = lib().orc_integrate_zone(self.h, y, float(t9_0), float(t9_1), float(rho_0),
-                                        float(rho_1), C.byref(yeo), float(dtwant), C.byref(st))
+        if v_mid is None:
+            ierr = lib().orc_integrate_zone(self.h, y, float(t9_0), float(t9_1), float(rho_0),
+                                            float(rho_1), C.byref(yeo), float(dtwant),
+                                            C.byref(st))
+        else:
+            ierr = lib().orc_integrate_zone_v(self.h, f64(v_mid), y, float(t9_0), float(t9_1),
+                                              float(rho_0), float(rho_1), C.byref(yeo),
+                                              float(dtwant), C.byref(st))
         return ierr, y, yeo.value, st
 
 
