@@ -76,6 +76,7 @@ typedef struct {
     long long n_newton;
     long long n_factor;
     long long n_solve;
+    long long lu_nnz;     /* sum of nnz(L) + nnz(U) over the zone's factorisations */
 } nuppn_zone_stats;
 
 typedef struct nuppn_ctx nuppn_ctx;
@@ -145,9 +146,10 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
                               double *ye, const double *dtwant, int *ierr,
                               nuppn_zone_stats *stats, void *stream);
 
-/* Kernel timing of the last batch call (HIP events on the launch stream), milliseconds:
- * [0] rate kernel, [1] integration kernel. */
-int nuppn_last_timing(nuppn_ctx *ctx, float *ms2);
+/* Timing of the last batch call from HIP events on its stream: out[0] rate kernel ms,
+ * out[1] whole integration ms, out[2] summed Newton-kernel ms, out[3] summed control-kernel
+ * ms, out[4] Newton-kernel launches, out[5] controller passes.  nout: entries wanted (<= 6). */
+int nuppn_last_timing(nuppn_ctx *ctx, double *out, int nout);
 
 /* Link-level drop-ins with the reference signatures and semantics.
  * slu_solve: solver/source/slu_f90/slu_f90.c:3-54 (Fortran interface slu_f90/slu.f90:6-11);

@@ -28,8 +28,16 @@ __device__ __forceinline__ int readlane_i(int v, int l) { return __shfl(v, l, WA
 
 __device__ __forceinline__ double readlane_d(double v, int l) { return __shfl(v, l, WAVE); }
 
-// compiler-level ordering point between dependent LDS phases of one wavefront
+// LDS address space: pointers into the workgroup's LDS compile to ds_* instructions
+#define NUPPN_LDS __attribute__((address_space(3)))
+
+// Full ordering point between dependent phases of one wavefront (global and LDS memory).
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); __syncthreads(); }
+
+// Ordering point between dependent LDS accesses of different lanes of one wavefront: the LDS
+// executes a wavefront's DS instructions in issue order, so only compiler reordering has to be
+// prevented.
+__device__ __forceinline__ void lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
@@ -219,13 +227,14 @@ __device__ inline int assemble_matrix(const DevNet &N, double scale, const doubl
 
 // ------------------------------------------------------------------------------------------
 // Left-looking GEPP on one wavefront.
-struct LUWork {           // LDS
-    double *x;            // [n] dense column / solve vector
-    unsigned char *flag;  // [n]
-    int *perm_r;          // [n]
-    int *pivrow;          // [n]
-    int *touched;         // [n]
-    int *Lp;              // [n+1]
+struct LUWork {                    // LDS
+    NUPPN_LDS double *x;           // [n] dense column / solve vector
+    NUPPN_LDS unsigned char *flag; // [n]
+    NUPPN_LDS int *perm_r;         // [n]
+    NUPPN_LDS int *pivrow;         // [n]
+    NUPPN_LDS int *touched;        // [n]
+    NUPPN_LDS int *Lp;             // [n+1]
+    NUPPN_LDS int *ulist;          // [n] pivot columns updating the current column, in order
 };
 
 struct LUStore {          // global, per zone slot
@@ -272,8 +281,9 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
             }
             nt += popc(m);
         }
-        wave_sync();
+        lds_fence();
         // updates from earlier pivot columns, ascending k
+        int nk = 0;
         for (int k0 = 0; k0 < j; k0 += WAVE) {
             const int kl = k0 + lane;
             const int prl = (kl < j) ? w.pivrow[kl] : 0;
@@ -287,8 +297,8 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
                 const int k = k0 + t;
                 const int pr = readlane_i(prl, t);
                 const double xk = w.x[pr];
-                if (lane == 0) { s.Ui[nu] = k; s.Ux[nu] = xk; }
-                ++nu;
+                if (lane == 0) w.ulist[nk] = k;
+                ++nk;
                 const int l0 = w.Lp[k], l1 = w.Lp[k + 1];
                 for (int base = l0; base < l1; base += WAVE) {
                     const int p = base + lane;
@@ -304,18 +314,25 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
                     }
                     nt += popc(nm);
                     if (act) w.x[i] = w.x[i] - lv * xk;
-                    wave_sync();
+                    lds_fence();
                 }
             }
         }
+        // U(:,j): the updating pivot rows' final values, written once per column
+        for (int q = lane; q < nk; q += WAVE) {
+            const int k = w.ulist[q];
+            s.Ui[nu + q] = k;
+            s.Ux[nu + q] = w.x[w.pivrow[k]];
+        }
+        nu += nk;
         // pivot search over the not-yet-pivoted rows, in discovery order
         double best = 0.0;
         int bestt = 0x7fffffff, firstt = 0x7fffffff, diagt = -1;
         for (int t = lane; t < nt; t += WAVE) {
             const int r = w.touched[t];
             if (w.perm_r[r] < 0) {
-                const double a = fabs(w.x[r]);
-                if (a > best) { best = a; bestt = t; }
+                const double av = fabs(w.x[r]);
+                if (av > best) { best = av; bestt = t; }
                 if (t < firstt) firstt = t;
                 if (r == c) diagt = t;
             }
@@ -327,8 +344,8 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
         if (firstt == 0x7fffffff || gmax == 0.0) return j + 1;
         int pt = bt;
         if (diagt >= 0) {
-            const double a = fabs(w.x[w.touched[diagt]]);
-            if (a != 0.0 && a >= 1.0 * gmax) pt = diagt;
+            const double av = fabs(w.x[w.touched[diagt]]);
+            if (av != 0.0 && av >= 1.0 * gmax) pt = diagt;
         }
         const int piv = w.touched[pt];
         const double pv = w.x[piv];
@@ -340,7 +357,7 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
             w.pivrow[j] = piv;
         }
         ++nu;
-        wave_sync();
+        lds_fence();
         const double temp = 1.0 / pv;
         for (int base = 0; base < nt; base += WAVE) {
             const int t = base + lane;
@@ -361,7 +378,7 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
             w.flag[r] = 0;
             w.x[r] = 0.0;
         }
-        wave_sync();
+        wave_sync();   // L(:,j) stores visible to the loads of the next columns
     }
     return 0;
 }
@@ -371,9 +388,9 @@ __device__ inline int gepp_factor(int n, const int *__restrict__ Acp,
 __device__ inline void gepp_solve(int n, const int *__restrict__ perm_c, LUWork w, LUStore s,
                                   double *b) {
     const int lane = lane_id();
-    double *y = w.x;
+    NUPPN_LDS double *y = w.x;
     for (int i = lane; i < n; i += WAVE) y[w.perm_r[i]] = b[i];
-    wave_sync();
+    lds_fence();
     for (int j = 0; j < n; ++j) {
         const double yj = y[j];
         const int l0 = w.Lp[j], l1 = w.Lp[j + 1];
@@ -381,17 +398,18 @@ __device__ inline void gepp_solve(int n, const int *__restrict__ perm_c, LUWork 
             const int i = w.perm_r[s.Li[p]];
             y[i] = y[i] - s.Lx[p] * yj;
         }
-        wave_sync();
+        lds_fence();
     }
     for (int j = n - 1; j >= 0; --j) {
         const int u0 = s.Up[j], pd = s.Up[j + 1] - 1;
         const double yj = y[j] / s.Ux[pd];
+        lds_fence();   // every lane has read y(j) before it is overwritten
+        if (lane == 0) y[j] = yj;
         for (int p = u0 + lane; p < pd; p += WAVE) {
             const int k = s.Ui[p];
             y[k] = y[k] - s.Ux[p] * yj;
         }
-        if (lane == 0) y[j] = yj;
-        wave_sync();
+        lds_fence();
     }
     for (int i = lane; i < n; i += WAVE) b[i] = y[perm_c[i]];
     wave_sync();

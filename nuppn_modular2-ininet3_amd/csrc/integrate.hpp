@@ -15,7 +15,7 @@ struct DevKnobs {
 // per-zone diagnostics, layout of nuppn_zone_stats
 struct ZoneStats {
     int nsubt, titers, iter, nvar1, nsucc, nfail;
-    long long n_newton, n_factor, n_solve;
+    long long n_newton, n_factor, n_solve, lu_nnz;
 };
 
 // Per-slot global scratch (one slot per resident wavefront of the persistent kernel).

@@ -178,7 +178,9 @@ struct nuppn_ctx {
     ZoneStats *d_stats = nullptr;
     int *d_status = nullptr;
     hipEvent_t ev[4] = {};
-    float last_ms[2] = {0.f, 0.f};
+    hipEvent_t evn[2] = {}, evc[2] = {};
+    double newton_ms = 0.0, control_ms = 0.0;
+    long long newton_launches = 0;
     long long last_passes = 0;
     std::mutex mu;
 };
@@ -406,6 +408,10 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     c->max_slots = std::max(1, std::min(ncu * per_cu, by_mem));
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    for (auto &e : c->evn)
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    for (auto &e : c->evc)
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
     if (hipHostMalloc((void **) &c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess)
         return nullptr;
     if (ensure_slots(c.get(), 1)) return nullptr;
@@ -429,6 +435,10 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_status})
         dev_free(p);
     for (auto &e : c->ev)
+        if (e) (void) hipEventDestroy(e);
+    for (auto &e : c->evn)
+        if (e) (void) hipEventDestroy(e);
+    for (auto &e : c->evc)
         if (e) (void) hipEventDestroy(e);
     if (c->h_flag) (void) hipHostFree(c->h_flag);
     delete c;
@@ -607,27 +617,49 @@ int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t
     // each pass advances every zone by at least one Newton iteration or finishes it
     const long long max_pass = (long long) c->kn.max_sub_steps * (c->kn.ittd + 2) + 16;
     long long pass = 0;
+    bool newton_pending = false;
+    c->newton_ms = 0.0;
+    c->control_ms = 0.0;
+    c->newton_launches = 0;
     for (;; ++pass) {
         hipLaunchKernelGGL(flag_reset_kernel, dim3(1), dim3(1), 0, s, c->d_flag);
+        HIP_OK(hipEventRecord(c->evc[0], s));
         hipLaunchKernelGGL(zone_control_kernel, dim3(nz), dim3(64), 0, s, a);
         HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(c->evc[1], s));
         HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        float ms = 0.f;
+        if (newton_pending) {
+            HIP_OK(hipEventElapsedTime(&ms, c->evn[0], c->evn[1]));
+            c->newton_ms += ms;
+            newton_pending = false;
+        }
+        HIP_OK(hipEventElapsedTime(&ms, c->evc[0], c->evc[1]));
+        c->control_ms += ms;
         if (c->h_flag[0] == 0) break;
         if (pass >= max_pass) return NUPPN_ERR_HIP;
+        HIP_OK(hipEventRecord(c->evn[0], s));
         hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
         HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(c->evn[1], s));
+        newton_pending = true;
+        c->newton_launches++;
     }
     c->last_passes = pass;
     HIP_OK(hipEventRecord(c->ev[3], s));
     return NUPPN_OK;
 }
 
-int nuppn_last_timing(nuppn_ctx *c, float *ms2) {
-    if (!c || !ms2) return NUPPN_ERR_ARG;
+int nuppn_last_timing(nuppn_ctx *c, double *out, int nout) {
+    if (!c || !out || nout <= 0) return NUPPN_ERR_ARG;
     HIP_OK(hipEventSynchronize(c->ev[3]));
-    HIP_OK(hipEventElapsedTime(&ms2[0], c->ev[0], c->ev[1]));
-    HIP_OK(hipEventElapsedTime(&ms2[1], c->ev[2], c->ev[3]));
+    float r = 0.f, t = 0.f;
+    HIP_OK(hipEventElapsedTime(&r, c->ev[0], c->ev[1]));
+    HIP_OK(hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
+    const double vals[6] = {r, t, c->newton_ms, c->control_ms, (double) c->newton_launches,
+                            (double) c->last_passes};
+    for (int i = 0; i < nout && i < 6; ++i) out[i] = vals[i];
     return NUPPN_OK;
 }
 

@@ -103,7 +103,7 @@ __device__ inline Slot make_slot(char *base, const size_t *off) {
 
 // LDS: x[n] doubles, perm_r / pivrow / touched [n] ints, Lp [n+1] ints, flag [n] bytes
 inline size_t lds_bytes(int n) {
-    size_t b = 8ull * n + 4ull * n * 3 + 4ull * (n + 1);
+    size_t b = 8ull * n + 4ull * n * 4 + 4ull * (n + 1);
     b = (b + 15) & ~(size_t) 15;
     b += (size_t) n;
     return (b + 255) & ~(size_t) 255;
@@ -111,19 +111,21 @@ inline size_t lds_bytes(int n) {
 
 __device__ inline LUWork make_work(char *smem, int n) {
     LUWork w;
-    char *p = smem;
-    w.x = (double *) p;
+    NUPPN_LDS char *p = (NUPPN_LDS char *) smem;
+    w.x = (NUPPN_LDS double *) p;
     p += 8ull * n;
-    w.perm_r = (int *) p;
+    w.perm_r = (NUPPN_LDS int *) p;
     p += 4ull * n;
-    w.pivrow = (int *) p;
+    w.pivrow = (NUPPN_LDS int *) p;
     p += 4ull * n;
-    w.touched = (int *) p;
+    w.touched = (NUPPN_LDS int *) p;
     p += 4ull * n;
-    w.Lp = (int *) p;
+    w.ulist = (NUPPN_LDS int *) p;
+    p += 4ull * n;
+    w.Lp = (NUPPN_LDS int *) p;
     p += 4ull * (n + 1);
-    p = (char *) (((uintptr_t) p + 15) & ~(uintptr_t) 15);
-    w.flag = (unsigned char *) p;
+    p = (NUPPN_LDS char *) (((uintptr_t) p + 15) & ~(uintptr_t) 15);
+    w.flag = (NUPPN_LDS unsigned char *) p;
     return w;
 }
 
@@ -261,7 +263,9 @@ __device__ inline void zone_control(const KArgs &a, int z) {
                 s.st.nfail++;
             }
             s.dttry = fmin(s.dtrcmd, dtwant - s.dtdone);
-            if (s.i == K.max_sub_steps) { s.st.nsubt = s.i; zone_finish(a, z, s, 2, false); break; }
+            // MAX_SUBSTEPS and SMALL_DT return before nsubt is assigned (solver.F90:174-178,
+            // 218-225): the per-call count stays 0
+            if (s.i == K.max_sub_steps) { zone_finish(a, z, s, 2, false); break; }
         }
         // next sub-step (solver.F90:108-164)
         s.i += 1;
@@ -359,6 +363,7 @@ __device__ inline void zone_newton(const KArgs &a, const Slot &S, LUWork w, int 
         const int info = gepp_factor(N.n, N.A_colptr, N.A_rowind, S.Aval, N.iperm_c, w, S.lu);
         s.st.n_factor++;
         if (info) e = -4;
+        else s.st.lu_nnz += (long long) w.Lp[N.n] + (long long) S.lu.Up[N.n];
     }
     if (!e) {
         gepp_solve(N.n, N.perm_c, w, S.lu, S.rhs);

@@ -41,7 +41,7 @@ class Knobs(C.Structure):
 class ZoneStats(C.Structure):
     _fields_ = [("nsubt", C.c_int), ("titers", C.c_int), ("iter", C.c_int), ("nvar1", C.c_int),
                 ("nsucc", C.c_int), ("nfail", C.c_int), ("n_newton", C.c_longlong),
-                ("n_factor", C.c_longlong), ("n_solve", C.c_longlong)]
+                ("n_factor", C.c_longlong), ("n_solve", C.c_longlong), ("lu_nnz", C.c_longlong)]
 
 
 class Network(C.Structure):
@@ -53,7 +53,8 @@ class Network(C.Structure):
 
 STATS_DTYPE = np.dtype([("nsubt", np.int32), ("titers", np.int32), ("iter", np.int32),
                         ("nvar1", np.int32), ("nsucc", np.int32), ("nfail", np.int32),
-                        ("n_newton", np.int64), ("n_factor", np.int64), ("n_solve", np.int64)])
+                        ("n_newton", np.int64), ("n_factor", np.int64), ("n_solve", np.int64),
+                        ("lu_nnz", np.int64)])
 
 _lib = None
 
@@ -86,7 +87,7 @@ def load():
     L.nuppn_integrate_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                         _ip, C.c_void_p]
     L.nuppn_integrate_batch_dev.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
-    L.nuppn_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.nuppn_last_timing.argtypes = [C.c_void_p, _dp, C.c_int]
     L.slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                             C.POINTER(C.c_int)]
     L.c_fortran_dgssv_.argtypes = [C.POINTER(C.c_int)] * 4 + [_dp, _ip, _ip, _dp,
@@ -245,9 +246,13 @@ class Context:
             raise RuntimeError(f"nuppn_integrate_batch_dev failed ({rc})")
 
     def last_timing(self):
-        ms = (C.c_float * 2)()
-        load().nuppn_last_timing(self.h, ms)
-        return float(ms[0]), float(ms[1])
+        """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes"""
+        out = np.zeros(6)
+        rc = load().nuppn_last_timing(self.h, out, 6)
+        if rc != OK:
+            raise RuntimeError(f"nuppn_last_timing failed ({rc})")
+        keys = ("rates_ms", "total_ms", "newton_ms", "control_ms", "newton_launches", "passes")
+        return dict(zip(keys, out.tolist()))
 
 
 def slu_solve(val, colind, rowptr, rhs, isrow=1, shiftindex=1):
