@@ -117,14 +117,18 @@ def main():
                                 r0.data_ptr(), r1.data_ptr(), ye.data_ptr(), dt.data_ptr(),
                                 ierr.data_ptr(), stats.data_ptr(), 0)
 
-    for _ in range(a.warmup):
+    for i in range(a.warmup):
         step()
+        torch.cuda.synchronize()
+        print(f"warmup step {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     timings = []
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for i in range(a.steps):
         step()
         timings.append(ctx.last_timing())
+        print(f"step {i + 1}/{a.steps}: {timings[-1]['total_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = np.frombuffer(stats.cpu().numpy().tobytes(), dtype=hip.STATS_DTYPE)

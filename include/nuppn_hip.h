@@ -76,7 +76,9 @@ typedef struct {
     long long n_newton;
     long long n_factor;
     long long n_solve;
-    long long lu_nnz;     /* sum of nnz(L) + nnz(U) over the zone's factorisations */
+    long long lu_nnz;     /* sum of nnz(L) + nnz(U) over the zone's factorisations (plain path) */
+    long long n_refine;   /* FP64 refinement residuals (batched path) */
+    long long n_factor_fp64;  /* factorisations done with FP64 factors (batched path) */
 } nuppn_zone_stats;
 
 typedef struct nuppn_ctx nuppn_ctx;
@@ -145,6 +147,40 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
                               const double *t9_1, const double *rho_0, const double *rho_1,
                               double *ye, const double *dtwant, int *ierr,
                               nuppn_zone_stats *stats, void *stream);
+
+/* Options of the batched path (nuppn_integrate_batch_mixed): the reduced system is factored
+ * without pivoting in a block-tridiagonal + border form (molar units, columns scaled by the
+ * diagonal), in FP32 (factor_precision 32) or FP64 (64); every solve is refined in FP64 against
+ * the FP64 matrix until max|correction| <= refine_tol * max|x|.  When FP32 refinement stalls above
+ * accept_tol the zone switches to FP64 factors. */
+typedef struct {
+    int factor_precision;   /* 32 (default) or 64 */
+    int max_refine;         /* refinement steps per solve (8) */
+    double refine_tol;      /* 1e-13 */
+    double accept_tol;      /* 1e-10 */
+} nuppn_batch_opts;
+
+void nuppn_default_batch_opts(nuppn_batch_opts *o);
+
+/* Block structure of the batched path: info[0..6] = block size NB (0: unavailable), blocks,
+ * border species, interior species, interior half-bandwidth, resident slots, LDS bytes per
+ * slot (FP32 factors).  Returns NUPPN_OK or NUPPN_ERR_UNSUPPORTED. */
+int nuppn_batch_block_info(nuppn_ctx *ctx, int *info, int ninfo);
+
+/* Batched mixed-precision path: all zones of an mppnp cycle (replaces the worker loop of
+ * frames/mppnp/source/mppnp.F90:713-753).  Same control flow, decisions code and outputs as
+ * nuppn_integrate_batch; the linear solves are the refined block factorisation above instead of
+ * the reference's GEPP, so results agree with the plain path to within the FP64 rounding spread
+ * of the solve, not bit for bit. */
+int nuppn_integrate_batch_mixed(nuppn_ctx *ctx, int nz, double *yps, const double *t9_0,
+                                const double *t9_1, const double *rho_0, const double *rho_1,
+                                double *ye, const double *dtwant, int *ierr,
+                                nuppn_zone_stats *stats, const nuppn_batch_opts *opts);
+int nuppn_integrate_batch_mixed_dev(nuppn_ctx *ctx, int nz, double *yps, const double *t9_0,
+                                    const double *t9_1, const double *rho_0, const double *rho_1,
+                                    double *ye, const double *dtwant, int *ierr,
+                                    nuppn_zone_stats *stats, const nuppn_batch_opts *opts,
+                                    void *stream);
 
 /* Timing of the last batch call from HIP events on its stream: out[0] rate kernel ms,
  * out[1] whole integration ms, out[2] summed Newton-kernel ms, out[3] summed control-kernel

@@ -1,0 +1,203 @@
+// Batched path: the Newton iteration of the backward-Euler step (newton_iteration,
+// solver/source/backward_euler.F90:120-176) with the pivot-free block factorisation of
+// block_lu.hpp and FP64 residual refinement, for every zone of a cycle that needs one.
+//
+// It plugs into the same per-zone state machine as the plain path (zone.hpp: zone_init /
+// zone_control run integrate_network's sub-step and Newton decisions in FP64), so both paths
+// take their decisions with one implementation; only the linear solve differs:
+//     x_0 = 0, r_0 = rhs;  d_k = (LU)^-1 r_k (factors in precision T),  x_{k+1} = x_k + d_k,
+//     r_{k+1} = rhs - A x_{k+1} (FP64, the assembled matrix)
+// stopping when max|d_k| <= refine_tol * max|x_{k+1}|.  When the corrections stop shrinking
+// (a contraction worse than 1/2 per step, i.e. cond(A) * eps(T) is not small) above accept_tol
+// with FP32 factors, the zone is marked for FP64 factors (for the rest of the call) and left in
+// the Newton phase: the FP64 kernel, launched after the FP32 one in the same pass, redoes the
+// iteration.  With FP64 factors a stalled refinement is accepted, as the reference accepts its
+// single GEPP solve.
+#pragma once
+#include "block_lu.hpp"
+#include "zone.hpp"
+
+namespace nuppn {
+
+struct BatchOpts {
+    int fp64_factors;   // 1: FP64 factors from the start (no FP32 stage)
+    int max_refine;     // refinement steps per solve
+    double refine_tol;  // stop when max|d| <= refine_tol * max|x|
+    double accept_tol;  // a stalled refinement is accepted when max|d| <= accept_tol * max|x|
+};
+
+// per-slot scratch of the block Newton kernels
+enum BSlotArr {
+    B_XN, B_VX, B_ST1, B_ST2, B_AVAL, B_RHS, B_X, B_RES, B_D, B_CS, B_SINV, B_W, B_BM, B_E, B_XF,
+    B_G, B_GPIV, B_COUNT
+};
+
+struct BSlotLayout {
+    size_t off[B_COUNT];
+    size_t bytes;
+};
+
+inline BSlotLayout bslot_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
+                                int nbd) {
+    const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
+    const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 8ull * nre, 8ull * nre, 8ull * nnzA,
+                                 8ull * n, 8ull * n, 8ull * n, 8ull * n, 8ull * n, bs, bs, bs,
+                                 8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
+                                 4ull * nbd + 4};
+    BSlotLayout L;
+    size_t o = 0;
+    for (int i = 0; i < B_COUNT; ++i) {
+        L.off[i] = o;
+        o += (len[i] + 255) & ~(size_t) 255;
+    }
+    L.bytes = o;
+    return L;
+}
+
+struct BKArgs {
+    DevBlock B;
+    BatchOpts O;
+    char *slot_base;
+    size_t slot_bytes;
+    size_t off[B_COUNT];
+};
+
+template <class T>
+__device__ inline BlockFactors<T> slot_factors(char *base, const size_t *off) {
+    BlockFactors<T> F;
+    F.Sinv = (T *) (base + off[B_SINV]);
+    F.W = (T *) (base + off[B_W]);
+    F.Bm = (T *) (base + off[B_BM]);
+    F.E = (T *) (base + off[B_E]);
+    F.XF = (T *) (base + off[B_XF]);
+    F.G = (double *) (base + off[B_G]);
+    F.gpiv = (int *) (base + off[B_GPIV]);
+    F.cs = (double *) (base + off[B_CS]);
+    return F;
+}
+
+// Factor + refined solve of the assembled system in precision T.  Returns 0 (x holds the
+// solution), 1 when FP32 factors should be replaced by FP64 ones, or -4 when the FP64
+// factorisation fails (zero or non-finite pivot).
+template <class T, int NB>
+__device__ int block_refined_solve(const BKArgs &b, const double *Aval, const double *rhs,
+                                   double *x, double *res, double *d, char *slot,
+                                   NUPPN_LDS char *lds, ZoneStats &st) {
+    constexpr bool fp64 = sizeof(T) == 8;
+    const DevBlock &B = b.B;
+    const int n = B.n;
+    const BlockFactors<T> F = slot_factors<T>(slot, b.off);
+    const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
+    const int fail = block_factor<T, NB>(B, Aval, F, L);
+    st.n_factor++;
+    if (fp64) st.n_factor_fp64++;
+    if (fail) return fp64 ? -4 : 1;
+    for (int i = lane_id(); i < n; i += WAVE) {
+        x[i] = 0.0;
+        res[i] = rhs[i];
+    }
+    wave_sync();
+    double eta_prev = 0.0;
+    for (int k = 0; k < b.O.max_refine; ++k) {
+        block_solve<T, NB>(B, F, L, res, d);
+        st.n_solve++;
+        double dm = 0.0, xm = 0.0;
+        bool fin = true;
+        for (int i = lane_id(); i < n; i += WAVE) {
+            const double xi = x[i] + d[i];
+            x[i] = xi;
+            dm = fmax(dm, fabs(d[i]));
+            xm = fmax(xm, fabs(xi));
+            fin = fin && isfinite(xi);
+        }
+        dm = wave_max(dm);
+        xm = wave_max(xm);
+        wave_sync();
+        if (ballot(!fin)) return fp64 ? -4 : 1;
+        const double eta = dm / fmax(xm, 1e-300);
+        if (eta <= b.O.refine_tol) return 0;
+        if ((k > 0 && eta > 0.5 * eta_prev) || k == b.O.max_refine - 1)
+            return (!fp64 && eta > b.O.accept_tol) ? 1 : 0;
+        eta_prev = eta;
+        block_residual(B, Aval, rhs, x, res);
+        st.n_refine++;
+    }
+    return 0;
+}
+
+// One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z with the
+// block solve in precision T; the zone must be in PH_NEWTON with factor precision T (state
+// pad0: 0 FP32, 1 FP64).  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON, or raises
+// the zone to FP64 and leaves it in PH_NEWTON.
+template <class T, int NB>
+__device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
+                                  NUPPN_LDS char *lds, int z) {
+    constexpr int want = sizeof(T) == 8 ? 1 : 0;
+    const DevNet &N = a.N;
+    const DevKnobs &K = a.K;
+    const ZoneData &Z = a.Z;
+    ZoneState s = Z.state[z];
+    if (s.phase != PH_NEWTON || s.pad0 != want) return;
+    const double *y_old = zrow(Z.y_old, Z.nsp_stride, z);
+    const double *y_cur = zrow(Z.y_cur, Z.nsp_stride, z);
+    double *y_new = zrow(Z.y_new, Z.nsp_stride, z);
+    const double *v = zrow(Z.v, Z.nre_stride, z);
+    double *xn = (double *) (slot + b.off[B_XN]);
+    double *vx = (double *) (slot + b.off[B_VX]);
+    double *st1 = (double *) (slot + b.off[B_ST1]);
+    double *st2 = (double *) (slot + b.off[B_ST2]);
+    double *Aval = (double *) (slot + b.off[B_AVAL]);
+    double *rhs = (double *) (slot + b.off[B_RHS]);
+    double *x = (double *) (slot + b.off[B_X]);
+    double *res = (double *) (slot + b.off[B_RES]);
+    double *d = (double *) (slot + b.off[B_D]);
+    const double dt = s.dttry;
+    molar(N, y_cur, xn);
+    wave_sync();
+    reaction_terms(N, xn, v, vx, st1, st2);
+    wave_sync();
+    (void) assemble_matrix(N, dt, st1, st2, Aval);
+    for (int r = lane_id(); r < N.n; r += WAVE) {
+        const int sp = N.red2sp[r];
+        rhs[r] = y_old[sp] - y_cur[sp] + dt * dxdt_species(N, r, vx);
+    }
+    wave_sync();
+    const int rc = block_refined_solve<T, NB>(b, Aval, rhs, x, res, d, slot, lds, s.st);
+    if (rc == 1) {   // FP32 factors insufficient: the FP64 kernel redoes this iteration
+        s.pad0 = 1;
+        if (lane_id() == 0) Z.state[z] = s;
+        wave_sync();
+        return;
+    }
+    if (rc == 0) {
+        for (int r = lane_id(); r < N.n; r += WAVE) {
+            const int sp = N.red2sp[r];
+            y_new[sp] = y_cur[sp] + x[r];
+        }
+        wave_sync();
+        double g = 0.0;
+        for (int i = lane_id(); i < N.nsp; i += WAVE) {
+            const double yc = y_cur[i];
+            const double dd = y_new[i] - yc;
+            const double ga = (yc > K.grdthreshold) ? fabs(dd / yc) : 0.0;
+            g = fmax(g, ga);
+        }
+        s.grd = wave_max(g);
+    }
+    s.lib_err = rc;
+    s.phase = PH_AFTER_NEWTON;
+    if (lane_id() == 0) Z.state[z] = s;
+    wave_sync();
+}
+
+// every zone in PH_NEWTON at precision T; each wavefront owns one scratch slot and strides over
+// the zones
+template <class T, int NB>
+__global__ void __launch_bounds__(64) zone_newton_block_kernel(KArgs a, BKArgs b) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
+    char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
+    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton_block<T, NB>(a, b, slot, lds, z);
+}
+
+}  // namespace nuppn

@@ -1,0 +1,445 @@
+// Pivot-free block-tridiagonal factorisation of the reduced system I - dt J on one wavefront,
+// used by the batched path (FP32 factors with FP64 refinement, or FP64 factors).
+//
+// Why no pivoting: in molar units (Y = X/A) the interior part of I - dt J (every species but the
+// dense light projectiles) is a column diagonally dominant Z-matrix: each column holds 1 + dt
+// times the destruction rate on the diagonal and minus dt times the production rates elsewhere,
+// and mass conservation bounds the latter by the former.  Gaussian elimination without pivoting
+// is backward stable on such matrices (growth factor <= 2).  The reference's threshold pivoting
+// (SuperLU_5.0/SRC/dpivotL.c:114-156) instead works in mass-fraction units, where the column
+// ratios are off by A_row / A_col and off-diagonal pivots are common; both give a backward-stable
+// solve, and this path's FP64 residual refinement compares them on equal footing.
+//
+// Layout (host-built, BlockSym): interior species in network order, split into nblk blocks of
+// NB; border species (nbd) last.  The matrix is scaled to molar units and its columns by the
+// reciprocal diagonal (values then lie in [-2, 2] with unit diagonal, inside FP32 range), and
+//     A = [ T  F ]      T block tridiagonal: diagonal D_i, super-diagonal C_i, sub-diagonal B_i
+//         [ E  G ]
+// is factored as
+//     S_0 = D_0,  W_i = S_i^-1 C_i,  S_{i+1} = D_{i+1} - B_i W_i          (block Thomas)
+//     X_F = T^-1 F,  G' = G - E X_F                                       (border Schur complement)
+// storing S_i^-1 (explicit, Gauss-Jordan), W_i and B_i per block, X_F, E and the LU of G'.
+// A solve is then two sweeps of dense NB x NB mat-vecs plus the border correction.
+#pragma once
+#include "device_common.hpp"
+
+namespace nuppn {
+
+struct DevBlock {
+    int NB, nbd, ni, nblk, nip, n;
+    const int *vpos;       // [n] reduced index -> padded vector slot
+    const double *mass;    // [n]
+    const int *diagq;      // [n]
+    const int *ent_ptr, *ent_q, *ent_loc, *ent_col;
+    const double *ent_ys;
+    const int *csr_ptr, *csr_q, *csr_col;
+};
+
+// factors of one zone slot; T is the factor precision
+template <class T>
+struct BlockFactors {
+    T *Sinv, *W, *Bm;   // [nblk][NB*NB] column-major
+    T *E, *XF;          // [nbd][nip]
+    double *G;          // [nbd*nbd] LU of G' (column-major), unit lower part below the diagonal
+    int *gpiv;          // [nbd] row interchanges of the LU of G'
+    double *cs;         // [n] 1 / diagonal of A: the column scaling
+};
+
+template <class T, int NB>
+struct BlockCfg {
+    static constexpr int H = WAVE / NB;   // lanes per block column
+    static constexpr int R = NB / H;      // rows of a block column held by one lane
+    static constexpr int LD = NB + 4;     // LDS leading dimension (16-byte aligned, conflict-free)
+};
+
+// LDS of one wavefront: three NB x LD operand buffers, pivot row / column, the solve vector
+template <class T>
+struct BlockLds {
+    NUPPN_LDS T *X1, *X2, *WL, *colbuf, *rowbuf, *vec;
+    NUPPN_LDS double *bscr;   // [16] border solve scratch
+};
+
+template <class T, int NB>
+__host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd) {
+    (void) nbd;   // nbd <= 8: the border entries fit the 8 slots after nip
+    return sizeof(T) * (3 * (size_t) NB * BlockCfg<T, NB>::LD + 2 * NB + (size_t) nip + 8) + 16 +
+           16 * sizeof(double);
+}
+
+template <class T, int NB>
+__device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
+    using C = BlockCfg<T, NB>;
+    BlockLds<T> L;
+    NUPPN_LDS T *p = (NUPPN_LDS T *) base;
+    L.X1 = p;
+    p += NB * C::LD;
+    L.X2 = p;
+    p += NB * C::LD;
+    L.WL = p;
+    p += NB * C::LD;
+    L.colbuf = p;
+    p += NB;
+    L.rowbuf = p;
+    p += NB;
+    L.vec = p;
+    NUPPN_LDS char *q = (NUPPN_LDS char *) (p + nip + 8);
+    q = (NUPPN_LDS char *) (((uintptr_t) q + 15) & ~(uintptr_t) 15);
+    L.bscr = (NUPPN_LDS double *) q;
+    return L;
+}
+
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) { return fma(a, b, c); }
+
+// the R elements of a block column held in registers by one lane
+template <class T, int NB>
+struct Regs {
+    T v[BlockCfg<T, NB>::R];
+};
+
+// --- lane-layout helpers: lane (c = lane % NB, h = lane / NB) holds rows h*R .. h*R+R-1 of
+//     block column c
+template <class T, int NB>
+__device__ __forceinline__ void lds_load_cm(const NUPPN_LDS T *X, Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) a.v[t] = X[c * C::LD + h * C::R + t];
+}
+
+template <class T, int NB>
+__device__ __forceinline__ void lds_store_cm(NUPPN_LDS T *X, const Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) X[c * C::LD + h * C::R + t] = a.v[t];
+}
+
+template <class T, int NB>
+__device__ __forceinline__ void lds_store_rm(NUPPN_LDS T *X, const Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) X[(h * C::R + t) * C::LD + c] = a.v[t];
+}
+
+template <class T, int NB>
+__device__ __forceinline__ void glb_store_cm(T *G, const Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) G[c * NB + h * C::R + t] = a.v[t];
+}
+
+// In-place inverse of the NB x NB block held in lane layout, Gauss-Jordan without pivoting.
+template <class T, int NB>
+__device__ inline void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
+                                  NUPPN_LDS T *rowbuf) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+    constexpr int KU = NB <= 32 ? NB : 2;
+#pragma unroll KU
+    for (int k = 0; k < NB; ++k) {
+        const int kh = k / C::R, kt = k % C::R;
+        if (c == k) {
+#pragma unroll
+            for (int t = 0; t < C::R; ++t) colbuf[h * C::R + t] = a.v[t];
+        }
+        if (h == kh) {
+            T rv = a.v[0];
+#pragma unroll
+            for (int t = 1; t < C::R; ++t)
+                if (t == kt) rv = a.v[t];
+            rowbuf[c] = rv;
+        }
+        lds_fence();
+        const T pinv = T(1) / colbuf[k];
+        const T mkc = rowbuf[c] * pinv;
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) {
+            const int r = h * C::R + t;
+            const T col = colbuf[h * C::R + t];
+            T v;
+            if (c == k) v = (r == k) ? pinv : -col * pinv;
+            else if (r == k) v = mkc;
+            else v = fma_t(-col, mkc, a.v[t]);
+            a.v[t] = v;
+        }
+        lds_fence();   // every read of this step precedes the next step's writes
+    }
+}
+
+// acc = A * B with A column-major and B row-major in LDS (NB x NB), result in lane layout
+template <class T, int NB>
+__device__ inline void gemm_lds(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
+                                Regs<T, NB> &acc) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) acc.v[t] = T(0);
+#pragma unroll 2
+    for (int k = 0; k < NB; ++k) {
+        const T bk = Brm[k * C::LD + c];
+        const NUPPN_LDS T *ap = Acm + k * C::LD + h * C::R;
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) acc.v[t] = fma_t(ap[t], bk, acc.v[t]);
+    }
+}
+
+// acc -= A * B (same layouts as gemm_lds)
+template <class T, int NB>
+__device__ inline void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
+                                    Regs<T, NB> &acc) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll 2
+    for (int k = 0; k < NB; ++k) {
+        const T bk = Brm[k * C::LD + c];
+        const NUPPN_LDS T *ap = Acm + k * C::LD + h * C::R;
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) acc.v[t] = fma_t(-ap[t], bk, acc.v[t]);
+    }
+}
+
+// scaled value of CSC entry e of a block list: molar units, column-scaled
+__device__ __forceinline__ double block_entry(const DevBlock &B, const double *Aval,
+                                              const double *cs, int e) {
+    return Aval[B.ent_q[e]] * B.ent_ys[e] * cs[B.ent_col[e]];
+}
+
+// zero an NB x LD LDS buffer and scatter the entries of list `list` into it; identity on the
+// padded diagonal positions of the last block when `pad_block` >= 0
+template <class T, int NB>
+__device__ inline void scatter_block(NUPPN_LDS T *X, bool rowmajor, const DevBlock &B, int list,
+                                     const double *Aval, const double *cs, int pad_block) {
+    using C = BlockCfg<T, NB>;
+    lds_fence();
+    for (int i = lane_id(); i < NB * C::LD; i += WAVE) X[i] = T(0);
+    lds_fence();
+    for (int e = B.ent_ptr[list] + lane_id(); e < B.ent_ptr[list + 1]; e += WAVE) {
+        const int loc = B.ent_loc[e], r = loc % NB, c = loc / NB;
+        X[rowmajor ? r * C::LD + c : c * C::LD + r] = (T) block_entry(B, Aval, cs, e);
+    }
+    if (pad_block >= 0) {
+        const int r = lane_id();
+        if (r < NB && pad_block * NB + r >= B.ni) X[r * C::LD + r] = T(1);
+    }
+    lds_fence();
+}
+
+// y = M x for an NB x NB column-major block M in global memory and x in LDS; the result is
+// returned in every lane whose lane % NB is the row (both halves hold the full sum)
+template <class T, int NB>
+__device__ __forceinline__ T matvec_row(const T *__restrict__ M, const NUPPN_LDS T *x) {
+    using C = BlockCfg<T, NB>;
+    const int rr = lane_id() % NB, h = lane_id() / NB;
+    constexpr int CPL = NB / C::H;   // columns per lane
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        const int col = h * CPL + j;
+        acc = fma_t(M[col * NB + rr], x[col], acc);
+    }
+    if (C::H == 2) acc += __shfl_xor(acc, 32, WAVE);
+    return acc;
+}
+
+// v <- T^-1 v for the interior block-tridiagonal part (v in LDS, nip entries)
+template <class T, int NB>
+__device__ inline void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
+                                    NUPPN_LDS T *v) {
+    const int rr = lane_id() % NB, h = lane_id() / NB;
+    constexpr size_t BS = (size_t) NB * NB;
+    for (int i = 0; i < B.nblk; ++i) {
+        NUPPN_LDS T *vi = v + i * NB;
+        if (i > 0) {
+            const T acc = matvec_row<T, NB>(F.Bm + (size_t) (i - 1) * BS, v + (i - 1) * NB);
+            lds_fence();
+            if (h == 0) vi[rr] = vi[rr] - acc;
+            lds_fence();
+        }
+        const T acc = matvec_row<T, NB>(F.Sinv + (size_t) i * BS, vi);
+        lds_fence();
+        if (h == 0) vi[rr] = acc;
+        lds_fence();
+    }
+    for (int i = B.nblk - 2; i >= 0; --i) {
+        NUPPN_LDS T *vi = v + i * NB;
+        const T acc = matvec_row<T, NB>(F.W + (size_t) i * BS, v + (i + 1) * NB);
+        lds_fence();
+        if (h == 0) vi[rr] = vi[rr] - acc;
+        lds_fence();
+    }
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Factor the reduced matrix (FP64 CSC values Aval) into F.  Returns 0, or 1 when a pivot or
+// the border Schur complement is not finite / zero (the caller falls back or reports).
+template <class T, int NB>
+__device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFactors<T> &F,
+                            const BlockLds<T> &L) {
+    using C = BlockCfg<T, NB>;
+    constexpr size_t BS = (size_t) NB * NB;
+    const int lane = lane_id();
+    const int nblk = B.nblk;
+    bool bad = false;
+    for (int i = lane; i < B.n; i += WAVE) {
+        const double d = Aval[B.diagq[i]];
+        if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
+        F.cs[i] = 1.0 / d;
+    }
+    if (ballot(bad)) return 1;
+    wave_sync();
+    Regs<T, NB> a;
+    scatter_block<T, NB>(L.X2, false, B, 0, Aval, F.cs, nblk == 1 ? 0 : -1);
+    lds_load_cm<T, NB>(L.X2, a);
+    for (int i = 0; i < nblk; ++i) {
+        gj_inverse<T, NB>(a, L.colbuf, L.rowbuf);
+        glb_store_cm<T, NB>(F.Sinv + i * BS, a);
+        bool fin = true;
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) fin = fin && isfinite((double) a.v[t]);
+        if (ballot(!fin)) return 1;
+        if (i == nblk - 1) break;
+        lds_fence();
+        lds_store_cm<T, NB>(L.X1, a);
+        scatter_block<T, NB>(L.X2, true, B, nblk + i, Aval, F.cs, -1);   // C_i, row-major
+        gemm_lds<T, NB>(L.X1, L.X2, a);                                  // W_i = S_i^-1 C_i
+        glb_store_cm<T, NB>(F.W + i * BS, a);
+        lds_fence();
+        lds_store_rm<T, NB>(L.WL, a);
+        scatter_block<T, NB>(L.X1, false, B, 2 * nblk - 1 + i, Aval, F.cs, -1);   // B_i
+        for (int e = lane; e < NB * NB; e += WAVE)
+            F.Bm[i * BS + e] = L.X1[(e / NB) * C::LD + e % NB];
+        scatter_block<T, NB>(L.X2, false, B, i + 1, Aval, F.cs, i + 1 == nblk - 1 ? i + 1 : -1);
+        lds_load_cm<T, NB>(L.X2, a);
+        gemm_lds_sub<T, NB>(L.X1, L.WL, a);                              // D - B_i W_i
+    }
+    wave_sync();
+    // border: E, X_F = T^-1 F, G' = G - E X_F and its LU with partial pivoting
+    const int nbd = B.nbd, nip = B.nip;
+    if (nbd > 0) {
+        const int LE = 3 * nblk - 2, LF = LE + 1, LG = LE + 2;
+        for (int i = lane; i < nbd * nip; i += WAVE) {
+            F.E[i] = T(0);
+            F.XF[i] = T(0);
+        }
+        for (int i = lane; i < nbd * nbd; i += WAVE) F.G[i] = 0.0;
+        wave_sync();
+        for (int e = B.ent_ptr[LE] + lane; e < B.ent_ptr[LE + 1]; e += WAVE)
+            F.E[B.ent_loc[e]] = (T) block_entry(B, Aval, F.cs, e);
+        for (int e = B.ent_ptr[LF] + lane; e < B.ent_ptr[LF + 1]; e += WAVE)
+            F.XF[B.ent_loc[e]] = (T) block_entry(B, Aval, F.cs, e);
+        for (int e = B.ent_ptr[LG] + lane; e < B.ent_ptr[LG + 1]; e += WAVE)
+            F.G[B.ent_loc[e]] = block_entry(B, Aval, F.cs, e);
+        wave_sync();
+        for (int b = 0; b < nbd; ++b) {
+            T *xf = F.XF + (size_t) b * nip;
+            for (int j = lane; j < nip; j += WAVE) L.vec[j] = xf[j];
+            lds_fence();
+            thomas_solve<T, NB>(B, F, L.vec);
+            for (int j = lane; j < nip; j += WAVE) xf[j] = L.vec[j];
+            lds_fence();
+        }
+        wave_sync();
+        for (int b2 = 0; b2 < nbd; ++b2)
+            for (int b1 = 0; b1 < nbd; ++b1) {
+                double s = 0.0;
+                for (int j = lane; j < nip; j += WAVE)
+                    s += (double) F.E[(size_t) b1 * nip + j] * (double) F.XF[(size_t) b2 * nip + j];
+                s = wave_sum(s);
+                if (lane == 0) F.G[b1 + b2 * nbd] -= s;
+            }
+        wave_sync();
+        int gbad = 0;
+        if (lane == 0) {
+            double *G = F.G;
+            for (int k = 0; k < nbd; ++k) {
+                int p = k;
+                double best = fabs(G[k + k * nbd]);
+                for (int r = k + 1; r < nbd; ++r)
+                    if (fabs(G[r + k * nbd]) > best) { best = fabs(G[r + k * nbd]); p = r; }
+                F.gpiv[k] = p;
+                if (!(best > 0.0) || !isfinite(best)) { gbad = 1; break; }
+                if (p != k)
+                    for (int c = 0; c < nbd; ++c) {
+                        const double t = G[k + c * nbd];
+                        G[k + c * nbd] = G[p + c * nbd];
+                        G[p + c * nbd] = t;
+                    }
+                const double inv = 1.0 / G[k + k * nbd];
+                for (int r = k + 1; r < nbd; ++r) G[r + k * nbd] *= inv;
+                for (int c = k + 1; c < nbd; ++c) {
+                    const double u = G[k + c * nbd];
+                    for (int r = k + 1; r < nbd; ++r) G[r + c * nbd] -= G[r + k * nbd] * u;
+                }
+            }
+        }
+        gbad = readlane_i(gbad, 0);
+        wave_sync();
+        if (gbad) return 1;
+    }
+    return 0;
+}
+
+// x = A^-1 b with the factors F (b, x: FP64 [n], reduced order, mass-fraction units)
+template <class T, int NB>
+__device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const BlockLds<T> &L,
+                            const double *b, double *x) {
+    const int lane = lane_id(), nip = B.nip, nbd = B.nbd;
+    NUPPN_LDS T *v = L.vec;
+    for (int j = B.ni + lane; j < nip; j += WAVE) v[j] = T(0);
+    for (int c = lane; c < B.n; c += WAVE) v[B.vpos[c]] = (T) (b[c] / B.mass[c]);
+    lds_fence();
+    thomas_solve<T, NB>(B, F, v);
+    if (nbd > 0) {
+        NUPPN_LDS double *tb = L.bscr;
+        for (int bb = 0; bb < nbd; ++bb) {
+            double s = 0.0;
+            for (int j = lane; j < nip; j += WAVE) s += (double) F.E[(size_t) bb * nip + j] * (double) v[j];
+            s = wave_sum(s);
+            if (lane == 0) tb[bb] = (double) v[nip + bb] - s;
+        }
+        lds_fence();
+        // G' u = t with the row interchanges, unit-lower and upper triangular solves (lane 0)
+        if (lane == 0) {
+            for (int k = 0; k < nbd; ++k) {
+                const int p = F.gpiv[k];
+                if (p != k) { const double t = tb[k]; tb[k] = tb[p]; tb[p] = t; }
+            }
+            for (int k = 0; k < nbd; ++k)
+                for (int r = k + 1; r < nbd; ++r) tb[r] -= F.G[r + k * nbd] * tb[k];
+            for (int k = nbd - 1; k >= 0; --k) {
+                tb[k] /= F.G[k + k * nbd];
+                for (int r = 0; r < k; ++r) tb[r] -= F.G[r + k * nbd] * tb[k];
+            }
+            for (int k = 0; k < nbd; ++k) v[nip + k] = (T) tb[k];
+        }
+        lds_fence();
+        for (int j = lane; j < nip; j += WAVE) {
+            T s = v[j];
+            for (int bb = 0; bb < nbd; ++bb) s = fma_t(-F.XF[(size_t) bb * nip + j], v[nip + bb], s);
+            v[j] = s;
+        }
+        lds_fence();
+    }
+    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[B.vpos[c]] * B.mass[c] * F.cs[c];
+    wave_sync();
+}
+
+// r = rhs - A x over the reduced CSC values (row-wise through the CSR view), FP64
+__device__ inline void block_residual(const DevBlock &B, const double *Aval, const double *rhs,
+                                      const double *x, double *r) {
+    for (int i = lane_id(); i < B.n; i += WAVE) {
+        double s = rhs[i];
+        for (int k = B.csr_ptr[i]; k < B.csr_ptr[i + 1]; ++k) s = fma(-Aval[B.csr_q[k]], x[B.csr_col[k]], s);
+        r[i] = s;
+    }
+    wave_sync();
+}
+
+}  // namespace nuppn

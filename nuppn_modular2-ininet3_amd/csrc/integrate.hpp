@@ -16,6 +16,7 @@ struct DevKnobs {
 struct ZoneStats {
     int nsubt, titers, iter, nvar1, nsucc, nfail;
     long long n_newton, n_factor, n_solve, lu_nnz;
+    long long n_refine, n_factor_fp64;
 };
 
 // Per-slot global scratch (one slot per resident wavefront of the persistent kernel).

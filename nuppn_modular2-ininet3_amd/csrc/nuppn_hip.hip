@@ -26,6 +26,7 @@
 #include "device_common.hpp"
 #include "integrate.hpp"
 #include "zone.hpp"
+#include "batched.hpp"
 #include "symbolic.hpp"
 
 using namespace nuppn;
@@ -76,6 +77,13 @@ __global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
 }
 
 __global__ void flag_reset_kernel(int *flag) { flag[0] = 0; }
+
+namespace nuppn {
+template __global__ void zone_newton_block_kernel<float, 32>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<double, 32>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<float, 64>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<double, 64>(KArgs a, BKArgs b);
+}  // namespace nuppn
 
 // single-state assembly entries (test hooks): mode 0 dX/dt, mode 1 reduced matrix values
 __global__ void __launch_bounds__(64) assemble_kernel(KArgs a, int mode, const double *x,
@@ -177,6 +185,17 @@ struct nuppn_ctx {
     int *d_ierr = nullptr;
     ZoneStats *d_stats = nullptr;
     int *d_status = nullptr;
+    // batched path (block-tridiagonal pivot-free factorisation)
+    BlockSym bsym;
+    int bnb = 0;   // block size, 0 when the network's interior is not block tridiagonal
+    int *d_b_vpos = nullptr, *d_b_diagq = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_q = nullptr;
+    int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_ptr = nullptr;
+    int *d_b_csr_q = nullptr, *d_b_csr_col = nullptr;
+    double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
+    BSlotLayout blayout{};
+    size_t blds32 = 0, blds64 = 0;
+    int bmax_slots = 0, bslots = 0, bgrid64 = 0;
+    char *d_bslots = nullptr;
     hipEvent_t ev[4] = {};
     hipEvent_t evn[2] = {}, evc[2] = {};
     double newton_ms = 0.0, control_ms = 0.0;
@@ -222,6 +241,97 @@ static DevKnobs dev_knobs(const nuppn_knobs &k) {
     K.min_t9_for_slv = k.min_t9_for_slv;
     K.delta_state_max = k.delta_state_max;
     return K;
+}
+
+static DevBlock dev_block(const nuppn_ctx *c) {
+    DevBlock B;
+    const BlockSym &b = c->bsym;
+    B.NB = b.NB;
+    B.nbd = b.nbd;
+    B.ni = b.ni;
+    B.nblk = b.nblk;
+    B.nip = b.nip;
+    B.n = c->sym.n;
+    B.vpos = c->d_b_vpos;
+    B.mass = c->d_b_mass;
+    B.diagq = c->d_b_diagq;
+    B.ent_ptr = c->d_b_ent_ptr;
+    B.ent_q = c->d_b_ent_q;
+    B.ent_loc = c->d_b_ent_loc;
+    B.ent_col = c->d_b_ent_col;
+    B.ent_ys = c->d_b_ent_ys;
+    B.csr_ptr = c->d_b_csr_ptr;
+    B.csr_q = c->d_b_csr_q;
+    B.csr_col = c->d_b_csr_col;
+    return B;
+}
+
+static const void *block_kernel_ptr(int nb, bool fp64) {
+    if (nb == 64)
+        return fp64 ? (const void *) zone_newton_block_kernel<double, 64>
+                    : (const void *) zone_newton_block_kernel<float, 64>;
+    return fp64 ? (const void *) zone_newton_block_kernel<double, 32>
+                : (const void *) zone_newton_block_kernel<float, 32>;
+}
+
+// resident wavefronts per CU of a kernel with `lds` bytes of dynamic LDS
+static int kernel_occupancy(const void *kfn, size_t lds) {
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) !=
+            hipSuccess)
+        return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64, lds) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return per_cu;
+}
+
+// block structure + device upload + occupancy of the batched Newton kernels (once per context)
+static int setup_batched(nuppn_ctx *c) {
+    const int nb = choose_block_size(c->net, c->sym);
+    if (nb == 0) return 0;   // the batched path stays unavailable; the plain path is unaffected
+    if (build_block_symbolic(c->net, c->sym, nb, c->bsym) != 0 || c->bsym.nbd > 8) return 0;
+    const BlockSym &b = c->bsym;
+    if (dev_upload(&c->d_b_vpos, b.vpos) || dev_upload(&c->d_b_diagq, b.diagq) ||
+        dev_upload(&c->d_b_ent_ptr, b.ent_ptr) || dev_upload(&c->d_b_ent_q, b.ent_q) ||
+        dev_upload(&c->d_b_ent_loc, b.ent_loc) || dev_upload(&c->d_b_ent_col, b.ent_col) ||
+        dev_upload(&c->d_b_csr_ptr, b.csr_ptr) || dev_upload(&c->d_b_csr_q, b.csr_q) ||
+        dev_upload(&c->d_b_csr_col, b.csr_col) || dev_upload(&c->d_b_mass, b.mass) ||
+        dev_upload(&c->d_b_ent_ys, b.ent_ys))
+        return NUPPN_ERR_HIP;
+    c->blayout = bslot_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
+                              b.nbd);
+    auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
+    c->blds32 = rnd(nb == 64 ? block_lds_bytes<float, 64>(b.nip, b.nbd)
+                             : block_lds_bytes<float, 32>(b.nip, b.nbd));
+    c->blds64 = rnd(nb == 64 ? block_lds_bytes<double, 64>(b.nip, b.nbd)
+                             : block_lds_bytes<double, 32>(b.nip, b.nbd));
+    if (c->blds64 > 160 * 1024) return 0;
+    const int occ32 = kernel_occupancy(block_kernel_ptr(nb, false), c->blds32);
+    const int occ64 = kernel_occupancy(block_kernel_ptr(nb, true), c->blds64);
+    if (occ32 <= 0 || occ64 <= 0) return 0;
+    int dev = 0, ncu = 0;
+    (void) hipGetDevice(&dev);
+    (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    size_t free_b = 0, total_b = 0;
+    (void) hipMemGetInfo(&free_b, &total_b);
+    const int by_mem = (int) std::max<size_t>(1, (free_b / 4) / c->blayout.bytes);
+    c->bmax_slots = std::max(1, std::min(ncu * std::max(occ32, occ64), by_mem));
+    c->bgrid64 = std::max(1, std::min(ncu * occ64, c->bmax_slots));
+    c->bnb = nb;
+    return 0;
+}
+
+static int ensure_bslots(nuppn_ctx *c, int want) {
+    want = std::max(1, std::min(want, c->bmax_slots));
+    if (want <= c->bslots) return 0;
+    dev_free(c->d_bslots);
+    c->d_bslots = nullptr;
+    c->bslots = 0;
+    HIP_OK(hipMalloc((void **) &c->d_bslots, c->blayout.bytes * (size_t) want));
+    c->bslots = want;
+    return 0;
 }
 
 static int ensure_slots(nuppn_ctx *c, int want) {
@@ -415,6 +525,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     if (hipHostMalloc((void **) &c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess)
         return nullptr;
     if (ensure_slots(c.get(), 1)) return nullptr;
+    if (setup_batched(c.get())) return nullptr;
     *status = NUPPN_OK;
     return c.release();
 }
@@ -432,7 +543,11 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
                     (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
                     (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
-                    (void *) c->d_status})
+                    (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagq,
+                    (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_q, (void *) c->d_b_ent_loc,
+                    (void *) c->d_b_ent_col, (void *) c->d_b_csr_ptr, (void *) c->d_b_csr_q,
+                    (void *) c->d_b_csr_col, (void *) c->d_b_mass, (void *) c->d_b_ent_ys,
+                    (void *) c->d_bslots})
         dev_free(p);
     for (auto &e : c->ev)
         if (e) (void) hipEventDestroy(e);
@@ -570,16 +685,23 @@ int nuppn_lu_solve_reduced(nuppn_ctx *c, const double *csc_val, int nrhs, double
     return info ? NUPPN_ERR_SINGULAR : NUPPN_OK;
 }
 
-int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
-                              const double *t9_1, const double *rho_0, const double *rho_1,
-                              double *ye, const double *dtwant, int *ierr,
-                              nuppn_zone_stats *stats, void *stream) {
-    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
-        return NUPPN_ERR_ARG;
-    std::lock_guard<std::mutex> g(c->mu);
-    hipStream_t s = (hipStream_t) stream;
-    if (ensure_slots(c, nz) || ensure_rate_buffer(c, nz)) return NUPPN_ERR_HIP;
-    if (ensure_zone_state(c, nz)) return NUPPN_ERR_HIP;
+}  // extern "C"
+
+// integrate_network for a batch of zones: rates at the cycle mid-point, then the per-zone state
+// machine (control kernel: sub-step and Newton decisions; Newton kernel: one Newton iteration per
+// zone that needs one) until every zone is done.  `bo` null: the plain path (device GEPP with the
+// reference's pivot order); otherwise the block-factorisation Newton kernels with those options.
+static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, const double *t9_1,
+                     const double *rho_0, const double *rho_1, double *ye, const double *dtwant,
+                     int *ierr, nuppn_zone_stats *stats, const nuppn_batch_opts *bo,
+                     hipStream_t s) {
+    const bool block = bo != nullptr;
+    if (block) {
+        if (ensure_bslots(c, nz)) return NUPPN_ERR_HIP;
+    } else if (ensure_slots(c, nz)) {
+        return NUPPN_ERR_HIP;
+    }
+    if (ensure_rate_buffer(c, nz) || ensure_zone_state(c, nz)) return NUPPN_ERR_HIP;
     const int nre = c->net.nre;
     HIP_OK(hipEventRecord(c->ev[0], s));
     dim3 grid((nre + 255) / 256, nz);
@@ -610,7 +732,21 @@ int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t
     Z.state = c->d_state;
     Z.flag = c->d_flag;
     Z.nz = nz;
-    const int grid_n = std::min(nz, c->nslots);
+    Z.init_prec = (block && bo->factor_precision == 64) ? 1 : 0;
+    BKArgs b;
+    memset(&b, 0, sizeof(b));
+    if (block) {
+        b.B = dev_block(c);
+        b.O.fp64_factors = bo->factor_precision == 64;
+        b.O.max_refine = bo->max_refine;
+        b.O.refine_tol = bo->refine_tol;
+        b.O.accept_tol = bo->accept_tol;
+        b.slot_base = c->d_bslots;
+        b.slot_bytes = c->blayout.bytes;
+        for (int i = 0; i < B_COUNT; ++i) b.off[i] = c->blayout.off[i];
+    }
+    const int grid_n = std::min(nz, block ? c->bslots : c->nslots);
+    const int grid64 = std::min(grid_n, c->bgrid64);
     HIP_OK(hipEventRecord(c->ev[2], s));
     hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), 0, s, a);
     HIP_OK(hipGetLastError());
@@ -640,7 +776,19 @@ int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t
         if (c->h_flag[0] == 0) break;
         if (pass >= max_pass) return NUPPN_ERR_HIP;
         HIP_OK(hipEventRecord(c->evn[0], s));
-        hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
+        if (!block) {
+            hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
+        } else if (c->bnb == 64) {
+            hipLaunchKernelGGL((zone_newton_block_kernel<float, 64>), dim3(grid_n), dim3(64),
+                               c->blds32, s, a, b);
+            hipLaunchKernelGGL((zone_newton_block_kernel<double, 64>), dim3(grid64), dim3(64),
+                               c->blds64, s, a, b);
+        } else {
+            hipLaunchKernelGGL((zone_newton_block_kernel<float, 32>), dim3(grid_n), dim3(64),
+                               c->blds32, s, a, b);
+            hipLaunchKernelGGL((zone_newton_block_kernel<double, 32>), dim3(grid64), dim3(64),
+                               c->blds64, s, a, b);
+        }
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evn[1], s));
         newton_pending = true;
@@ -651,13 +799,27 @@ int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t
     return NUPPN_OK;
 }
 
+extern "C" {
+
+int nuppn_integrate_batch_dev(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                              const double *t9_1, const double *rho_0, const double *rho_1,
+                              double *ye, const double *dtwant, int *ierr,
+                              nuppn_zone_stats *stats, void *stream) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    return run_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, nullptr,
+                     (hipStream_t) stream);
+}
+
 int nuppn_last_timing(nuppn_ctx *c, double *out, int nout) {
     if (!c || !out || nout <= 0) return NUPPN_ERR_ARG;
     HIP_OK(hipEventSynchronize(c->ev[3]));
     float r = 0.f, t = 0.f;
     HIP_OK(hipEventElapsedTime(&r, c->ev[0], c->ev[1]));
     HIP_OK(hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
-    const double vals[6] = {r, t, c->newton_ms, c->control_ms, (double) c->newton_launches,
+    const double nms = c->newton_ms < 0.0 ? (double) t : c->newton_ms;
+    const double vals[6] = {r, t, nms, c->control_ms, (double) c->newton_launches,
                             (double) c->last_passes};
     for (int i = 0; i < nout && i < 6; ++i) out[i] = vals[i];
     return NUPPN_OK;
@@ -683,6 +845,70 @@ int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
     int rc = nuppn_integrate_batch_dev(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
                                        c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr,
                                        (nuppn_zone_stats *) c->d_stats, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(yps, c->d_yps, sizeof(double) * nsp * nz, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ye, c->d_ye, sizeof(double) * nz, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ierr, c->d_ierr, sizeof(int) * nz, hipMemcpyDeviceToHost));
+    if (stats)
+        HIP_OK(hipMemcpy(stats, c->d_stats, sizeof(ZoneStats) * nz, hipMemcpyDeviceToHost));
+    return NUPPN_OK;
+}
+
+void nuppn_default_batch_opts(nuppn_batch_opts *o) {
+    o->factor_precision = 32;
+    o->max_refine = 8;
+    o->refine_tol = 1.0e-13;
+    o->accept_tol = 1.0e-10;
+}
+
+int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {
+    if (!c || !info || ninfo <= 0) return NUPPN_ERR_ARG;
+    const BlockSym &b = c->bsym;
+    const int vals[7] = {c->bnb, b.nblk, b.nbd, b.ni, b.bandwidth, c->bmax_slots, (int) c->blds32};
+    for (int i = 0; i < ninfo && i < 7; ++i) info[i] = vals[i];
+    return c->bnb ? NUPPN_OK : NUPPN_ERR_UNSUPPORTED;
+}
+
+int nuppn_integrate_batch_mixed_dev(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                                    const double *t9_1, const double *rho_0, const double *rho_1,
+                                    double *ye, const double *dtwant, int *ierr,
+                                    nuppn_zone_stats *stats, const nuppn_batch_opts *opts,
+                                    void *stream) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    if (!c->bnb) return NUPPN_ERR_UNSUPPORTED;
+    nuppn_batch_opts o;
+    if (opts) o = *opts;
+    else nuppn_default_batch_opts(&o);
+    if ((o.factor_precision != 32 && o.factor_precision != 64) || o.max_refine < 1)
+        return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    return run_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, &o,
+                     (hipStream_t) stream);
+}
+
+int nuppn_integrate_batch_mixed(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                                const double *t9_1, const double *rho_0, const double *rho_1,
+                                double *ye, const double *dtwant, int *ierr,
+                                nuppn_zone_stats *stats, const nuppn_batch_opts *opts) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
+    }
+    const size_t nsp = c->net.nsp;
+    HIP_OK(hipMemcpy(c->d_yps, yps, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_0, t9_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_1, t9_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_0, rho_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_ye, ye, sizeof(double) * nz, hipMemcpyHostToDevice));
+    int rc = nuppn_integrate_batch_mixed_dev(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
+                                             c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr,
+                                             (nuppn_zone_stats *) c->d_stats, opts, nullptr);
     if (rc) return rc;
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(yps, c->d_yps, sizeof(double) * nsp * nz, hipMemcpyDeviceToHost));

@@ -243,3 +243,139 @@ int superlu_column_order(int n, const int *colptr, const int *rowind, std::vecto
 }
 
 }  // namespace nuppn
+
+// ------------------------------------------------------------------------------------------
+// Block structure for the pivot-free batched factorisation (see symbolic.hpp).
+
+namespace nuppn {
+
+namespace {
+
+// border: reduced species whose row + column entry count exceeds 8x the median count
+std::vector<char> border_species(const Symbolic &s) {
+    const int n = s.n;
+    std::vector<int> deg(n, 0);
+    for (int c = 0; c < n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
+            deg[c]++;
+            deg[s.A_rowind[p]]++;
+        }
+    std::vector<int> sorted(deg);
+    std::nth_element(sorted.begin(), sorted.begin() + n / 2, sorted.end());
+    const int med = std::max(1, sorted[n / 2]);
+    std::vector<char> isb(n, 0);
+    for (int i = 0; i < n; ++i) isb[i] = deg[i] > 8 * med;
+    return isb;
+}
+
+int interior_bandwidth(const Symbolic &s, const std::vector<int> &pos, int ni) {
+    int bw = 0;
+    for (int c = 0; c < s.n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
+            const int pr = pos[s.A_rowind[p]], pc = pos[c];
+            if (pr < ni && pc < ni) bw = std::max(bw, std::abs(pr - pc));
+        }
+    return bw;
+}
+
+}  // namespace
+
+int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, BlockSym &b) {
+    const int n = s.n;
+    const std::vector<char> isb = border_species(s);
+    b.NB = NB;
+    b.ord.clear();
+    for (int i = 0; i < n; ++i)
+        if (!isb[i]) b.ord.push_back(i);
+    b.ni = (int) b.ord.size();
+    for (int i = 0; i < n; ++i)
+        if (isb[i]) b.ord.push_back(i);
+    b.nbd = n - b.ni;
+    b.pos.assign(n, 0);
+    for (int p = 0; p < n; ++p) b.pos[b.ord[p]] = p;
+    b.bandwidth = interior_bandwidth(s, b.pos, b.ni);
+    if (b.bandwidth > NB || b.ni == 0) return -1;
+    b.nblk = (b.ni + NB - 1) / NB;
+    b.nip = b.nblk * NB;
+    b.mass.resize(n);
+    for (int i = 0; i < n; ++i) b.mass[i] = net.aw[s.red2sp[i]];
+    b.diagq.assign(n, -1);
+    for (int c = 0; c < n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p)
+            if (s.A_rowind[p] == c) b.diagq[c] = p;
+    for (int c = 0; c < n; ++c)
+        if (b.diagq[c] < 0 || b.mass[c] <= 0.0) return -2;
+
+    const int nblk = b.nblk, nlists = 3 * nblk - 2 + 3;
+    const int LE = 3 * nblk - 2, LF = LE + 1, LG = LE + 2;
+    std::vector<std::vector<int>> q(nlists), loc(nlists);
+    for (int c = 0; c < n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
+            const int r = s.A_rowind[p];
+            const int pr = b.pos[r], pc = b.pos[c];
+            int list, l;
+            if (pr < b.ni && pc < b.ni) {
+                const int bi = pr / NB, bj = pc / NB, lr = pr % NB, lc = pc % NB;
+                if (bi == bj) list = bi;
+                else if (bj == bi + 1) list = nblk + bi;
+                else if (bi == bj + 1) list = 2 * nblk - 1 + bj;
+                else return -1;
+                l = lr + lc * NB;
+            } else if (pr >= b.ni && pc < b.ni) {
+                list = LE;
+                l = (pr - b.ni) * b.nip + pc;
+            } else if (pr < b.ni) {
+                list = LF;
+                l = (pc - b.ni) * b.nip + pr;
+            } else {
+                list = LG;
+                l = (pr - b.ni) + (pc - b.ni) * b.nbd;
+            }
+            q[list].push_back(p);
+            loc[list].push_back(l);
+        }
+    b.ent_ptr.assign(1, 0);
+    b.ent_q.clear();
+    b.ent_loc.clear();
+    b.ent_col.clear();
+    b.ent_ys.clear();
+    std::vector<int> colof(s.nnzA);
+    for (int c = 0; c < n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) colof[p] = c;
+    for (int L = 0; L < nlists; ++L) {
+        for (size_t e = 0; e < q[L].size(); ++e) {
+            const int p = q[L][e], c = colof[p], r = s.A_rowind[p];
+            b.ent_q.push_back(p);
+            b.ent_loc.push_back(loc[L][e]);
+            b.ent_col.push_back(c);
+            b.ent_ys.push_back(b.mass[c] / b.mass[r]);
+        }
+        b.ent_ptr.push_back((int) b.ent_q.size());
+    }
+    b.vpos.assign(n, 0);
+    for (int i = 0; i < n; ++i) b.vpos[i] = b.pos[i] < b.ni ? b.pos[i] : b.nip + b.pos[i] - b.ni;
+    // CSR view for the residual r = rhs - A x
+    b.csr_ptr.assign(n + 1, 0);
+    for (int p = 0; p < s.nnzA; ++p) b.csr_ptr[s.A_rowind[p] + 1]++;
+    for (int i = 0; i < n; ++i) b.csr_ptr[i + 1] += b.csr_ptr[i];
+    b.csr_q.assign(s.nnzA, 0);
+    b.csr_col.assign(s.nnzA, 0);
+    std::vector<int> fill(b.csr_ptr.begin(), b.csr_ptr.end() - 1);
+    for (int c = 0; c < n; ++c)
+        for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
+            const int k = fill[s.A_rowind[p]]++;
+            b.csr_q[k] = p;
+            b.csr_col[k] = c;
+        }
+    return 0;
+}
+
+int choose_block_size(const HostNetwork &net, const Symbolic &s) {
+    for (int NB : {32, 64}) {
+        BlockSym b;
+        if (build_block_symbolic(net, s, NB, b) == 0) return NB;
+    }
+    return 0;
+}
+
+}  // namespace nuppn

@@ -32,6 +32,45 @@ struct Symbolic {
 // builds everything but the column order; returns 0 or a negative error
 int build_symbolic(const HostNetwork &net, Symbolic &s);
 
+// Static block structure of the reduced matrix for the pivot-free batched factorisation.
+//
+// Species that couple to far more species than the median (the light projectiles n, p, alpha of
+// a capture network) form the border and are ordered last; the other (interior) species keep the
+// network order.  In that order the interior part of I - dt J is banded; with blocks of NB >= the
+// half-bandwidth it is block tridiagonal:
+//     A = [ D_0  C_0              F_0 ]
+//         [ B_0  D_1  C_1         F_1 ]
+//         [      B_1  D_2  ...    ... ]
+//         [ E_0  E_1  ...         G   ]
+// Every CSC entry of the reduced pattern is assigned to one dense block (or to E, F, G) with its
+// position inside it; the device builds the dense blocks from these lists.
+struct BlockSym {
+    int NB = 0, nbd = 0, ni = 0, nblk = 0, nip = 0, bandwidth = 0;
+    std::vector<int> pos;       // [n] reduced index -> ordered position (interior, then border)
+    std::vector<int> ord;       // [n] ordered position -> reduced index
+    std::vector<int> vpos;      // [n] reduced index -> slot in a padded vector (interior
+                                //     positions 0..ni-1, border nip..nip+nbd-1)
+    std::vector<double> mass;   // [n] mass number of each reduced species
+    std::vector<int> diagq;     // [n] CSC index of the diagonal entry of each reduced column
+    // entry lists: lists 0..nblk-1 = D_i, nblk..2nblk-2 = C_i (block row i, block col i+1),
+    // 2nblk-1..3nblk-3 = B_i (block row i+1, block col i), then E, F, G (3 lists)
+    std::vector<int> ent_ptr;   // [3*nblk - 2 + 3 + 1]
+    std::vector<int> ent_q;     // CSC entry index
+    std::vector<int> ent_loc;   // D/C/B: r_local + c_local*NB; E: b*nip + j; F: b*nip + j; G: r + c*nbd
+    std::vector<int> ent_col;   // reduced column index
+    std::vector<double> ent_ys; // mass(col) / mass(row): X units -> molar (Y) units
+    // CSR view of the CSC pattern for the residual: per reduced row, (CSC entry, column)
+    std::vector<int> csr_ptr, csr_q, csr_col;
+};
+
+// Builds the block structure for block size NB (32 or 64); returns 0, or -1 when the interior
+// half-bandwidth exceeds NB (the block-tridiagonal form does not hold).
+int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, BlockSym &b);
+
+// smallest supported block size (32, 64) for which the network's interior is block tridiagonal;
+// 0 if none
+int choose_block_size(const HostNetwork &net, const Symbolic &s);
+
 // COLAMD (v2.3 semantics, SuperLU_5.0/SRC/colamd.c) on an m x n CSC pattern, default knobs;
 // p receives the column ordering: p[k] = column eliminated k-th.  Returns 0 or an error.
 int colamd_order(int m, int n, const int *colptr, const int *rowind, std::vector<int> &p);

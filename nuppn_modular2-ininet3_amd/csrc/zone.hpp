@@ -60,6 +60,7 @@ struct ZoneData {
     ZoneState *state;
     int *flag;          // [0] set when some zone needs a Newton iteration
     int nz;
+    int init_prec;      // factor precision a zone starts with on the batched path (0 FP32, 1 FP64)
 };
 
 struct KArgs {
@@ -154,6 +155,7 @@ __device__ inline void zone_init(const KArgs &a, int z) {
     if (lane_id() == 0) {
         ZoneState s = {};
         s.phase = PH_SUBSTEP;
+        s.pad0 = Z.init_prec;   // factor precision of the batched path
         s.i = 0;
         const double t9_0 = Z.t9_0[z], t9_1 = Z.t9_1[z], rho_0 = Z.rho_0[z], rho_1 = Z.rho_1[z];
         s.dttry = Z.dtwant[z];

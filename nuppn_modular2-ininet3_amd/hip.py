@@ -41,7 +41,14 @@ class Knobs(C.Structure):
 class ZoneStats(C.Structure):
     _fields_ = [("nsubt", C.c_int), ("titers", C.c_int), ("iter", C.c_int), ("nvar1", C.c_int),
                 ("nsucc", C.c_int), ("nfail", C.c_int), ("n_newton", C.c_longlong),
-                ("n_factor", C.c_longlong), ("n_solve", C.c_longlong), ("lu_nnz", C.c_longlong)]
+                ("n_factor", C.c_longlong), ("n_solve", C.c_longlong), ("lu_nnz", C.c_longlong),
+                ("n_refine", C.c_longlong), ("n_factor_fp64", C.c_longlong)]
+
+
+class BatchOpts(C.Structure):
+    """nuppn_batch_opts: factor precision (32 / 64) and the FP64 refinement tolerances."""
+    _fields_ = [("factor_precision", C.c_int), ("max_refine", C.c_int),
+                ("refine_tol", C.c_double), ("accept_tol", C.c_double)]
 
 
 class Network(C.Structure):
@@ -54,7 +61,7 @@ class Network(C.Structure):
 STATS_DTYPE = np.dtype([("nsubt", np.int32), ("titers", np.int32), ("iter", np.int32),
                         ("nvar1", np.int32), ("nsucc", np.int32), ("nfail", np.int32),
                         ("n_newton", np.int64), ("n_factor", np.int64), ("n_solve", np.int64),
-                        ("lu_nnz", np.int64)])
+                        ("lu_nnz", np.int64), ("n_refine", np.int64), ("n_factor_fp64", np.int64)])
 
 _lib = None
 
@@ -66,6 +73,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    # torch ships its own HIP runtime under the same soname; importing it first makes this
+    # library bind to that one, so device pointers from torch tensors and this library's kernels
+    # share one runtime in the process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.nuppn_default_knobs.argtypes = [C.POINTER(Knobs)]
     L.nuppn_column_order.argtypes = [C.c_int, _ip, _ip, _ip, C.c_void_p]
@@ -88,6 +102,12 @@ def load():
                                         _ip, C.c_void_p]
     L.nuppn_integrate_batch_dev.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
     L.nuppn_last_timing.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.nuppn_default_batch_opts.argtypes = [C.POINTER(BatchOpts)]
+    L.nuppn_batch_block_info.argtypes = [C.c_void_p, _ip, C.c_int]
+    L.nuppn_integrate_batch_mixed.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                              _dp, _ip, C.c_void_p, C.POINTER(BatchOpts)]
+    L.nuppn_integrate_batch_mixed_dev.argtypes = ([C.c_void_p, C.c_int] + [C.c_void_p] * 9 +
+                                                  [C.POINTER(BatchOpts), C.c_void_p])
     L.slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                             C.POINTER(C.c_int)]
     L.c_fortran_dgssv_.argtypes = [C.POINTER(C.c_int)] * 4 + [_dp, _ip, _ip, _dp,
@@ -104,6 +124,14 @@ def default_knobs(**over) -> Knobs:
     for key, val in over.items():
         setattr(k, key, val)
     return k
+
+
+def batch_opts(**over) -> BatchOpts:
+    o = BatchOpts()
+    load().nuppn_default_batch_opts(C.byref(o))
+    for key, val in over.items():
+        setattr(o, key, val)
+    return o
 
 
 def _f64(a):
@@ -244,6 +272,39 @@ class Context:
                                               dtwant, ierr, stats, stream)
         if rc != OK:
             raise RuntimeError(f"nuppn_integrate_batch_dev failed ({rc})")
+
+    def block_info(self):
+        """dict of the batched path's block structure (nuppn_batch_block_info)."""
+        info = np.zeros(7, np.int32)
+        rc = load().nuppn_batch_block_info(self.h, info, 7)
+        keys = ("NB", "nblk", "nbd", "ni", "bandwidth", "slots", "lds_bytes")
+        d = dict(zip(keys, info.tolist()))
+        d["available"] = rc == OK
+        return d
+
+    def integrate_batch_mixed(self, yps, t9_0, t9_1, rho_0, rho_1, dtwant, ye=None, opts=None):
+        """The batched path (block factorisation + FP64 refinement) on host arrays."""
+        y = _f64(yps).copy()
+        nz = y.shape[0]
+        ye = np.full(nz, 0.5) if ye is None else _f64(ye).copy()
+        ierr = np.zeros(nz, np.int32)
+        stats = np.zeros(nz, STATS_DTYPE)
+        o = opts if opts is not None else batch_opts()
+        rc = load().nuppn_integrate_batch_mixed(self.h, nz, y, _f64(t9_0), _f64(t9_1),
+                                                _f64(rho_0), _f64(rho_1), ye, _f64(dtwant), ierr,
+                                                stats.ctypes.data_as(C.c_void_p), C.byref(o))
+        if rc != OK:
+            raise RuntimeError(f"nuppn_integrate_batch_mixed failed ({rc})")
+        return ierr, y, ye, stats
+
+    def integrate_batch_mixed_dev(self, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr,
+                                  stats, opts=None, stream=0):
+        """Device-pointer variant of the batched path."""
+        o = opts if opts is not None else batch_opts()
+        rc = load().nuppn_integrate_batch_mixed_dev(self.h, nz, yps, t9_0, t9_1, rho_0, rho_1, ye,
+                                                    dtwant, ierr, stats, C.byref(o), stream)
+        if rc != OK:
+            raise RuntimeError(f"nuppn_integrate_batch_mixed_dev failed ({rc})")
 
     def last_timing(self):
         """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes"""
