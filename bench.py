@@ -1,15 +1,18 @@
 """Benchmark: zone-timesteps/s of one mppnp-style cycle on one MI355X.
 
 Workload: 2,048 independent zones sharing the synthetic 1,100-species network (7,128 reactions),
-T9 log-uniform 0.05..3, rho 1e0..1e8, one cycle of dt = 1e4 s (backward-Euler Newton with the
-device GEPP, FP64).  A step is one cycle over all zones: the rate kernel, then the sub-step /
-Newton pipeline until every zone is done.  Inputs are resident in HBM when the timed region
-starts; each step restores the initial abundances with a device-to-device copy.
+T9 log-uniform 0.05..3, rho 1e0..1e8, one cycle of dt = 1e4 s (backward-Euler Newton).  A step is
+one cycle over all zones: the rate kernel, then the sub-step / Newton pipeline until every zone
+is done.  Inputs are resident in HBM when the timed region starts; each step restores the initial
+abundances with a device-to-device copy.
 
-Prints one JSON line (contract of the driver).  `roofline` is for the dominant kernel (the
-per-zone Newton iteration: Jacobian, rhs, GEPP, solve) against HBM bandwidth using algorithmic
-bytes; `cpu_baseline` times the CPU restatement of the same path (oracle driver + the reference
-SuperLU 5.0 compiled from source) on a bounded sample of the same zones on the host cores.
+`value` is the batched mixed-precision path (block factorisation in FP32, FP64 refinement, FP64
+fallback per zone).  Beside it: the same path with FP64 factors, and the plain FP64 path (device
+GEPP in the reference's pivot order) on a bounded sample of zones, timed together with the
+batched path on the same sample.  `roofline` is for the dominant kernels (the block Newton
+kernels), algorithmic bytes per launch over the HIP-event launch time; `cpu_baseline` times the
+oracle integrator with the reference SuperLU 5.0 built from source on a bounded sample of the same
+zones on the host cores.
 """
 import argparse
 import json
@@ -28,45 +31,65 @@ HBM_PEAK_GBS = 8000.0
 def _args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=1)
-    p.add_argument("--warmup", type=int, default=0)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--zones", type=int, default=2048)
+    p.add_argument("--plain-sample", type=int, default=48,
+                   help="zones (T9 < 0.25) on which the plain FP64 path is timed")
     p.add_argument("--cpu-sample", type=int, default=32, help="zones in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
     return p.parse_args()
 
 
-def algorithmic_bytes(net, n, nnzA, stats):
-    """Bytes per Newton iteration summed over all iterations of the step (definitions of the
-    measurement plan): Jacobian assembly 8R + 8N + 8 nnz(A); rhs 8R + 16N; LU 12 nnz(L+U)
-    (value + index); solve 8 nnz(L+U) + 16N."""
-    R = net.nre
-    iters = int(stats["n_newton"].sum())
-    lu = int(stats["lu_nnz"].sum())
-    per_iter = (8 * R + 8 * n + 8 * nnzA) + (8 * R + 16 * n) + 16 * n
-    return iters * per_iter + 20 * lu, iters, lu
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, sample, cores):
-    """Oracle driver + reference SuperLU on `sample` zones spread over the batch, one process per
-    core (the reference's one-zone-per-worker model)."""
+def block_bytes(net, ctx, st):
+    """Algorithmic HBM bytes of the block Newton kernels over one step, from the per-zone
+    counters: per Newton iteration the Jacobian assembly (8R + 8N + 8 nnz(A)) and the rhs
+    (8R + 16N); per factorisation the stored factors (3 nblk NB^2 + 2 nbd nip entries, 4 or 8
+    bytes) plus reading A (8 nnz(A)); per solve the factors again plus 16N; per refinement
+    residual 8 nnz(A) + 16N."""
+    info = ctx.block_info()
+    R, N, nnzA = net.nre, ctx.n, ctx.nnzA
+    fac_entries = 3 * info["nblk"] * info["NB"] ** 2 + 2 * info["nbd"] * (info["nblk"] * info["NB"])
+    nf = st["n_factor"].astype(np.int64)
+    nf64 = st["n_factor_fp64"].astype(np.int64)
+    nf32 = nf - nf64
+    # solves are attributed to the precision of the zone's factorisations
+    frac64 = np.where(nf > 0, nf64 / np.maximum(nf, 1), 0.0)
+    ns = st["n_solve"].astype(np.float64)
+    per_newton = (8 * R + 8 * N + 8 * nnzA) + (8 * R + 16 * N)
+    total = (st["n_newton"].sum() * per_newton
+             + (nf32.sum() * 4 + nf64.sum() * 8) * fac_entries + nf.sum() * 8 * nnzA
+             + (ns * (frac64 * 8 + (1 - frac64) * 4)).sum() * fac_entries + ns.sum() * 16 * N
+             + st["n_refine"].sum() * (8 * nnzA + 16 * N))
+    return float(total)
+
+
+def cpu_baseline(sample_idx, cores):
+    """Oracle integrate_network + reference SuperLU on the sampled zones, one process per core
+    (the reference's one-zone-per-worker model)."""
     from concurrent.futures import ProcessPoolExecutor
-    idx = np.linspace(0, w.nz - 1, sample).round().astype(int)
     t = time.time()
-    with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_init, initargs=(w.name,)) as ex:
-        results = list(ex.map(_cpu_zone, idx.tolist()))
+    with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_init) as ex:
+        results = list(ex.map(_cpu_zone, [int(i) for i in sample_idx]))
     dt = time.time() - t
     ok = sum(1 for r in results if r == 0)
-    return {"value": len(idx) / dt, "unit": "zone-timesteps/s", "cores": cores, "kind": "port",
-            "sample": f"{len(idx)} of the {w.nz} zones (every ~{w.nz // len(idx)}th), one cycle "
-                      f"each; oracle integrate_network + reference SuperLU 5.0 built from source; "
+    return {"value": len(sample_idx) / dt, "unit": "zone-timesteps/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"{len(sample_idx)} of the 2,048 zones (every ~{2048 // len(sample_idx)}th), "
+                      f"one cycle each; oracle integrate_network driving the reference SuperLU 5.0 "
+                      f"built from source (slu_solve per Newton iteration, as linalg.F90:115); "
                       f"{ok} ok; {dt:.1f} s wall"}
 
 
 _CPU = {}
 
 
-def _cpu_init(name):
+def _cpu_init():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from __graft_entry__ import _load_package
@@ -87,8 +110,9 @@ def _cpu_zone(z):
 def main():
     a = _args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path shards "
-              f"zones across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path shards "
+              f"zones across GPUs (one contiguous block of zones per GPU), which this one-GPU "
+              f"build does not run", file=sys.stderr)
         sys.exit(2)
     import torch
     from __graft_entry__ import _load_package
@@ -99,57 +123,94 @@ def main():
     ctx = hip.Context(net)
     dev = torch.device("cuda:0")
     nz = w.nz
-    x0 = torch.tensor(w.x0, dtype=torch.float64, device=dev)
+    tens = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device=dev)
+    x0 = tens(w.x0)
     yps = x0.clone()
-    t9_0 = torch.tensor(w.t9[:, 0], dtype=torch.float64, device=dev)
-    t9_1 = torch.tensor(w.t9[:, 1], dtype=torch.float64, device=dev)
-    r0 = torch.tensor(w.rho[:, 0], dtype=torch.float64, device=dev)
-    r1 = torch.tensor(w.rho[:, 1], dtype=torch.float64, device=dev)
-    dt = torch.tensor(w.dt, dtype=torch.float64, device=dev)
+    t9_0, t9_1, r0, r1, dt = (tens(x) for x in (w.t9[:, 0], w.t9[:, 1], w.rho[:, 0], w.rho[:, 1],
+                                                 w.dt))
     ye = torch.full((nz,), 0.5, dtype=torch.float64, device=dev)
     ierr = torch.zeros(nz, dtype=torch.int32, device=dev)
-    stats = torch.zeros((nz, hip.STATS_DTYPE.itemsize // 8), dtype=torch.int64, device=dev)
+    nstat = hip.STATS_DTYPE.itemsize // 8
+    stats = torch.zeros((nz, nstat), dtype=torch.int64, device=dev)
 
-    def step():
-        yps.copy_(x0)
+    def run(path, sel=None):
+        """one step (cycle) of `path` over all zones, or over the zone subset `sel`"""
+        if sel is None:
+            y, a0, a1, b0, b1, d, e, ie, s, n = yps, t9_0, t9_1, r0, r1, dt, ye, ierr, stats, nz
+            y.copy_(x0)
+        else:
+            n = len(sel)
+            idx = torch.tensor(sel, device=dev)
+            y = x0[idx].clone()
+            a0, a1, b0, b1, d = (t[idx].contiguous() for t in (t9_0, t9_1, r0, r1, dt))
+            e = torch.full((n,), 0.5, dtype=torch.float64, device=dev)
+            ie = torch.zeros(n, dtype=torch.int32, device=dev)
+            s = torch.zeros((n, nstat), dtype=torch.int64, device=dev)
         torch.cuda.synchronize()
-        ctx.integrate_batch_dev(nz, yps.data_ptr(), t9_0.data_ptr(), t9_1.data_ptr(),
-                                r0.data_ptr(), r1.data_ptr(), ye.data_ptr(), dt.data_ptr(),
-                                ierr.data_ptr(), stats.data_ptr(), 0)
+        t0 = time.perf_counter()
+        ptrs = [t.data_ptr() for t in (y, a0, a1, b0, b1, e, d, ie, s)]
+        if path == "plain":
+            ctx.integrate_batch_dev(n, *ptrs, 0)
+        else:
+            ctx.integrate_batch_mixed_dev(n, *ptrs, opts=hip.batch_opts(
+                factor_precision=32 if path == "mixed" else 64))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        st = np.frombuffer(s.cpu().numpy().tobytes(), dtype=hip.STATS_DTYPE)
+        return el, y.cpu().numpy(), ie.cpu().numpy(), st, ctx.last_timing()
 
+    # --- timed: the batched mixed path over all zones (the workload's metric)
     for i in range(a.warmup):
-        step()
-        torch.cuda.synchronize()
-        print(f"warmup step {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
+        el, *_ = run("mixed")
+        log(f"warmup step {i + 1}/{a.warmup}: {el * 1e3:.1f} ms")
+    elapsed = 0.0
     timings = []
-    t0 = time.perf_counter()
     for i in range(a.steps):
-        step()
-        timings.append(ctx.last_timing())
-        print(f"step {i + 1}/{a.steps}: {timings[-1]['total_ms']:.1f} ms", file=sys.stderr,
-              flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = np.frombuffer(stats.cpu().numpy().tobytes(), dtype=hip.STATS_DTYPE)
-    ie = ierr.cpu().numpy()
-    y = yps.cpu().numpy()
-    sums = np.abs(y.sum(axis=1) - 1.0)
-
-    # dominant kernel: the Newton-iteration kernel, HIP events around each launch
+        el, y_mixed, ie_mixed, st_mixed, tm = run("mixed")
+        elapsed += el
+        timings.append(tm)
+        log(f"step {i + 1}/{a.steps}: {el * 1e3:.1f} ms")
+    value = nz * a.steps / elapsed
     tm = timings[-1]
-    bytes_step, iters, lu = algorithmic_bytes(net, ctx.n, ctx.nnzA, st)
-    newton_s = tm["newton_ms"] / 1e3
+    bytes_step = block_bytes(net, ctx, st_mixed)
     launches = max(1, int(tm["newton_launches"]))
+    newton_s = tm["newton_ms"] / 1e3
     avg_launch_s = newton_s / launches
     achieved = (bytes_step / launches) / avg_launch_s / 1e9 if newton_s > 0 else 0.0
+
+    # --- the same path with FP64 factors (accuracy reference and throughput)
+    el64, y64, ie64, st64, _ = run("fp64")
+    log(f"fp64-factor step: {el64 * 1e3:.1f} ms")
+    m = (y64 > 1e-20) | (y_mixed > 1e-20)
+    err_mixed_vs_fp64 = float(np.max(np.abs(y_mixed[m] - y64[m]) / np.maximum(y_mixed[m], y64[m])))
+
+    # --- plain FP64 path (reference pivot order) on a bounded sample of light zones
+    plain = None
+    if not a.no_plain:
+        light = np.nonzero(w.t9[:, 0] < 0.25)[0]
+        sel = light[np.linspace(0, light.size - 1, min(a.plain_sample, light.size)).round()
+                    .astype(int)]
+        elp, yp, iep, stp, _ = run("plain", sel)
+        elb, yb, ieb, stb, _ = run("mixed", sel)
+        mm = (yp > 1e-20) | (yb > 1e-20)
+        plain = {"zones": int(len(sel)), "selection": "T9 < 0.25 (the plain path's per-iteration "
+                 "latency makes hot zones take minutes)",
+                 "plain_zone_timesteps_per_s": len(sel) / elp,
+                 "mixed_zone_timesteps_per_s": len(sel) / elb, "mixed_over_plain": elp / elb,
+                 "max_rel_err_mixed_vs_plain": float(np.max(np.abs(yp[mm] - yb[mm]) /
+                                                            np.maximum(yp[mm], yb[mm]))),
+                 "same_decisions": bool(np.array_equal(stp["nsubt"], stb["nsubt"]) and
+                                        np.array_equal(stp["n_newton"], stb["n_newton"]))}
+        log(f"plain sample: {elp * 1e3:.1f} ms vs mixed {elb * 1e3:.1f} ms on {len(sel)} zones")
 
     cpu = None
     if not a.no_cpu:
         cores = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(w, min(a.cpu_sample, nz), cores)
+        idx = np.linspace(0, nz - 1, min(a.cpu_sample, nz)).round().astype(int)
+        cpu = cpu_baseline(idx, cores)
+        log(f"cpu baseline: {cpu['value']:.2f} zone-timesteps/s on {cores} cores")
 
-    value = nz * a.steps / elapsed
+    sums = np.abs(y_mixed.sum(axis=1) - 1.0)
     out = {
         "metric": "zone-timesteps/s @1,100 species, mixed vs FP64 GPU, at <=1e-8 rel err",
         "value": value,
@@ -161,23 +222,34 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 factors + f64 state/residual/refinement",
         "data": "synthetic (seeded generator of the 2,048-zone x 1,100-species workload)",
         "config": {"workload": "mppnp-style batch: 2,048 zones x 1,100 species (7,128 reactions), "
-                               "one cycle dt=1e4 s, backward Euler + device GEPP, plain FP64",
-                   "zones": nz, "species": net.nsp, "reactions": net.nre, "path": "plain-fp64"},
+                               "one cycle dt=1e4 s, backward Euler, batched block factorisation "
+                               "FP32 + FP64 refinement",
+                   "zones": nz, "species": net.nsp, "reactions": net.nre, "path": "mixed"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "zone_newton_kernel", "launches": launches,
-                     "avg_launch_ms": 1e3 * avg_launch_s},
+                     "kernel": "zone_newton_block_kernel<float,32> + <double,32> (per pass)",
+                     "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s},
         "cpu_baseline": cpu,
-        "detail": {"newton_iterations": iters, "lu_nnz_total": lu,
-                   "max_newton_per_zone": int(st["n_newton"].max()),
-                   "max_substeps_per_zone": int(st["nsubt"].max()),
-                   "ierr_nonzero": int((ie != 0).sum()), "max_abs_sumX_minus_1": float(sums.max()),
-                   "rates_ms": tm["rates_ms"], "pipeline_ms": tm["total_ms"],
-                   "newton_ms": tm["newton_ms"], "control_ms": tm["control_ms"],
-                   "passes": tm["passes"]},
+        "detail": {
+            "fp64_factor_path_zone_timesteps_per_s": nz / el64,
+            "max_rel_err_mixed_vs_fp64_factors": err_mixed_vs_fp64,
+            "max_abs_sumX_minus_1": float(sums.max()),
+            "ierr_nonzero": int((ie_mixed != 0).sum()),
+            "newton_iterations": int(st_mixed["n_newton"].sum()),
+            "max_newton_per_zone": int(st_mixed["n_newton"].max()),
+            "factorisations": int(st_mixed["n_factor"].sum()),
+            "fp64_factorisations": int(st_mixed["n_factor_fp64"].sum()),
+            "zones_on_fp64": int((st_mixed["n_factor_fp64"] > 0).sum()),
+            "solves": int(st_mixed["n_solve"].sum()),
+            "refinement_residuals": int(st_mixed["n_refine"].sum()),
+            "rates_ms": tm["rates_ms"], "pipeline_ms": tm["total_ms"],
+            "newton_ms": tm["newton_ms"], "control_ms": tm["control_ms"],
+            "passes": tm["passes"], "algorithmic_bytes_per_step": bytes_step,
+            "plain_sample": plain,
+        },
     }
     print(json.dumps(out))
 

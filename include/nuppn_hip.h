@@ -150,12 +150,14 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
 
 /* Options of the batched path (nuppn_integrate_batch_mixed): the reduced system is factored
  * without pivoting in a block-tridiagonal + border form (molar units, columns scaled by the
- * diagonal), in FP32 (factor_precision 32) or FP64 (64); every solve is refined in FP64 against
- * the FP64 matrix until max|correction| <= refine_tol * max|x|.  When FP32 refinement stalls above
- * accept_tol the zone switches to FP64 factors. */
+ * diagonal), in FP32 (factor_precision 32) or FP64 (64).  With FP32 factors every solve is
+ * refined in FP64 against the FP64 matrix until each correction satisfies |d_i| <= refine_tol *
+ * (|x_i| + |y_i| + 1e-30), y the current abundances; when that refinement stalls above accept_tol
+ * the zone switches to FP64 factors (direct solve plus one refinement step) for the rest of the
+ * call. */
 typedef struct {
     int factor_precision;   /* 32 (default) or 64 */
-    int max_refine;         /* refinement steps per solve (8) */
+    int max_refine;         /* refinement steps per solve with FP32 factors (8) */
     double refine_tol;      /* 1e-13 */
     double accept_tol;      /* 1e-10 */
 } nuppn_batch_opts;

@@ -7,12 +7,16 @@
 // take their decisions with one implementation; only the linear solve differs:
 //     x_0 = 0, r_0 = rhs;  d_k = (LU)^-1 r_k (factors in precision T),  x_{k+1} = x_k + d_k,
 //     r_{k+1} = rhs - A x_{k+1} (FP64, the assembled matrix)
-// stopping when max|d_k| <= refine_tol * max|x_{k+1}|.  When the corrections stop shrinking
-// (a contraction worse than 1/2 per step, i.e. cond(A) * eps(T) is not small) above accept_tol
-// with FP32 factors, the zone is marked for FP64 factors (for the rest of the call) and left in
-// the Newton phase: the FP64 kernel, launched after the FP32 one in the same pass, redoes the
-// iteration.  With FP64 factors a stalled refinement is accepted, as the reference accepts its
-// single GEPP solve.
+// stopping when every correction is small against its own species, |d_i| <= refine_tol *
+// (|x_i| + |y_i| + 1e-30) with y the current abundances (a normwise test would leave the
+// abundances of rare species, 20 orders of magnitude below the dominant ones, unconverged; the
+// 1e-30 floor ignores species far below any reported abundance).
+// When the corrections stop shrinking (a contraction worse than 1/2 per step) with FP32 factors,
+// the solve is accepted if an earlier step contracted strongly (the stall is then the FP64
+// residual's rounding floor); otherwise cond(A) * eps(T) is not small and the zone is marked for
+// FP64 factors (for the rest of the call) and left in the Newton phase: the FP64 kernel,
+// launched after the FP32 one in the same pass, redoes the iteration.  FP64 factors give the direct solve plus one
+// refinement step, accepted as the reference accepts its single GEPP solve.
 #pragma once
 #include "block_lu.hpp"
 #include "zone.hpp"
@@ -21,14 +25,14 @@ namespace nuppn {
 
 struct BatchOpts {
     int fp64_factors;   // 1: FP64 factors from the start (no FP32 stage)
-    int max_refine;     // refinement steps per solve
-    double refine_tol;  // stop when max|d| <= refine_tol * max|x|
-    double accept_tol;  // a stalled refinement is accepted when max|d| <= accept_tol * max|x|
+    int max_refine;     // refinement steps per solve with FP32 factors
+    double refine_tol;  // stop when every |d_i| <= refine_tol * (|x_i| + |y_i| + 1e-30)
+    double accept_tol;  // a stalled FP32 refinement is accepted below this (same measure)
 };
 
 // per-slot scratch of the block Newton kernels
 enum BSlotArr {
-    B_XN, B_VX, B_ST1, B_ST2, B_AVAL, B_RHS, B_X, B_RES, B_D, B_CS, B_SINV, B_W, B_BM, B_E, B_XF,
+    B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_CS, B_PART, B_SINV, B_W, B_BM, B_E, B_XF,
     B_G, B_GPIV, B_COUNT
 };
 
@@ -38,10 +42,10 @@ struct BSlotLayout {
 };
 
 inline BSlotLayout bslot_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
-                                int nbd) {
+                                int nbd, int npart) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
-    const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 8ull * nre, 8ull * nre, 8ull * nnzA,
-                                 8ull * n, 8ull * n, 8ull * n, 8ull * n, 8ull * n, bs, bs, bs,
+    const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
+                                 8ull * n, 8ull * n, 8ull * n, 8ull * n, 8ull * npart, bs, bs, bs,
                                  8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                  4ull * nbd + 4};
     BSlotLayout L;
@@ -76,13 +80,15 @@ __device__ inline BlockFactors<T> slot_factors(char *base, const size_t *off) {
     return F;
 }
 
-// Factor + refined solve of the assembled system in precision T.  Returns 0 (x holds the
-// solution), 1 when FP32 factors should be replaced by FP64 ones, or -4 when the FP64
-// factorisation fails (zero or non-finite pivot).
+// Factor + refined solve of the assembled system (A in CSR order) in precision T.  y: current
+// abundances (species order) for the componentwise test.  Returns 0 (x holds the solution), 1
+// when FP32 factors should be replaced by FP64 ones, or -4 when the FP64 factorisation fails
+// (zero or non-finite pivot).
 template <class T, int NB>
-__device__ int block_refined_solve(const BKArgs &b, const double *Aval, const double *rhs,
-                                   double *x, double *res, double *d, char *slot,
-                                   NUPPN_LDS char *lds, ZoneStats &st) {
+__device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
+                                   const double *rhs, const double *y, double *x, double *res,
+                                   double *d, double *part, char *slot, NUPPN_LDS char *lds,
+                                   ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
     const DevBlock &B = b.B;
     const int n = B.n;
@@ -97,32 +103,67 @@ __device__ int block_refined_solve(const BKArgs &b, const double *Aval, const do
         res[i] = rhs[i];
     }
     wave_sync();
-    double eta_prev = 0.0;
-    for (int k = 0; k < b.O.max_refine; ++k) {
+    const int steps = fp64 ? 2 : b.O.max_refine;
+    double eta_prev = 0.0, best_ratio = 1.0;
+    for (int k = 0; k < steps; ++k) {
         block_solve<T, NB>(B, F, L, res, d);
         st.n_solve++;
-        double dm = 0.0, xm = 0.0;
+        double eta = 0.0;
         bool fin = true;
         for (int i = lane_id(); i < n; i += WAVE) {
             const double xi = x[i] + d[i];
             x[i] = xi;
-            dm = fmax(dm, fabs(d[i]));
-            xm = fmax(xm, fabs(xi));
+            const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + 1.0e-30;
+            eta = fmax(eta, fabs(d[i]) / scale);
             fin = fin && isfinite(xi);
         }
-        dm = wave_max(dm);
-        xm = wave_max(xm);
+        eta = wave_max(eta);
         wave_sync();
         if (ballot(!fin)) return fp64 ? -4 : 1;
-        const double eta = dm / fmax(xm, 1e-300);
-        if (eta <= b.O.refine_tol) return 0;
-        if ((k > 0 && eta > 0.5 * eta_prev) || k == b.O.max_refine - 1)
-            return (!fp64 && eta > b.O.accept_tol) ? 1 : 0;
+        if (eta <= b.O.refine_tol || (fp64 && k == steps - 1)) return 0;
+        // contraction from the second correction on (the first solve's eta is 1 by construction:
+        // x starts at zero).  A stall after a strong contraction is the FP64 residual's own
+        // rounding floor and is accepted; a stall without one means cond(A) * eps(T) is not
+        // small, and FP32 factors cannot deliver the FP64 solution.
+        if (k > 1) {
+            const double ratio = eta / eta_prev;
+            if (ratio > 0.5) return (!fp64 && best_ratio > 0.1) ? 1 : 0;
+            best_ratio = fmin(best_ratio, ratio);
+        }
+        if (k == steps - 1) return (!fp64 && eta > b.O.accept_tol) ? 1 : 0;
         eta_prev = eta;
-        block_residual(B, Aval, rhs, x, res);
+        block_residual(B, Aval, rhs, x, res, part);
         st.n_refine++;
     }
     return 0;
+}
+
+// the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order) and the
+// Newton rhs y_old - y_cur + dt dX/dt as chunked sums over precomputed coefficient lists
+__device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double dt,
+                                      const double *y_old, const double *y_cur, const double *v,
+                                      double *xn, double *vx, double *stem, double *Aval,
+                                      double *rhs, double *part) {
+    molar(N, y_cur, xn);
+    wave_sync();
+    reaction_terms(N, xn, v, vx, stem, stem + N.nre);
+    wave_sync();
+    const DevSeg &J = B.jac;
+    seg_chunks(J, part, [&](int e) { return J.e_coef[e] * stem[J.e_idx[e]]; });
+    for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+        const double jv = 1.0e-99 + seg_total(J, part, k);
+        double val = jv * (-dt);
+        if (B.csr_isdiag[k]) val = val + 1.0;
+        Aval[k] = val;
+    }
+    wave_sync();
+    const DevSeg &D = B.dxdt;
+    seg_chunks(D, part, [&](int e) { return D.e_coef[e] * vx[D.e_idx[e]]; });
+    for (int r = lane_id(); r < N.n; r += WAVE) {
+        const int sp = N.red2sp[r];
+        rhs[r] = y_old[sp] - y_cur[sp] + dt * seg_total(D, part, r);
+    }
+    wave_sync();
 }
 
 // One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z with the
@@ -144,25 +185,17 @@ __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
     const double *v = zrow(Z.v, Z.nre_stride, z);
     double *xn = (double *) (slot + b.off[B_XN]);
     double *vx = (double *) (slot + b.off[B_VX]);
-    double *st1 = (double *) (slot + b.off[B_ST1]);
-    double *st2 = (double *) (slot + b.off[B_ST2]);
+    double *stem = (double *) (slot + b.off[B_ST]);
     double *Aval = (double *) (slot + b.off[B_AVAL]);
     double *rhs = (double *) (slot + b.off[B_RHS]);
     double *x = (double *) (slot + b.off[B_X]);
     double *res = (double *) (slot + b.off[B_RES]);
     double *d = (double *) (slot + b.off[B_D]);
+    double *part = (double *) (slot + b.off[B_PART]);
     const double dt = s.dttry;
-    molar(N, y_cur, xn);
-    wave_sync();
-    reaction_terms(N, xn, v, vx, st1, st2);
-    wave_sync();
-    (void) assemble_matrix(N, dt, st1, st2, Aval);
-    for (int r = lane_id(); r < N.n; r += WAVE) {
-        const int sp = N.red2sp[r];
-        rhs[r] = y_old[sp] - y_cur[sp] + dt * dxdt_species(N, r, vx);
-    }
-    wave_sync();
-    const int rc = block_refined_solve<T, NB>(b, Aval, rhs, x, res, d, slot, lds, s.st);
+    block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs, part);
+    const int rc = block_refined_solve<T, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
+                                              s.st);
     if (rc == 1) {   // FP32 factors insufficient: the FP64 kernel redoes this iteration
         s.pad0 = 1;
         if (lane_id() == 0) Z.state[z] = s;

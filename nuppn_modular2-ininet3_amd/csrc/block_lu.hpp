@@ -25,14 +25,23 @@
 
 namespace nuppn {
 
+// chunked segmented sum (see SegChunks in symbolic.hpp)
+struct DevSeg {
+    int nseg, nch;
+    const int *seg_ch, *ch_beg, *e_idx;
+    const double *e_coef;
+};
+
 struct DevBlock {
-    int NB, nbd, ni, nblk, nip, n;
+    int NB, nbd, ni, nblk, nip, n, nnzA;
     const int *vpos;       // [n] reduced index -> padded vector slot
     const double *mass;    // [n]
-    const int *diagq;      // [n]
-    const int *ent_ptr, *ent_q, *ent_loc, *ent_col;
+    const int *diag_a;     // [n] position of each column's diagonal in the A value array
+    const int *ent_ptr, *ent_a, *ent_loc, *ent_col;   // ent_a: position in the A value array
     const double *ent_ys;
-    const int *csr_ptr, *csr_q, *csr_col;
+    const int *csr_col;    // [nnzA] column of each A entry (A kept in CSR order)
+    const unsigned char *csr_isdiag;
+    DevSeg jac, dxdt, resid;
 };
 
 // factors of one zone slot; T is the factor precision
@@ -204,7 +213,7 @@ __device__ inline void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *B
 // scaled value of CSC entry e of a block list: molar units, column-scaled
 __device__ __forceinline__ double block_entry(const DevBlock &B, const double *Aval,
                                               const double *cs, int e) {
-    return Aval[B.ent_q[e]] * B.ent_ys[e] * cs[B.ent_col[e]];
+    return Aval[B.ent_a[e]] * B.ent_ys[e] * cs[B.ent_col[e]];
 }
 
 // zero an NB x LD LDS buffer and scatter the entries of list `list` into it; identity on the
@@ -227,48 +236,75 @@ __device__ inline void scatter_block(NUPPN_LDS T *X, bool rowmajor, const DevBlo
     lds_fence();
 }
 
-// y = M x for an NB x NB column-major block M in global memory and x in LDS; the result is
-// returned in every lane whose lane % NB is the row (both halves hold the full sum)
+// lane (rr = lane % NB, h = lane / NB) part of an NB x NB column-major block for matvec_regs
 template <class T, int NB>
-__device__ __forceinline__ T matvec_row(const T *__restrict__ M, const NUPPN_LDS T *x) {
-    using C = BlockCfg<T, NB>;
+__device__ __forceinline__ void load_mv(const T *__restrict__ M, T (&m)[NB / BlockCfg<T, NB>::H]) {
+    constexpr int CPL = NB / BlockCfg<T, NB>::H;
     const int rr = lane_id() % NB, h = lane_id() / NB;
-    constexpr int CPL = NB / C::H;   // columns per lane
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) m[j] = M[(h * CPL + j) * NB + rr];
+}
+
+// row rr of M x with M's lane part in registers and x in LDS (full sum in both halves)
+template <class T, int NB>
+__device__ __forceinline__ T matvec_regs(const T (&m)[NB / BlockCfg<T, NB>::H],
+                                         const NUPPN_LDS T *x) {
+    constexpr int CPL = NB / BlockCfg<T, NB>::H;
+    const int h = lane_id() / NB;
     T acc = T(0);
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-        const int col = h * CPL + j;
-        acc = fma_t(M[col * NB + rr], x[col], acc);
-    }
-    if (C::H == 2) acc += __shfl_xor(acc, 32, WAVE);
+    for (int j = 0; j < CPL; ++j) acc = fma_t(m[j], x[h * CPL + j], acc);
+    if (BlockCfg<T, NB>::H == 2) acc += __shfl_xor(acc, 32, WAVE);
     return acc;
 }
 
-// v <- T^-1 v for the interior block-tridiagonal part (v in LDS, nip entries)
+// v <- T^-1 v for the interior block-tridiagonal part (v in LDS, nip entries).  The block
+// matrices do not depend on v, so each sweep loads the next block's matrices before it works on
+// the current block: the global-memory latency overlaps the dependent chain of mat-vecs.
 template <class T, int NB>
 __device__ inline void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
                                     NUPPN_LDS T *v) {
+    constexpr int CPL = NB / BlockCfg<T, NB>::H;
     const int rr = lane_id() % NB, h = lane_id() / NB;
     constexpr size_t BS = (size_t) NB * NB;
-    for (int i = 0; i < B.nblk; ++i) {
+    const int nblk = B.nblk;
+    T mS[CPL], mB[CPL], nS[CPL], nB[CPL];
+    load_mv<T, NB>(F.Sinv, mS);
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) mB[j] = T(0);
+    for (int i = 0; i < nblk; ++i) {
+        if (i + 1 < nblk) {
+            load_mv<T, NB>(F.Bm + (size_t) i * BS, nB);
+            load_mv<T, NB>(F.Sinv + (size_t) (i + 1) * BS, nS);
+        }
         NUPPN_LDS T *vi = v + i * NB;
         if (i > 0) {
-            const T acc = matvec_row<T, NB>(F.Bm + (size_t) (i - 1) * BS, v + (i - 1) * NB);
+            const T acc = matvec_regs<T, NB>(mB, v + (i - 1) * NB);
             lds_fence();
             if (h == 0) vi[rr] = vi[rr] - acc;
             lds_fence();
         }
-        const T acc = matvec_row<T, NB>(F.Sinv + (size_t) i * BS, vi);
+        const T acc = matvec_regs<T, NB>(mS, vi);
         lds_fence();
         if (h == 0) vi[rr] = acc;
         lds_fence();
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            mS[j] = nS[j];
+            mB[j] = nB[j];
+        }
     }
-    for (int i = B.nblk - 2; i >= 0; --i) {
+    if (nblk < 2) return;
+    load_mv<T, NB>(F.W + (size_t) (nblk - 2) * BS, mS);
+    for (int i = nblk - 2; i >= 0; --i) {
+        if (i > 0) load_mv<T, NB>(F.W + (size_t) (i - 1) * BS, nS);
         NUPPN_LDS T *vi = v + i * NB;
-        const T acc = matvec_row<T, NB>(F.W + (size_t) i * BS, v + (i + 1) * NB);
+        const T acc = matvec_regs<T, NB>(mS, v + (i + 1) * NB);
         lds_fence();
         if (h == 0) vi[rr] = vi[rr] - acc;
         lds_fence();
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) mS[j] = nS[j];
     }
 }
 
@@ -288,7 +324,7 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
     const int nblk = B.nblk;
     bool bad = false;
     for (int i = lane; i < B.n; i += WAVE) {
-        const double d = Aval[B.diagq[i]];
+        const double d = Aval[B.diag_a[i]];
         if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
         F.cs[i] = 1.0 / d;
     }
@@ -431,14 +467,41 @@ __device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const B
     wave_sync();
 }
 
-// r = rhs - A x over the reduced CSC values (row-wise through the CSR view), FP64
-__device__ inline void block_residual(const DevBlock &B, const double *Aval, const double *rhs,
-                                      const double *x, double *r) {
-    for (int i = lane_id(); i < B.n; i += WAVE) {
-        double s = rhs[i];
-        for (int k = B.csr_ptr[i]; k < B.csr_ptr[i + 1]; ++k) s = fma(-Aval[B.csr_q[k]], x[B.csr_col[k]], s);
-        r[i] = s;
+// chunk pass of a segmented sum: part[c] = sum over the chunk's elements of val(e)
+template <class F>
+__device__ inline void seg_chunks(const DevSeg &S, double *part, F val) {
+    for (int c = lane_id(); c < S.nch; c += WAVE) {
+        const int e0 = S.ch_beg[c], e1 = S.ch_beg[c + 1];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int e = e0;
+        for (; e + 3 < e1; e += 4) {
+            s0 += val(e);
+            s1 += val(e + 1);
+            s2 += val(e + 2);
+            s3 += val(e + 3);
+        }
+        for (; e < e1; ++e) s0 += val(e);
+        part[c] = (s0 + s1) + (s2 + s3);
     }
+    wave_sync();
+}
+
+// segment total of the chunk partials
+__device__ __forceinline__ double seg_total(const DevSeg &S, const double *part, int s) {
+    double t = 0.0;
+    for (int c = S.seg_ch[s]; c < S.seg_ch[s + 1]; ++c) t += part[c];
+    return t;
+}
+
+// r = rhs - A x (A in CSR order), FP64
+__device__ inline void block_residual(const DevBlock &B, const double *Acsr, const double *rhs,
+                                      const double *x, double *r, double *part) {
+    const DevSeg &S = B.resid;
+    seg_chunks(S, part, [&](int e) {
+        const int k = S.e_idx[e];
+        return Acsr[k] * x[B.csr_col[k]];
+    });
+    for (int i = lane_id(); i < B.n; i += WAVE) r[i] = rhs[i] - seg_total(S, part, i);
     wave_sync();
 }
 

@@ -188,10 +188,12 @@ struct nuppn_ctx {
     // batched path (block-tridiagonal pivot-free factorisation)
     BlockSym bsym;
     int bnb = 0;   // block size, 0 when the network's interior is not block tridiagonal
-    int *d_b_vpos = nullptr, *d_b_diagq = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_q = nullptr;
-    int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_ptr = nullptr;
-    int *d_b_csr_q = nullptr, *d_b_csr_col = nullptr;
+    int *d_b_vpos = nullptr, *d_b_diagk = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_k = nullptr;
+    int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr;
+    unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
+    int *d_seg_i[9] = {};         // jac, dxdt, resid: seg_ch, ch_beg, e_idx
+    double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     BSlotLayout blayout{};
     size_t blds32 = 0, blds64 = 0;
     int bmax_slots = 0, bslots = 0, bgrid64 = 0;
@@ -252,17 +254,27 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     B.nblk = b.nblk;
     B.nip = b.nip;
     B.n = c->sym.n;
+    B.nnzA = c->sym.nnzA;
     B.vpos = c->d_b_vpos;
     B.mass = c->d_b_mass;
-    B.diagq = c->d_b_diagq;
+    B.diag_a = c->d_b_diagk;
     B.ent_ptr = c->d_b_ent_ptr;
-    B.ent_q = c->d_b_ent_q;
+    B.ent_a = c->d_b_ent_k;
     B.ent_loc = c->d_b_ent_loc;
     B.ent_col = c->d_b_ent_col;
     B.ent_ys = c->d_b_ent_ys;
-    B.csr_ptr = c->d_b_csr_ptr;
-    B.csr_q = c->d_b_csr_q;
     B.csr_col = c->d_b_csr_col;
+    B.csr_isdiag = c->d_b_isdiag;
+    const SegChunks *sc[3] = {&b.jac, &b.dxdt, &b.resid};
+    DevSeg *ds[3] = {&B.jac, &B.dxdt, &B.resid};
+    for (int i = 0; i < 3; ++i) {
+        ds[i]->nseg = (int) sc[i]->seg_ch.size() - 1;
+        ds[i]->nch = (int) sc[i]->ch_beg.size() - 1;
+        ds[i]->seg_ch = c->d_seg_i[3 * i];
+        ds[i]->ch_beg = c->d_seg_i[3 * i + 1];
+        ds[i]->e_idx = c->d_seg_i[3 * i + 2];
+        ds[i]->e_coef = c->d_seg_c[i];
+    }
     return B;
 }
 
@@ -293,15 +305,24 @@ static int setup_batched(nuppn_ctx *c) {
     if (nb == 0) return 0;   // the batched path stays unavailable; the plain path is unaffected
     if (build_block_symbolic(c->net, c->sym, nb, c->bsym) != 0 || c->bsym.nbd > 8) return 0;
     const BlockSym &b = c->bsym;
-    if (dev_upload(&c->d_b_vpos, b.vpos) || dev_upload(&c->d_b_diagq, b.diagq) ||
-        dev_upload(&c->d_b_ent_ptr, b.ent_ptr) || dev_upload(&c->d_b_ent_q, b.ent_q) ||
+    if (dev_upload(&c->d_b_vpos, b.vpos) || dev_upload(&c->d_b_diagk, b.diagk) ||
+        dev_upload(&c->d_b_ent_ptr, b.ent_ptr) || dev_upload(&c->d_b_ent_k, b.ent_k) ||
         dev_upload(&c->d_b_ent_loc, b.ent_loc) || dev_upload(&c->d_b_ent_col, b.ent_col) ||
-        dev_upload(&c->d_b_csr_ptr, b.csr_ptr) || dev_upload(&c->d_b_csr_q, b.csr_q) ||
-        dev_upload(&c->d_b_csr_col, b.csr_col) || dev_upload(&c->d_b_mass, b.mass) ||
-        dev_upload(&c->d_b_ent_ys, b.ent_ys))
+        dev_upload(&c->d_b_csr_col, b.csr_col) || dev_upload(&c->d_b_isdiag, b.csr_isdiag) ||
+        dev_upload(&c->d_b_mass, b.mass) || dev_upload(&c->d_b_ent_ys, b.ent_ys))
         return NUPPN_ERR_HIP;
+    const SegChunks *sc[3] = {&b.jac, &b.dxdt, &b.resid};
+    int npart = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (dev_upload(&c->d_seg_i[3 * i], sc[i]->seg_ch) ||
+            dev_upload(&c->d_seg_i[3 * i + 1], sc[i]->ch_beg) ||
+            dev_upload(&c->d_seg_i[3 * i + 2], sc[i]->e_idx) ||
+            dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
+            return NUPPN_ERR_HIP;
+        npart = std::max(npart, (int) sc[i]->ch_beg.size() - 1);
+    }
     c->blayout = bslot_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
-                              b.nbd);
+                              b.nbd, npart);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
     c->blds32 = rnd(nb == 64 ? block_lds_bytes<float, 64>(b.nip, b.nbd)
                              : block_lds_bytes<float, 32>(b.nip, b.nbd));
@@ -543,12 +564,13 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
                     (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
                     (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
-                    (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagq,
-                    (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_q, (void *) c->d_b_ent_loc,
-                    (void *) c->d_b_ent_col, (void *) c->d_b_csr_ptr, (void *) c->d_b_csr_q,
-                    (void *) c->d_b_csr_col, (void *) c->d_b_mass, (void *) c->d_b_ent_ys,
-                    (void *) c->d_bslots})
+                    (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagk,
+                    (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_k, (void *) c->d_b_ent_loc,
+                    (void *) c->d_b_ent_col, (void *) c->d_b_csr_col, (void *) c->d_b_isdiag,
+                    (void *) c->d_b_mass, (void *) c->d_b_ent_ys, (void *) c->d_bslots})
         dev_free(p);
+    for (void *p : c->d_seg_i) dev_free(p);
+    for (void *p : c->d_seg_c) dev_free(p);
     for (auto &e : c->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e : c->evn)

@@ -280,6 +280,8 @@ int interior_bandwidth(const Symbolic &s, const std::vector<int> &pos, int ni) {
 
 }  // namespace
 
+static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b);
+
 int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, BlockSym &b) {
     const int n = s.n;
     const std::vector<char> isb = border_species(s);
@@ -367,7 +369,85 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
             b.csr_q[k] = p;
             b.csr_col[k] = c;
         }
+    build_batched_sums(net, s, b);
     return 0;
+}
+
+namespace {
+
+constexpr int kChunk = 16;
+
+// segments given as per-segment element lists -> chunked layout
+void build_chunks(const std::vector<std::vector<std::pair<int, double>>> &segs, SegChunks &out) {
+    out.seg_ch.assign(1, 0);
+    out.ch_beg.assign(1, 0);
+    out.e_idx.clear();
+    out.e_coef.clear();
+    for (const auto &seg : segs) {
+        for (size_t i = 0; i < seg.size(); i += kChunk) {
+            const size_t j = std::min(seg.size(), i + kChunk);
+            for (size_t k = i; k < j; ++k) {
+                out.e_idx.push_back(seg[k].first);
+                out.e_coef.push_back(seg[k].second);
+            }
+            out.ch_beg.push_back((int) out.e_idx.size());
+        }
+        out.seg_ch.push_back((int) out.ch_beg.size() - 1);
+    }
+}
+
+}  // namespace
+
+// chunked J / dX/dt / residual lists of the batched path (after build_block_symbolic's CSR view)
+static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b) {
+    const int n = s.n, nre = net.nre, nnz = s.nnzA;
+    std::vector<int> q2k(nnz);
+    for (int k = 0; k < nnz; ++k) q2k[b.csr_q[k]] = k;
+    b.ent_k.resize(b.ent_q.size());
+    for (size_t e = 0; e < b.ent_q.size(); ++e) b.ent_k[e] = q2k[b.ent_q[e]];
+    b.diagk.resize(n);
+    for (int c = 0; c < n; ++c) b.diagk[c] = q2k[b.diagq[c]];
+    b.csr_isdiag.resize(nnz);
+    for (int k = 0; k < nnz; ++k) b.csr_isdiag[k] = s.A_isdiag[b.csr_q[k]];
+    const double *an = net.aw.data();
+    // J contributions (statement order of calculate_jacobian, jac_rhs.F90:365-392):
+    // value = (+-count) * an[row] / an[col] * stem, stem = st1[l] (m in 0,1,3,4) or st2[l]
+    std::vector<std::vector<std::pair<int, double>>> segs(nnz);
+    for (int k = 0; k < nnz; ++k) {
+        const int q = b.csr_q[k];
+        for (int e = s.jq_ptr[q]; e < s.jq_ptr[q + 1]; ++e) {
+            const int l = s.jq_ent[e] >> 3, m = s.jq_ent[e] & 7;
+            const int in1 = net.k[l], n1 = net.k[nre + l], in2 = net.k[2 * nre + l];
+            const int n2 = net.k[3 * nre + l], out1 = net.k[4 * nre + l], no1 = net.k[5 * nre + l];
+            const int out2 = net.k[6 * nre + l], no2 = net.k[7 * nre + l];
+            const int rows[8] = {in1, in2, in1, out1, out2, in2, out1, out2};
+            const int cols[8] = {in1, in1, in2, in1, in1, in2, in2, in2};
+            const int cnt[8] = {-n1, -n2, -n1, no1, no2, -n2, no1, no2};
+            const bool stem2[8] = {false, false, true, false, false, true, true, true};
+            segs[k].push_back({stem2[m] ? nre + l : l, cnt[m] * an[rows[m]] / an[cols[m]]});
+        }
+    }
+    build_chunks(segs, b.jac);
+    // dX/dt (calculate_dxdt, jac_rhs.F90:250-272): roles k1 -k2, k3 -1, k5 +k6, k7 +k8, times A
+    std::vector<std::vector<std::pair<int, double>>> dsegs(n);
+    for (int r = 0; r < n; ++r) {
+        const double a = an[s.red2sp[r]];
+        for (int e = s.dx_ptr[r]; e < s.dx_ptr[r + 1]; ++e) {
+            const int l = s.dx_ent[e] >> 2, role = s.dx_ent[e] & 3;
+            double c;
+            if (role == 0) c = -net.k[nre + l];
+            else if (role == 1) c = -1.0;
+            else if (role == 2) c = net.k[5 * nre + l];
+            else c = net.k[7 * nre + l];
+            dsegs[r].push_back({l, c * a});
+        }
+    }
+    build_chunks(dsegs, b.dxdt);
+    // residual rows: element = CSR position k, operand = column
+    std::vector<std::vector<std::pair<int, double>>> rsegs(n);
+    for (int i = 0; i < n; ++i)
+        for (int k = b.csr_ptr[i]; k < b.csr_ptr[i + 1]; ++k) rsegs[i].push_back({k, 1.0});
+    build_chunks(rsegs, b.resid);
 }
 
 int choose_block_size(const HostNetwork &net, const Symbolic &s) {

@@ -32,6 +32,16 @@ struct Symbolic {
 // builds everything but the column order; returns 0 or a negative error
 int build_symbolic(const HostNetwork &net, Symbolic &s);
 
+// Segmented sums in chunks: segment s owns elements [e of its chunks]; chunk c covers elements
+// ch_beg[c] .. ch_beg[c+1]-1 (at most a fixed number) and segment s chunks seg_ch[s] ..
+// seg_ch[s+1]-1.  A wavefront sums the chunks in parallel, then the chunk partials per segment,
+// so a segment with thousands of terms (the light projectiles' rows) does not serialise one lane.
+struct SegChunks {
+    std::vector<int> seg_ch, ch_beg;
+    std::vector<int> e_idx;       // element operand index
+    std::vector<double> e_coef;   // element coefficient
+};
+
 // Static block structure of the reduced matrix for the pivot-free batched factorisation.
 //
 // Species that couple to far more species than the median (the light projectiles n, p, alpha of
@@ -61,6 +71,13 @@ struct BlockSym {
     std::vector<double> ent_ys; // mass(col) / mass(row): X units -> molar (Y) units
     // CSR view of the CSC pattern for the residual: per reduced row, (CSC entry, column)
     std::vector<int> csr_ptr, csr_q, csr_col;
+    // the batched path keeps A in CSR order: ent_k / diagk are ent_q / diagq as CSR positions
+    std::vector<int> ent_k, diagk;
+    std::vector<unsigned char> csr_isdiag;
+    // chunked sums: J entries (CSR order) from the per-reaction stems [st1 | st2] (2 nre),
+    // dX/dt of reduced species from the fluxes vx (nre, coefficients include the mass number),
+    // the residual rows from the CSR entries (element = CSR position, operand = column)
+    SegChunks jac, dxdt, resid;
 };
 
 // Builds the block structure for block size NB (32 or 64); returns 0, or -1 when the interior
