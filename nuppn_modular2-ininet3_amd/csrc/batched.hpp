@@ -30,31 +30,40 @@ struct BatchOpts {
     double accept_tol;  // a stalled FP32 refinement is accepted below this (same measure)
 };
 
-// per-slot scratch of the block Newton kernels
-enum BSlotArr {
-    B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_CS, B_PART, B_SINV, B_W, B_BM, B_E, B_XF,
-    B_G, B_GPIV, B_COUNT
-};
+// per-slot scratch of the block Newton kernels (one slot per resident wavefront)
+enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_PART, B_COUNT };
 
-struct BSlotLayout {
+// per-zone factor storage: kept between Newton iterations, so an iteration can reuse the
+// previous iteration's factors as its preconditioner
+enum BFacArr { F_CS, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
+
+struct BLayout {
     size_t off[B_COUNT];
     size_t bytes;
+    size_t foff[F_COUNT];
+    size_t fbytes;
 };
 
-inline BSlotLayout bslot_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
-                                int nbd, int npart) {
+inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
+                            int nbd, int npart) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
-                                 8ull * n, 8ull * n, 8ull * n, 8ull * n, 8ull * npart, bs, bs, bs,
-                                 8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
-                                 4ull * nbd + 4};
-    BSlotLayout L;
+                                 8ull * n, 8ull * n, 8ull * n, 8ull * npart};
+    const size_t flen[F_COUNT] = {8ull * n, bs, bs, bs, 8ull * nbd * nip + 8,
+                                  8ull * nbd * nip + 8, 8ull * nbd * nbd + 8, 4ull * nbd + 4};
+    BLayout L;
     size_t o = 0;
     for (int i = 0; i < B_COUNT; ++i) {
         L.off[i] = o;
         o += (len[i] + 255) & ~(size_t) 255;
     }
     L.bytes = o;
+    o = 0;
+    for (int i = 0; i < F_COUNT; ++i) {
+        L.foff[i] = o;
+        o += (flen[i] + 255) & ~(size_t) 255;
+    }
+    L.fbytes = o;
     return L;
 }
 
@@ -64,76 +73,112 @@ struct BKArgs {
     char *slot_base;
     size_t slot_bytes;
     size_t off[B_COUNT];
+    char *fac_base;      // [nz] factor areas
+    size_t fac_bytes;
+    size_t foff[F_COUNT];
 };
 
 template <class T>
-__device__ inline BlockFactors<T> slot_factors(char *base, const size_t *off) {
+__device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
     BlockFactors<T> F;
-    F.Sinv = (T *) (base + off[B_SINV]);
-    F.W = (T *) (base + off[B_W]);
-    F.Bm = (T *) (base + off[B_BM]);
-    F.E = (T *) (base + off[B_E]);
-    F.XF = (T *) (base + off[B_XF]);
-    F.G = (double *) (base + off[B_G]);
-    F.gpiv = (int *) (base + off[B_GPIV]);
-    F.cs = (double *) (base + off[B_CS]);
+    F.Sinv = (T *) (base + foff[F_SINV]);
+    F.W = (T *) (base + foff[F_W]);
+    F.Bm = (T *) (base + foff[F_BM]);
+    F.E = (T *) (base + foff[F_E]);
+    F.XF = (T *) (base + foff[F_XF]);
+    F.G = (double *) (base + foff[F_G]);
+    F.gpiv = (int *) (base + foff[F_GPIV]);
+    F.cs = (double *) (base + foff[F_CS]);
     return F;
 }
 
-// Factor + refined solve of the assembled system (A in CSR order) in precision T.  y: current
-// abundances (species order) for the componentwise test.  Returns 0 (x holds the solution), 1
-// when FP32 factors should be replaced by FP64 ones, or -4 when the FP64 factorisation fails
-// (zero or non-finite pivot).
+// Refined solve of the assembled system (A in CSR order) with factors in precision T.  fstate
+// holds the zone's factor state (0 none, 1 FP32, 2 FP64): factors of the previous Newton
+// iteration are reused as the preconditioner while they contract well (first correction below
+// reuse_tol), and refreshed otherwise.  y: current abundances (species order) for the
+// componentwise test.  Returns 0 (x holds the solution), 1 when FP32 factors cannot deliver the
+// FP64 solution (the zone moves to FP64), or -4 when an FP64 factorisation fails.
 template <class T, int NB>
 __device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
                                    const double *rhs, const double *y, double *x, double *res,
-                                   double *d, double *part, char *slot, NUPPN_LDS char *lds,
-                                   ZoneStats &st) {
+                                   double *d, double *part, char *fac, NUPPN_LDS char *lds,
+                                   int &fstate, ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
+    constexpr int mine = fp64 ? 2 : 1;
+    constexpr double reuse_tol = 0.05;
     const DevBlock &B = b.B;
     const int n = B.n;
-    const BlockFactors<T> F = slot_factors<T>(slot, b.off);
+    const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
-    const int fail = block_factor<T, NB>(B, Aval, F, L);
-    st.n_factor++;
-    if (fp64) st.n_factor_fp64++;
-    if (fail) return fp64 ? -4 : 1;
-    for (int i = lane_id(); i < n; i += WAVE) {
-        x[i] = 0.0;
-        res[i] = rhs[i];
-    }
-    wave_sync();
-    const int steps = fp64 ? 2 : b.O.max_refine;
-    double eta_prev = 0.0, best_ratio = 1.0;
-    for (int k = 0; k < steps; ++k) {
-        block_solve<T, NB>(B, F, L, res, d);
-        st.n_solve++;
-        double eta = 0.0;
-        bool fin = true;
+    bool fresh = false;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (fstate != mine) {
+            const int fail = block_factor<T, NB>(B, Aval, F, L);
+            st.n_factor++;
+            if (fp64) st.n_factor_fp64++;
+            if (fail) {
+                fstate = 0;
+                return fp64 ? -4 : 1;
+            }
+            fstate = mine;
+            fresh = true;
+        }
         for (int i = lane_id(); i < n; i += WAVE) {
-            const double xi = x[i] + d[i];
-            x[i] = xi;
-            const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + 1.0e-30;
-            eta = fmax(eta, fabs(d[i]) / scale);
-            fin = fin && isfinite(xi);
+            x[i] = 0.0;
+            res[i] = rhs[i];
         }
-        eta = wave_max(eta);
         wave_sync();
-        if (ballot(!fin)) return fp64 ? -4 : 1;
-        if (eta <= b.O.refine_tol || (fp64 && k == steps - 1)) return 0;
-        // contraction from the second correction on (the first solve's eta is 1 by construction:
-        // x starts at zero).  A stall after a strong contraction is the FP64 residual's own
-        // rounding floor and is accepted; a stall without one means cond(A) * eps(T) is not
-        // small, and FP32 factors cannot deliver the FP64 solution.
-        if (k > 1) {
-            const double ratio = eta / eta_prev;
-            if (ratio > 0.5) return (!fp64 && best_ratio > 0.1) ? 1 : 0;
-            best_ratio = fmin(best_ratio, ratio);
+        const int steps = (fp64 && fresh) ? 2 : b.O.max_refine;
+        double eta_prev = 0.0, best_ratio = 1.0;
+        bool refresh = false;
+        for (int k = 0; k < steps; ++k) {
+            block_solve<T, NB>(B, F, L, res, d);
+            st.n_solve++;
+            double eta = 0.0;
+            bool fin = true;
+            for (int i = lane_id(); i < n; i += WAVE) {
+                const double xi = x[i] + d[i];
+                x[i] = xi;
+                const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + 1.0e-30;
+                eta = fmax(eta, fabs(d[i]) / scale);
+                fin = fin && isfinite(xi);
+            }
+            eta = wave_max(eta);
+            wave_sync();
+            if (ballot(!fin)) {
+                if (!fresh) { refresh = true; break; }
+                fstate = 0;
+                return fp64 ? -4 : 1;
+            }
+            if (eta <= b.O.refine_tol || (fp64 && fresh && k == steps - 1)) return 0;
+            // stale factors that no longer contract well are refreshed
+            if (!fresh && k == 1 && eta > reuse_tol) { refresh = true; break; }
+            // contraction from the second correction on (the first solve's eta is 1 by
+            // construction: x starts at zero).  A stall after a strong contraction is the FP64
+            // residual's own rounding floor and is accepted; a stall without one means
+            // cond(A) * eps(T) is not small, and FP32 factors cannot deliver the FP64 solution.
+            if (k > 1) {
+                const double ratio = eta / eta_prev;
+                if (ratio > 0.5) {
+                    if (best_ratio <= 0.1) return 0;
+                    if (!fresh) { refresh = true; break; }
+                    fstate = 0;
+                    return fp64 ? 0 : 1;
+                }
+                best_ratio = fmin(best_ratio, ratio);
+            }
+            if (k == steps - 1) {
+                if (eta <= b.O.accept_tol) return 0;
+                if (!fresh) { refresh = true; break; }
+                fstate = 0;
+                return fp64 ? 0 : 1;
+            }
+            eta_prev = eta;
+            block_residual(B, Aval, rhs, x, res, part);
+            st.n_refine++;
         }
-        if (k == steps - 1) return (!fp64 && eta > b.O.accept_tol) ? 1 : 0;
-        eta_prev = eta;
-        block_residual(B, Aval, rhs, x, res, part);
-        st.n_refine++;
+        if (!refresh) return 0;
+        fstate = 0;   // refactor with the current matrix and solve again
     }
     return 0;
 }
@@ -173,6 +218,8 @@ __device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double
 template <class T, int NB>
 __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
                                   NUPPN_LDS char *lds, int z) {
+    // zone state: pad0 factor precision (0 FP32, 1 FP64), pad1 factor state (0 none, 1 FP32
+    // factors, 2 FP64 factors) of the zone's factor area
     constexpr int want = sizeof(T) == 8 ? 1 : 0;
     const DevNet &N = a.N;
     const DevKnobs &K = a.K;
@@ -194,10 +241,12 @@ __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
     double *part = (double *) (slot + b.off[B_PART]);
     const double dt = s.dttry;
     block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs, part);
-    const int rc = block_refined_solve<T, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
-                                              s.st);
+    char *fac = b.fac_base + (size_t) z * b.fac_bytes;
+    const int rc = block_refined_solve<T, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, fac, lds,
+                                              s.pad1, s.st);
     if (rc == 1) {   // FP32 factors insufficient: the FP64 kernel redoes this iteration
         s.pad0 = 1;
+        s.pad1 = 0;
         if (lane_id() == 0) Z.state[z] = s;
         wave_sync();
         return;

@@ -38,6 +38,7 @@ struct DevBlock {
     const double *mass;    // [n]
     const int *diag_a;     // [n] position of each column's diagonal in the A value array
     const int *ent_ptr, *ent_a, *ent_loc, *ent_col;   // ent_a: position in the A value array
+    const int *ent_dst;    // D/C/B entries: [area][block][NB*NB] destination (area 0 D, 1 C, 2 B)
     const double *ent_ys;
     const int *csr_col;    // [nnzA] column of each A entry (A kept in CSR order)
     const unsigned char *csr_isdiag;
@@ -100,6 +101,20 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
 template <class T>
 __device__ __forceinline__ T fma_t(T a, T b, T c) { return fma(a, b, c); }
 
+// 1/x for a pivot of the column-scaled matrix (magnitude near 1, never denormal): the hardware
+// reciprocal refined by Newton-Raphson steps to the precision of T, without the IEEE division's
+// range fix-ups
+__device__ __forceinline__ double pivot_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ float pivot_rcp(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 // the R elements of a block column held in registers by one lane
 template <class T, int NB>
 struct Regs {
@@ -133,6 +148,14 @@ __device__ __forceinline__ void lds_store_rm(NUPPN_LDS T *X, const Regs<T, NB> &
 }
 
 template <class T, int NB>
+__device__ __forceinline__ void glb_load_cm(const T *G, Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    const int c = lane_id() % NB, h = lane_id() / NB;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) a.v[t] = G[c * NB + h * C::R + t];
+}
+
+template <class T, int NB>
 __device__ __forceinline__ void glb_store_cm(T *G, const Regs<T, NB> &a) {
     using C = BlockCfg<T, NB>;
     const int c = lane_id() % NB, h = lane_id() / NB;
@@ -162,17 +185,21 @@ __device__ inline void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
             rowbuf[c] = rv;
         }
         lds_fence();
-        const T pinv = T(1) / colbuf[k];
+        const T pinv = pivot_rcp(colbuf[k]);
         const T mkc = rowbuf[c] * pinv;
+        // column k becomes -col/p, the others a - col * (row k / p); row k is set below
+        const bool pc = c == k;
+        const T f = pc ? pinv : mkc;
 #pragma unroll
         for (int t = 0; t < C::R; ++t) {
-            const int r = h * C::R + t;
             const T col = colbuf[h * C::R + t];
-            T v;
-            if (c == k) v = (r == k) ? pinv : -col * pinv;
-            else if (r == k) v = mkc;
-            else v = fma_t(-col, mkc, a.v[t]);
-            a.v[t] = v;
+            a.v[t] = fma_t(-col, f, pc ? T(0) : a.v[t]);
+        }
+        if (h == kh) {
+            const T rv = pc ? pinv : mkc;
+#pragma unroll
+            for (int t = 0; t < C::R; ++t)
+                if (t == kt) a.v[t] = rv;
         }
         lds_fence();   // every read of this step precedes the next step's writes
     }
@@ -214,26 +241,6 @@ __device__ inline void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *B
 __device__ __forceinline__ double block_entry(const DevBlock &B, const double *Aval,
                                               const double *cs, int e) {
     return Aval[B.ent_a[e]] * B.ent_ys[e] * cs[B.ent_col[e]];
-}
-
-// zero an NB x LD LDS buffer and scatter the entries of list `list` into it; identity on the
-// padded diagonal positions of the last block when `pad_block` >= 0
-template <class T, int NB>
-__device__ inline void scatter_block(NUPPN_LDS T *X, bool rowmajor, const DevBlock &B, int list,
-                                     const double *Aval, const double *cs, int pad_block) {
-    using C = BlockCfg<T, NB>;
-    lds_fence();
-    for (int i = lane_id(); i < NB * C::LD; i += WAVE) X[i] = T(0);
-    lds_fence();
-    for (int e = B.ent_ptr[list] + lane_id(); e < B.ent_ptr[list + 1]; e += WAVE) {
-        const int loc = B.ent_loc[e], r = loc % NB, c = loc / NB;
-        X[rowmajor ? r * C::LD + c : c * C::LD + r] = (T) block_entry(B, Aval, cs, e);
-    }
-    if (pad_block >= 0) {
-        const int r = lane_id();
-        if (r < NB && pad_block * NB + r >= B.ni) X[r * C::LD + r] = T(1);
-    }
-    lds_fence();
 }
 
 // lane (rr = lane % NB, h = lane / NB) part of an NB x NB column-major block for matvec_regs
@@ -330,29 +337,55 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
     }
     if (ballot(bad)) return 1;
     wave_sync();
-    Regs<T, NB> a;
-    scatter_block<T, NB>(L.X2, false, B, 0, Aval, F.cs, nblk == 1 ? 0 : -1);
-    lds_load_cm<T, NB>(L.X2, a);
+    // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
+    // where W_i goes, B_i in place (the loop below then only streams them, prefetched)
+    {
+        const size_t area = (size_t) nblk * BS;
+        for (size_t e = lane; e < area; e += WAVE) {
+            F.Sinv[e] = T(0);
+            F.W[e] = T(0);
+            F.Bm[e] = T(0);
+        }
+        wave_sync();
+        const int nent = B.ent_ptr[3 * nblk - 2];
+        for (int e = lane; e < nent; e += WAVE) {
+            const int dst = B.ent_dst[e];
+            const int ar = (int) (dst / area);
+            T *base = ar == 0 ? F.Sinv : (ar == 1 ? F.W : F.Bm);
+            base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
+        }
+        const int pr = lane, last = nblk - 1;   // identity on the padded diagonal positions
+        if (pr < NB && last * NB + pr >= B.ni) F.Sinv[(size_t) last * BS + pr * NB + pr] = T(1);
+        wave_sync();
+    }
+    Regs<T, NB> a, cb, dn, bb;
+    glb_load_cm<T, NB>(F.Sinv, a);
     for (int i = 0; i < nblk; ++i) {
+        const bool more = i + 1 < nblk;
+        if (more) {   // operands of this step's products, independent of the inversion
+            glb_load_cm<T, NB>(F.W + i * BS, cb);
+            glb_load_cm<T, NB>(F.Bm + i * BS, bb);
+            glb_load_cm<T, NB>(F.Sinv + (i + 1) * BS, dn);
+        }
         gj_inverse<T, NB>(a, L.colbuf, L.rowbuf);
         glb_store_cm<T, NB>(F.Sinv + i * BS, a);
         bool fin = true;
 #pragma unroll
         for (int t = 0; t < C::R; ++t) fin = fin && isfinite((double) a.v[t]);
         if (ballot(!fin)) return 1;
-        if (i == nblk - 1) break;
+        if (!more) break;
         lds_fence();
         lds_store_cm<T, NB>(L.X1, a);
-        scatter_block<T, NB>(L.X2, true, B, nblk + i, Aval, F.cs, -1);   // C_i, row-major
+        lds_store_rm<T, NB>(L.X2, cb);
+        lds_fence();
         gemm_lds<T, NB>(L.X1, L.X2, a);                                  // W_i = S_i^-1 C_i
         glb_store_cm<T, NB>(F.W + i * BS, a);
         lds_fence();
         lds_store_rm<T, NB>(L.WL, a);
-        scatter_block<T, NB>(L.X1, false, B, 2 * nblk - 1 + i, Aval, F.cs, -1);   // B_i
-        for (int e = lane; e < NB * NB; e += WAVE)
-            F.Bm[i * BS + e] = L.X1[(e / NB) * C::LD + e % NB];
-        scatter_block<T, NB>(L.X2, false, B, i + 1, Aval, F.cs, i + 1 == nblk - 1 ? i + 1 : -1);
-        lds_load_cm<T, NB>(L.X2, a);
+        lds_store_cm<T, NB>(L.X1, bb);
+        lds_fence();
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) a.v[t] = dn.v[t];
         gemm_lds_sub<T, NB>(L.X1, L.WL, a);                              // D - B_i W_i
     }
     wave_sync();

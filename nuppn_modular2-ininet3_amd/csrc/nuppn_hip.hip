@@ -189,12 +189,14 @@ struct nuppn_ctx {
     BlockSym bsym;
     int bnb = 0;   // block size, 0 when the network's interior is not block tridiagonal
     int *d_b_vpos = nullptr, *d_b_diagk = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_k = nullptr;
-    int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr;
+    int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr, *d_b_ent_dst = nullptr;
     unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
     int *d_seg_i[9] = {};         // jac, dxdt, resid: seg_ch, ch_beg, e_idx
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
-    BSlotLayout blayout{};
+    BLayout blayout{};
+    int nz_fcap = 0;
+    char *d_bfac = nullptr;   // per-zone factor areas of the batched path
     size_t blds32 = 0, blds64 = 0;
     int bmax_slots = 0, bslots = 0, bgrid64 = 0;
     char *d_bslots = nullptr;
@@ -261,6 +263,7 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     B.ent_ptr = c->d_b_ent_ptr;
     B.ent_a = c->d_b_ent_k;
     B.ent_loc = c->d_b_ent_loc;
+    B.ent_dst = c->d_b_ent_dst;
     B.ent_col = c->d_b_ent_col;
     B.ent_ys = c->d_b_ent_ys;
     B.csr_col = c->d_b_csr_col;
@@ -309,7 +312,8 @@ static int setup_batched(nuppn_ctx *c) {
         dev_upload(&c->d_b_ent_ptr, b.ent_ptr) || dev_upload(&c->d_b_ent_k, b.ent_k) ||
         dev_upload(&c->d_b_ent_loc, b.ent_loc) || dev_upload(&c->d_b_ent_col, b.ent_col) ||
         dev_upload(&c->d_b_csr_col, b.csr_col) || dev_upload(&c->d_b_isdiag, b.csr_isdiag) ||
-        dev_upload(&c->d_b_mass, b.mass) || dev_upload(&c->d_b_ent_ys, b.ent_ys))
+        dev_upload(&c->d_b_mass, b.mass) || dev_upload(&c->d_b_ent_ys, b.ent_ys) ||
+        dev_upload(&c->d_b_ent_dst, b.ent_dst))
         return NUPPN_ERR_HIP;
     const SegChunks *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     int npart = 1;
@@ -321,7 +325,7 @@ static int setup_batched(nuppn_ctx *c) {
             return NUPPN_ERR_HIP;
         npart = std::max(npart, (int) sc[i]->ch_beg.size() - 1);
     }
-    c->blayout = bslot_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
+    c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
                               b.nbd, npart);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
     c->blds32 = rnd(nb == 64 ? block_lds_bytes<float, 64>(b.nip, b.nbd)
@@ -341,6 +345,16 @@ static int setup_batched(nuppn_ctx *c) {
     c->bmax_slots = std::max(1, std::min(ncu * std::max(occ32, occ64), by_mem));
     c->bgrid64 = std::max(1, std::min(ncu * occ64, c->bmax_slots));
     c->bnb = nb;
+    return 0;
+}
+
+static int ensure_bfac(nuppn_ctx *c, int nz) {
+    if (nz <= c->nz_fcap) return 0;
+    dev_free(c->d_bfac);
+    c->d_bfac = nullptr;
+    c->nz_fcap = 0;
+    HIP_OK(hipMalloc((void **) &c->d_bfac, c->blayout.fbytes * (size_t) nz));
+    c->nz_fcap = nz;
     return 0;
 }
 
@@ -567,7 +581,8 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagk,
                     (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_k, (void *) c->d_b_ent_loc,
                     (void *) c->d_b_ent_col, (void *) c->d_b_csr_col, (void *) c->d_b_isdiag,
-                    (void *) c->d_b_mass, (void *) c->d_b_ent_ys, (void *) c->d_bslots})
+                    (void *) c->d_b_mass, (void *) c->d_b_ent_ys, (void *) c->d_bslots,
+                    (void *) c->d_b_ent_dst, (void *) c->d_bfac})
         dev_free(p);
     for (void *p : c->d_seg_i) dev_free(p);
     for (void *p : c->d_seg_c) dev_free(p);
@@ -719,7 +734,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
                      hipStream_t s) {
     const bool block = bo != nullptr;
     if (block) {
-        if (ensure_bslots(c, nz)) return NUPPN_ERR_HIP;
+        if (ensure_bslots(c, nz) || ensure_bfac(c, nz)) return NUPPN_ERR_HIP;
     } else if (ensure_slots(c, nz)) {
         return NUPPN_ERR_HIP;
     }
@@ -766,6 +781,9 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         b.slot_base = c->d_bslots;
         b.slot_bytes = c->blayout.bytes;
         for (int i = 0; i < B_COUNT; ++i) b.off[i] = c->blayout.off[i];
+        b.fac_base = c->d_bfac;
+        b.fac_bytes = c->blayout.fbytes;
+        for (int i = 0; i < F_COUNT; ++i) b.foff[i] = c->blayout.foff[i];
     }
     const int grid_n = std::min(nz, block ? c->bslots : c->nslots);
     const int grid64 = std::min(grid_n, c->bgrid64);

@@ -405,6 +405,14 @@ static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockS
     for (int k = 0; k < nnz; ++k) q2k[b.csr_q[k]] = k;
     b.ent_k.resize(b.ent_q.size());
     for (size_t e = 0; e < b.ent_q.size(); ++e) b.ent_k[e] = q2k[b.ent_q[e]];
+    const int nblk = b.nblk, BS = b.NB * b.NB;
+    b.ent_dst.assign(b.ent_ptr[3 * nblk - 2], 0);
+    for (int L = 0; L < 3 * nblk - 2; ++L) {
+        const int area = L < nblk ? 0 : (L < 2 * nblk - 1 ? 1 : 2);
+        const int blk = L < nblk ? L : (L < 2 * nblk - 1 ? L - nblk : L - (2 * nblk - 1));
+        for (int e = b.ent_ptr[L]; e < b.ent_ptr[L + 1]; ++e)
+            b.ent_dst[e] = (area * nblk + blk) * BS + b.ent_loc[e];
+    }
     b.diagk.resize(n);
     for (int c = 0; c < n; ++c) b.diagk[c] = q2k[b.diagq[c]];
     b.csr_isdiag.resize(nnz);

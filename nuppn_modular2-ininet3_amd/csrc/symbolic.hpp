@@ -73,6 +73,9 @@ struct BlockSym {
     std::vector<int> csr_ptr, csr_q, csr_col;
     // the batched path keeps A in CSR order: ent_k / diagk are ent_q / diagq as CSR positions
     std::vector<int> ent_k, diagk;
+    // D/C/B entries: destination in the dense block storage [area][block][NB*NB] (area 0 D_i,
+    // 1 C_i, 2 B_i; column-major inside a block)
+    std::vector<int> ent_dst;
     std::vector<unsigned char> csr_isdiag;
     // chunked sums: J entries (CSR order) from the per-reaction stems [st1 | st2] (2 nre),
     // dX/dt of reduced species from the fluxes vx (nre, coefficients include the mass number),
