@@ -150,23 +150,28 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
 
 /* Options of the batched path (nuppn_integrate_batch_mixed): the reduced system is factored
  * without pivoting in a block-tridiagonal + border form (molar units, columns scaled by the
- * diagonal), in FP32 (factor_precision 32) or FP64 (64).  With FP32 factors every solve is
- * refined in FP64 against the FP64 matrix until each correction satisfies |d_i| <= refine_tol *
- * (|x_i| + |y_i| + 1e-30), y the current abundances; when that refinement stalls above accept_tol
- * the zone switches to FP64 factors (direct solve plus one refinement step) for the rest of the
- * call. */
+ * diagonal), in FP32 (factor_precision 32) or FP64 (64).  Every solve is refined in FP64 against
+ * the FP64 matrix, x_{k+1} = x_k + (LU)^-1 (rhs - A x_k), and stops when
+ *   - each correction satisfies |d_i| <= refine_tol * (|x_i| + |y_i| + 1e-30), y the current
+ *     abundances, or
+ *   - x is backward stable: max_i |rhs - A x|_i / (|A| |x| + |rhs| + 1e-30)_i <= accept_tol (x
+ *     solves a system within accept_tol of A and rhs, entry by entry; rows whose magnitude is
+ *     below the 1e-30 abundance floor do not count).
+ * When the corrections stop contracting, or max_refine steps pass, with FP32 factors, the zone
+ * switches to FP64 factors for the rest of the call and the iteration is redone; with FP64
+ * factors the last iterate is kept. */
 typedef struct {
     int factor_precision;   /* 32 (default) or 64 */
-    int max_refine;         /* refinement steps per solve with FP32 factors (8) */
+    int max_refine;         /* refinement steps per solve (8) */
     double refine_tol;      /* 1e-13 */
-    double accept_tol;      /* 1e-10 */
+    double accept_tol;      /* 1e-12 */
 } nuppn_batch_opts;
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o);
 
 /* Block structure of the batched path: info[0..6] = block size NB (0: unavailable), blocks,
  * border species, interior species, interior half-bandwidth, resident slots, LDS bytes per
- * slot (FP32 factors).  Returns NUPPN_OK or NUPPN_ERR_UNSUPPORTED. */
+ * slot.  Returns NUPPN_OK or NUPPN_ERR_UNSUPPORTED. */
 int nuppn_batch_block_info(nuppn_ctx *ctx, int *info, int ninfo);
 
 /* Batched mixed-precision path: all zones of an mppnp cycle (replaces the worker loop of

@@ -10,13 +10,16 @@
 // stopping when every correction is small against its own species, |d_i| <= refine_tol *
 // (|x_i| + |y_i| + 1e-30) with y the current abundances (a normwise test would leave the
 // abundances of rare species, 20 orders of magnitude below the dominant ones, unconverged; the
-// 1e-30 floor ignores species far below any reported abundance).
-// When the corrections stop shrinking (a contraction worse than 1/2 per step) with FP32 factors,
-// the solve is accepted if an earlier step contracted strongly (the stall is then the FP64
-// residual's rounding floor); otherwise cond(A) * eps(T) is not small and the zone is marked for
-// FP64 factors (for the rest of the call) and left in the Newton phase: the FP64 kernel,
-// launched after the FP32 one in the same pass, redoes the iteration.  FP64 factors give the direct solve plus one
-// refinement step, accepted as the reference accepts its single GEPP solve.
+// 1e-30 floor ignores species far below any reported abundance), or when x is backward stable
+// entry by entry, max_i |r_i| / (|A| |x| + |rhs|)_i <= accept_tol (Oettli-Prager; the
+// correction test alone cannot stop on components whose condition number is near 1/eps(FP64),
+// where the FP64 residual's own rounding keeps moving them).  The same stopping rule holds for
+// both factor precisions, so FP32-factor and FP64-factor zones converge to the same solution.
+// When the corrections stop shrinking (a contraction worse than 1/2 per step) or the step limit
+// passes with FP32 factors, cond(A) * eps(FP32) is not small: the zone is marked for FP64
+// factors (for the rest of the call) and left in the Newton phase, and the FP64 kernel,
+// launched after the FP32 one in the same pass, redoes the iteration.  With FP64 factors the
+// last iterate is kept.
 #pragma once
 #include "block_lu.hpp"
 #include "zone.hpp"
@@ -25,23 +28,21 @@ namespace nuppn {
 
 struct BatchOpts {
     int fp64_factors;   // 1: FP64 factors from the start (no FP32 stage)
-    int max_refine;     // refinement steps per solve with FP32 factors
+    int max_refine;     // refinement steps per solve
     double refine_tol;  // stop when every |d_i| <= refine_tol * (|x_i| + |y_i| + 1e-30)
-    double accept_tol;  // a stalled FP32 refinement is accepted below this (same measure)
+    double accept_tol;  // stop when max_i |r_i| / (|A| |x| + |rhs|)_i <= accept_tol
 };
 
 // per-slot scratch of the block Newton kernels (one slot per resident wavefront)
 enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_PART, B_COUNT };
 
-// per-zone factor storage: kept between Newton iterations, so an iteration can reuse the
-// previous iteration's factors as its preconditioner
-enum BFacArr { F_CS, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
+// the factors, in the same slot after the arrays above
+enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
 
 struct BLayout {
     size_t off[B_COUNT];
-    size_t bytes;
     size_t foff[F_COUNT];
-    size_t fbytes;
+    size_t bytes;   // one slot: arrays, then factors
 };
 
 inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
@@ -49,21 +50,20 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
                                  8ull * n, 8ull * n, 8ull * n, 8ull * npart};
-    const size_t flen[F_COUNT] = {8ull * n, bs, bs, bs, 8ull * nbd * nip + 8,
-                                  8ull * nbd * nip + 8, 8ull * nbd * nbd + 8, 4ull * nbd + 4};
+    const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
+                                  8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
+                                  4ull * nbd + 4};
     BLayout L;
     size_t o = 0;
     for (int i = 0; i < B_COUNT; ++i) {
         L.off[i] = o;
         o += (len[i] + 255) & ~(size_t) 255;
     }
-    L.bytes = o;
-    o = 0;
     for (int i = 0; i < F_COUNT; ++i) {
         L.foff[i] = o;
         o += (flen[i] + 255) & ~(size_t) 255;
     }
-    L.fbytes = o;
+    L.bytes = o;
     return L;
 }
 
@@ -73,8 +73,6 @@ struct BKArgs {
     char *slot_base;
     size_t slot_bytes;
     size_t off[B_COUNT];
-    char *fac_base;      // [nz] factor areas
-    size_t fac_bytes;
     size_t foff[F_COUNT];
 };
 
@@ -89,96 +87,62 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
     F.G = (double *) (base + foff[F_G]);
     F.gpiv = (int *) (base + foff[F_GPIV]);
     F.cs = (double *) (base + foff[F_CS]);
+    F.colsc = (double *) (base + foff[F_COLSC]);
+    F.rowinv = (double *) (base + foff[F_ROWINV]);
     return F;
 }
 
-// Refined solve of the assembled system (A in CSR order) with factors in precision T.  fstate
-// holds the zone's factor state (0 none, 1 FP32, 2 FP64): factors of the previous Newton
-// iteration are reused as the preconditioner while they contract well (first correction below
-// reuse_tol), and refreshed otherwise.  y: current abundances (species order) for the
-// componentwise test.  Returns 0 (x holds the solution), 1 when FP32 factors cannot deliver the
-// FP64 solution (the zone moves to FP64), or -4 when an FP64 factorisation fails.
+// Refined solve of the assembled system (A in CSR order) with fresh factors in precision T
+// (factor area fac).  y: current abundances (species order) for the correction test.  Returns
+// 0 (x holds the solution), 1 (FP32 factors cannot deliver it: the caller switches to FP64
+// factors) or -4 (FP64 factorisation failed: the reference's singular-matrix outcome).
 template <class T, int NB>
 __device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
                                    const double *rhs, const double *y, double *x, double *res,
                                    double *d, double *part, char *fac, NUPPN_LDS char *lds,
-                                   int &fstate, ZoneStats &st) {
+                                   ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
-    constexpr int mine = fp64 ? 2 : 1;
-    constexpr double reuse_tol = 0.05;
     const DevBlock &B = b.B;
     const int n = B.n;
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
-    bool fresh = false;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        if (fstate != mine) {
-            const int fail = block_factor<T, NB>(B, Aval, F, L);
-            st.n_factor++;
-            if (fp64) st.n_factor_fp64++;
-            if (fail) {
-                fstate = 0;
-                return fp64 ? -4 : 1;
-            }
-            fstate = mine;
-            fresh = true;
-        }
+    const int fail = block_factor<T, NB>(B, Aval, F, L);
+    st.n_factor++;
+    if (fp64) st.n_factor_fp64++;
+    if (fail) return fp64 ? -4 : 1;
+    for (int i = lane_id(); i < n; i += WAVE) {
+        x[i] = 0.0;
+        res[i] = rhs[i];
+    }
+    wave_sync();
+    double eta_prev = 0.0;
+    for (int k = 0; k < b.O.max_refine; ++k) {
+        block_solve<T, NB>(B, F, L, res, d);
+        st.n_solve++;
+        double eta = 0.0;
+        bool fin = true;
         for (int i = lane_id(); i < n; i += WAVE) {
-            x[i] = 0.0;
-            res[i] = rhs[i];
+            const double xi = x[i] + d[i];
+            x[i] = xi;
+            const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + kAbundanceFloor;
+            eta = fmax(eta, fabs(d[i]) / scale);
+            fin = fin && isfinite(xi);
         }
+        eta = wave_max(eta);
         wave_sync();
-        const int steps = (fp64 && fresh) ? 2 : b.O.max_refine;
-        double eta_prev = 0.0, best_ratio = 1.0;
-        bool refresh = false;
-        for (int k = 0; k < steps; ++k) {
-            block_solve<T, NB>(B, F, L, res, d);
-            st.n_solve++;
-            double eta = 0.0;
-            bool fin = true;
-            for (int i = lane_id(); i < n; i += WAVE) {
-                const double xi = x[i] + d[i];
-                x[i] = xi;
-                const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + 1.0e-30;
-                eta = fmax(eta, fabs(d[i]) / scale);
-                fin = fin && isfinite(xi);
-            }
-            eta = wave_max(eta);
-            wave_sync();
-            if (ballot(!fin)) {
-                if (!fresh) { refresh = true; break; }
-                fstate = 0;
-                return fp64 ? -4 : 1;
-            }
-            if (eta <= b.O.refine_tol || (fp64 && fresh && k == steps - 1)) return 0;
-            // stale factors that no longer contract well are refreshed
-            if (!fresh && k == 1 && eta > reuse_tol) { refresh = true; break; }
-            // contraction from the second correction on (the first solve's eta is 1 by
-            // construction: x starts at zero).  A stall after a strong contraction is the FP64
-            // residual's own rounding floor and is accepted; a stall without one means
-            // cond(A) * eps(T) is not small, and FP32 factors cannot deliver the FP64 solution.
-            if (k > 1) {
-                const double ratio = eta / eta_prev;
-                if (ratio > 0.5) {
-                    if (best_ratio <= 0.1) return 0;
-                    if (!fresh) { refresh = true; break; }
-                    fstate = 0;
-                    return fp64 ? 0 : 1;
-                }
-                best_ratio = fmin(best_ratio, ratio);
-            }
-            if (k == steps - 1) {
-                if (eta <= b.O.accept_tol) return 0;
-                if (!fresh) { refresh = true; break; }
-                fstate = 0;
-                return fp64 ? 0 : 1;
-            }
-            eta_prev = eta;
-            block_residual(B, Aval, rhs, x, res, part);
-            st.n_refine++;
-        }
-        if (!refresh) return 0;
-        fstate = 0;   // refactor with the current matrix and solve again
+        if (ballot(!fin)) return fp64 ? -4 : 1;
+        if (eta <= b.O.refine_tol) return 0;   // converged corrections
+        // x is accepted once it is backward stable in FP64: it solves a system within
+        // accept_tol, entry by entry, of A and rhs
+        const double omega = block_residual(B, Aval, rhs, x, res, part);
+        st.n_refine++;
+        if (omega <= b.O.accept_tol) return 0;
+        // otherwise the corrections must keep contracting (the first correction's eta is
+        // about 1 by construction, so contraction is judged from the second one on).  A
+        // stall or the step limit with FP32 factors means cond(A) * eps(FP32) is not small
+        // and the FP64 factors take over; with FP64 factors the last iterate is the result.
+        if ((k > 1 && eta > 0.5 * eta_prev) || k == b.O.max_refine - 1) return fp64 ? 0 : 1;
+        eta_prev = eta;
     }
     return 0;
 }
@@ -211,21 +175,18 @@ __device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double
     wave_sync();
 }
 
-// One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z with the
-// block solve in precision T; the zone must be in PH_NEWTON with factor precision T (state
-// pad0: 0 FP32, 1 FP64).  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON, or raises
-// the zone to FP64 and leaves it in PH_NEWTON.
-template <class T, int NB>
+// One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z in PH_NEWTON:
+// the solve with FP32 factors while the zone's factor precision (state pad0) is FP32, and with
+// FP64 factors when that fails (the zone then stays on FP64 for the rest of the call) or when
+// the zone is on FP64 already.  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON.
+template <int NB>
 __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
                                   NUPPN_LDS char *lds, int z) {
-    // zone state: pad0 factor precision (0 FP32, 1 FP64), pad1 factor state (0 none, 1 FP32
-    // factors, 2 FP64 factors) of the zone's factor area
-    constexpr int want = sizeof(T) == 8 ? 1 : 0;
     const DevNet &N = a.N;
     const DevKnobs &K = a.K;
     const ZoneData &Z = a.Z;
     ZoneState s = Z.state[z];
-    if (s.phase != PH_NEWTON || s.pad0 != want) return;
+    if (s.phase != PH_NEWTON) return;
     const double *y_old = zrow(Z.y_old, Z.nsp_stride, z);
     const double *y_cur = zrow(Z.y_cur, Z.nsp_stride, z);
     double *y_new = zrow(Z.y_new, Z.nsp_stride, z);
@@ -241,16 +202,15 @@ __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
     double *part = (double *) (slot + b.off[B_PART]);
     const double dt = s.dttry;
     block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs, part);
-    char *fac = b.fac_base + (size_t) z * b.fac_bytes;
-    const int rc = block_refined_solve<T, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, fac, lds,
-                                              s.pad1, s.st);
-    if (rc == 1) {   // FP32 factors insufficient: the FP64 kernel redoes this iteration
-        s.pad0 = 1;
-        s.pad1 = 0;
-        if (lane_id() == 0) Z.state[z] = s;
-        wave_sync();
-        return;
+    int rc = 1;
+    if (s.pad0 == 0) {
+        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
+                                            s.st);
+        if (rc == 1) s.pad0 = 1;
     }
+    if (rc == 1)
+        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
+                                             s.st);
     if (rc == 0) {
         for (int r = lane_id(); r < N.n; r += WAVE) {
             const int sp = N.red2sp[r];
@@ -272,14 +232,14 @@ __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
     wave_sync();
 }
 
-// every zone in PH_NEWTON at precision T; each wavefront owns one scratch slot and strides over
-// the zones
-template <class T, int NB>
+// every zone in PH_NEWTON; each wavefront owns one scratch slot (arrays and factors) and
+// strides over the zones
+template <int NB>
 __global__ void __launch_bounds__(64) zone_newton_block_kernel(KArgs a, BKArgs b) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton_block<T, NB>(a, b, slot, lds, z);
+    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton_block<NB>(a, b, slot, lds, z);
 }
 
 }  // namespace nuppn

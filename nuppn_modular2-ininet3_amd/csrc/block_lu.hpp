@@ -53,6 +53,8 @@ struct BlockFactors {
     double *G;          // [nbd*nbd] LU of G' (column-major), unit lower part below the diagonal
     int *gpiv;          // [nbd] row interchanges of the LU of G'
     double *cs;         // [n] 1 / diagonal of A: the column scaling
+    double *colsc;      // [nip + nbd] by padded position: mass * column scaling
+    double *rowinv;     // [nip + nbd] by padded position: 1 / mass
 };
 
 template <class T, int NB>
@@ -315,6 +317,11 @@ __device__ inline void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
     }
 }
 
+// mass fractions far below any reported abundance (1e-20) and below the reference's clipping
+// of abundances to 1e-99 (y < 1e-27, utils.F90:726-740): the refinement tests measure every
+// species against this floor at least
+constexpr double kAbundanceFloor = 1.0e-30;
+
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
@@ -336,6 +343,17 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
         F.cs[i] = 1.0 / d;
     }
     if (ballot(bad)) return 1;
+    wave_sync();
+    // scalings by padded position (interior blocks, then border): padding rows/columns stay 1
+    for (int p = B.ni + lane; p < B.nip; p += WAVE) {
+        F.colsc[p] = 1.0;
+        F.rowinv[p] = 1.0;
+    }
+    for (int i = lane; i < B.n; i += WAVE) {
+        const int p = B.vpos[i];
+        F.colsc[p] = B.mass[i] * F.cs[i];
+        F.rowinv[p] = 1.0 / B.mass[i];
+    }
     wave_sync();
     // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
     // where W_i goes, B_i in place (the loop below then only streams them, prefetched)
@@ -461,8 +479,16 @@ __device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const B
                             const double *b, double *x) {
     const int lane = lane_id(), nip = B.nip, nbd = B.nbd;
     NUPPN_LDS T *v = L.vec;
+    // an exact power of two brings the largest entry to [1, 2): FP32 then resolves entries down
+    // to 2^-126 of the largest one rather than 2^-126 absolute (refinement residuals shrink far
+    // below the FP32 range while the abundances they correct stay meaningful)
+    double m = 0.0;
+    for (int c = lane; c < B.n; c += WAVE) m = fmax(m, fabs(b[c] * F.rowinv[B.vpos[c]]));
+    m = wave_max(m);
+    const int ex = (m > 0.0 && isfinite(m)) ? ilogb(m) : 0;
+    const double sc = ldexp(1.0, -ex), unsc = ldexp(1.0, ex);
     for (int j = B.ni + lane; j < nip; j += WAVE) v[j] = T(0);
-    for (int c = lane; c < B.n; c += WAVE) v[B.vpos[c]] = (T) (b[c] / B.mass[c]);
+    for (int c = lane; c < B.n; c += WAVE) v[B.vpos[c]] = (T) (b[c] * F.rowinv[B.vpos[c]] * sc);
     lds_fence();
     thomas_solve<T, NB>(B, F, v);
     if (nbd > 0) {
@@ -496,7 +522,7 @@ __device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const B
         }
         lds_fence();
     }
-    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[B.vpos[c]] * B.mass[c] * F.cs[c];
+    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[B.vpos[c]] * F.colsc[B.vpos[c]] * unsc;
     wave_sync();
 }
 
@@ -526,16 +552,50 @@ __device__ __forceinline__ double seg_total(const DevSeg &S, const double *part,
     return t;
 }
 
-// r = rhs - A x (A in CSR order), FP64
-__device__ inline void block_residual(const DevBlock &B, const double *Acsr, const double *rhs,
-                                      const double *x, double *r, double *part) {
+// r = rhs - A x (A in CSR order), FP64.  Returns the componentwise backward error of x,
+// max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager): x solves a system whose
+// every entry is within that relative distance of A and rhs, ignoring rows whose whole
+// magnitude is below the abundance floor.  part holds 2 x nch partials.
+__device__ inline double block_residual(const DevBlock &B, const double *Acsr, const double *rhs,
+                                        const double *x, double *r, double *part) {
     const DevSeg &S = B.resid;
-    seg_chunks(S, part, [&](int e) {
-        const int k = S.e_idx[e];
-        return Acsr[k] * x[B.csr_col[k]];
-    });
-    for (int i = lane_id(); i < B.n; i += WAVE) r[i] = rhs[i] - seg_total(S, part, i);
+    double *pabs = part + S.nch;
+    for (int c = lane_id(); c < S.nch; c += WAVE) {
+        const int e0 = S.ch_beg[c], e1 = S.ch_beg[c + 1];
+        double s0 = 0.0, s1 = 0.0, a0 = 0.0, a1 = 0.0;
+        int e = e0;
+        for (; e + 1 < e1; e += 2) {
+            const int k0 = S.e_idx[e], k1 = S.e_idx[e + 1];
+            const double t0 = Acsr[k0] * x[B.csr_col[k0]], t1 = Acsr[k1] * x[B.csr_col[k1]];
+            s0 += t0;
+            s1 += t1;
+            a0 += fabs(t0);
+            a1 += fabs(t1);
+        }
+        if (e < e1) {
+            const int k0 = S.e_idx[e];
+            const double t0 = Acsr[k0] * x[B.csr_col[k0]];
+            s0 += t0;
+            a0 += fabs(t0);
+        }
+        part[c] = s0 + s1;
+        pabs[c] = a0 + a1;
+    }
     wave_sync();
+    double om = 0.0;
+    for (int i = lane_id(); i < B.n; i += WAVE) {
+        double t = 0.0, a = 0.0;
+        for (int c = S.seg_ch[i]; c < S.seg_ch[i + 1]; ++c) {
+            t += part[c];
+            a += pabs[c];
+        }
+        const double ri = rhs[i] - t;
+        r[i] = ri;
+        om = fmax(om, fabs(ri) / (a + fabs(rhs[i]) + kAbundanceFloor));
+    }
+    om = wave_max(om);
+    wave_sync();
+    return om;
 }
 
 }  // namespace nuppn

@@ -79,10 +79,8 @@ __global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
 __global__ void flag_reset_kernel(int *flag) { flag[0] = 0; }
 
 namespace nuppn {
-template __global__ void zone_newton_block_kernel<float, 32>(KArgs a, BKArgs b);
-template __global__ void zone_newton_block_kernel<double, 32>(KArgs a, BKArgs b);
-template __global__ void zone_newton_block_kernel<float, 64>(KArgs a, BKArgs b);
-template __global__ void zone_newton_block_kernel<double, 64>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<32>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<64>(KArgs a, BKArgs b);
 }  // namespace nuppn
 
 // single-state assembly entries (test hooks): mode 0 dX/dt, mode 1 reduced matrix values
@@ -195,10 +193,8 @@ struct nuppn_ctx {
     int *d_seg_i[9] = {};         // jac, dxdt, resid: seg_ch, ch_beg, e_idx
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     BLayout blayout{};
-    int nz_fcap = 0;
-    char *d_bfac = nullptr;   // per-zone factor areas of the batched path
-    size_t blds32 = 0, blds64 = 0;
-    int bmax_slots = 0, bslots = 0, bgrid64 = 0;
+    size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
+    int bmax_slots = 0, bslots = 0;
     char *d_bslots = nullptr;
     hipEvent_t ev[4] = {};
     hipEvent_t evn[2] = {}, evc[2] = {};
@@ -281,12 +277,9 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     return B;
 }
 
-static const void *block_kernel_ptr(int nb, bool fp64) {
-    if (nb == 64)
-        return fp64 ? (const void *) zone_newton_block_kernel<double, 64>
-                    : (const void *) zone_newton_block_kernel<float, 64>;
-    return fp64 ? (const void *) zone_newton_block_kernel<double, 32>
-                : (const void *) zone_newton_block_kernel<float, 32>;
+static const void *block_kernel_ptr(int nb) {
+    return nb == 64 ? (const void *) zone_newton_block_kernel<64>
+                    : (const void *) zone_newton_block_kernel<32>;
 }
 
 // resident wavefronts per CU of a kernel with `lds` bytes of dynamic LDS
@@ -323,7 +316,8 @@ static int setup_batched(nuppn_ctx *c) {
             dev_upload(&c->d_seg_i[3 * i + 2], sc[i]->e_idx) ||
             dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
             return NUPPN_ERR_HIP;
-        npart = std::max(npart, (int) sc[i]->ch_beg.size() - 1);
+        // the residual keeps signed and absolute partials
+        npart = std::max(npart, (i == 2 ? 2 : 1) * ((int) sc[i]->ch_beg.size() - 1));
     }
     c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
                               b.nbd, npart);
@@ -333,28 +327,17 @@ static int setup_batched(nuppn_ctx *c) {
     c->blds64 = rnd(nb == 64 ? block_lds_bytes<double, 64>(b.nip, b.nbd)
                              : block_lds_bytes<double, 32>(b.nip, b.nbd));
     if (c->blds64 > 160 * 1024) return 0;
-    const int occ32 = kernel_occupancy(block_kernel_ptr(nb, false), c->blds32);
-    const int occ64 = kernel_occupancy(block_kernel_ptr(nb, true), c->blds64);
-    if (occ32 <= 0 || occ64 <= 0) return 0;
+    // one kernel runs both factor precisions: it takes the FP64 LDS size
+    const int occ = kernel_occupancy(block_kernel_ptr(nb), c->blds64);
+    if (occ <= 0) return 0;
     int dev = 0, ncu = 0;
     (void) hipGetDevice(&dev);
     (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     size_t free_b = 0, total_b = 0;
     (void) hipMemGetInfo(&free_b, &total_b);
     const int by_mem = (int) std::max<size_t>(1, (free_b / 4) / c->blayout.bytes);
-    c->bmax_slots = std::max(1, std::min(ncu * std::max(occ32, occ64), by_mem));
-    c->bgrid64 = std::max(1, std::min(ncu * occ64, c->bmax_slots));
+    c->bmax_slots = std::max(1, std::min(ncu * occ, by_mem));
     c->bnb = nb;
-    return 0;
-}
-
-static int ensure_bfac(nuppn_ctx *c, int nz) {
-    if (nz <= c->nz_fcap) return 0;
-    dev_free(c->d_bfac);
-    c->d_bfac = nullptr;
-    c->nz_fcap = 0;
-    HIP_OK(hipMalloc((void **) &c->d_bfac, c->blayout.fbytes * (size_t) nz));
-    c->nz_fcap = nz;
     return 0;
 }
 
@@ -582,7 +565,7 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_k, (void *) c->d_b_ent_loc,
                     (void *) c->d_b_ent_col, (void *) c->d_b_csr_col, (void *) c->d_b_isdiag,
                     (void *) c->d_b_mass, (void *) c->d_b_ent_ys, (void *) c->d_bslots,
-                    (void *) c->d_b_ent_dst, (void *) c->d_bfac})
+                    (void *) c->d_b_ent_dst})
         dev_free(p);
     for (void *p : c->d_seg_i) dev_free(p);
     for (void *p : c->d_seg_c) dev_free(p);
@@ -734,7 +717,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
                      hipStream_t s) {
     const bool block = bo != nullptr;
     if (block) {
-        if (ensure_bslots(c, nz) || ensure_bfac(c, nz)) return NUPPN_ERR_HIP;
+        if (ensure_bslots(c, nz)) return NUPPN_ERR_HIP;
     } else if (ensure_slots(c, nz)) {
         return NUPPN_ERR_HIP;
     }
@@ -781,12 +764,9 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         b.slot_base = c->d_bslots;
         b.slot_bytes = c->blayout.bytes;
         for (int i = 0; i < B_COUNT; ++i) b.off[i] = c->blayout.off[i];
-        b.fac_base = c->d_bfac;
-        b.fac_bytes = c->blayout.fbytes;
         for (int i = 0; i < F_COUNT; ++i) b.foff[i] = c->blayout.foff[i];
     }
     const int grid_n = std::min(nz, block ? c->bslots : c->nslots);
-    const int grid64 = std::min(grid_n, c->bgrid64);
     HIP_OK(hipEventRecord(c->ev[2], s));
     hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), 0, s, a);
     HIP_OK(hipGetLastError());
@@ -819,15 +799,11 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         if (!block) {
             hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
         } else if (c->bnb == 64) {
-            hipLaunchKernelGGL((zone_newton_block_kernel<float, 64>), dim3(grid_n), dim3(64),
-                               c->blds32, s, a, b);
-            hipLaunchKernelGGL((zone_newton_block_kernel<double, 64>), dim3(grid64), dim3(64),
-                               c->blds64, s, a, b);
+            hipLaunchKernelGGL((zone_newton_block_kernel<64>), dim3(grid_n), dim3(64), c->blds64,
+                               s, a, b);
         } else {
-            hipLaunchKernelGGL((zone_newton_block_kernel<float, 32>), dim3(grid_n), dim3(64),
-                               c->blds32, s, a, b);
-            hipLaunchKernelGGL((zone_newton_block_kernel<double, 32>), dim3(grid64), dim3(64),
-                               c->blds64, s, a, b);
+            hipLaunchKernelGGL((zone_newton_block_kernel<32>), dim3(grid_n), dim3(64), c->blds64,
+                               s, a, b);
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evn[1], s));
@@ -899,13 +875,13 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->factor_precision = 32;
     o->max_refine = 8;
     o->refine_tol = 1.0e-13;
-    o->accept_tol = 1.0e-10;
+    o->accept_tol = 1.0e-12;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {
     if (!c || !info || ninfo <= 0) return NUPPN_ERR_ARG;
     const BlockSym &b = c->bsym;
-    const int vals[7] = {c->bnb, b.nblk, b.nbd, b.ni, b.bandwidth, c->bmax_slots, (int) c->blds32};
+    const int vals[7] = {c->bnb, b.nblk, b.nbd, b.ni, b.bandwidth, c->bmax_slots, (int) c->blds64};
     for (int i = 0; i < ninfo && i < 7; ++i) info[i] = vals[i];
     return c->bnb ? NUPPN_OK : NUPPN_ERR_UNSUPPORTED;
 }
