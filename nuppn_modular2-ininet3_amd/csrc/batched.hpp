@@ -21,6 +21,7 @@
 // launched after the FP32 one in the same pass, redoes the iteration.  With FP64 factors the
 // last iterate is kept.
 #pragma once
+#include "bd.hpp"
 #include "block_lu.hpp"
 #include "zone.hpp"
 
@@ -34,7 +35,9 @@ struct BatchOpts {
 };
 
 // per-slot scratch of the block Newton kernels (one slot per resident wavefront)
-enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_PART, B_COUNT };
+// (B_JAC .. B_SORT: the Bader-Deuflhard step's J, tableau and work vectors)
+enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_PART, B_JAC, B_YK, B_YEXT,
+                B_YSCAL, B_DELTA, B_QCOL, B_YERR, B_SORT, B_COUNT };
 
 // the factors, in the same slot after the arrays above
 enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
@@ -48,8 +51,12 @@ struct BLayout {
 inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
                             int nbd, int npart) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
+    size_t pow2 = 1;
+    while (pow2 < (size_t) nsp) pow2 <<= 1;
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
-                                 8ull * n, 8ull * n, 8ull * n, 8ull * npart};
+                                 8ull * n, 8ull * n, 8ull * n, 8ull * npart, 8ull * nnzA,
+                                 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
+                                 64ull * nsp, 8ull * pow2};
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                   4ull * nbd + 4};
@@ -92,24 +99,33 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
     return F;
 }
 
-// Refined solve of the assembled system (A in CSR order) with fresh factors in precision T
-// (factor area fac).  y: current abundances (species order) for the correction test.  Returns
-// 0 (x holds the solution), 1 (FP32 factors cannot deliver it: the caller switches to FP64
-// factors) or -4 (FP64 factorisation failed: the reference's singular-matrix outcome).
+// Factors of the assembled matrix (A in CSR order) in precision T into the factor area fac.
+// Returns 0, or 1 when a pivot or the border Schur complement is not finite or zero.
 template <class T, int NB>
-__device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
-                                   const double *rhs, const double *y, double *x, double *res,
-                                   double *d, double *part, char *fac, NUPPN_LDS char *lds,
-                                   ZoneStats &st) {
+__device__ int block_factor_zone(const BKArgs &b, const double *Aval, char *fac,
+                                 NUPPN_LDS char *lds, ZoneStats &st) {
+    const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
+    const BlockLds<T> L = make_block_lds<T, NB>(lds, b.B.nip);
+    const int fail = block_factor<T, NB>(b.B, Aval, F, L);
+    st.n_factor++;
+    if (sizeof(T) == 8) st.n_factor_fp64++;
+    return fail;
+}
+
+// Refined solve of A x = rhs with the precision-T factors in fac.  y: current abundances
+// (species order) for the correction test.  Returns 0 (x holds the solution), 1 (FP32 factors
+// cannot deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64
+// factors: the reference's singular-matrix outcome).
+template <class T, int NB>
+__device__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
+                            const double *rhs, const double *y, double *x, double *res,
+                            double *d, double *part, char *fac, NUPPN_LDS char *lds,
+                            ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
     const DevBlock &B = b.B;
     const int n = B.n;
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
-    const int fail = block_factor<T, NB>(B, Aval, F, L);
-    st.n_factor++;
-    if (fp64) st.n_factor_fp64++;
-    if (fail) return fp64 ? -4 : 1;
     for (int i = lane_id(); i < n; i += WAVE) {
         x[i] = 0.0;
         res[i] = rhs[i];
@@ -145,6 +161,18 @@ __device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const doubl
         eta_prev = eta;
     }
     return 0;
+}
+
+
+// Refined solve with fresh factors (the backward-Euler Newton iteration: one solve per
+// factorisation).  Same return codes as block_refine.
+template <class T, int NB>
+__device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
+                                   const double *rhs, const double *y, double *x, double *res,
+                                   double *d, double *part, char *fac, NUPPN_LDS char *lds,
+                                   ZoneStats &st) {
+    if (block_factor_zone<T, NB>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
+    return block_refine<T, NB>(N, b, Aval, rhs, y, x, res, d, part, fac, lds, st);
 }
 
 // the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order) and the
@@ -232,6 +260,112 @@ __device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
     wave_sync();
 }
 
+// The batched path's solver for the Bader-Deuflhard step: J once per step by the chunked sums
+// (CSR order), I - h J per level factored pivot-free in the zone's precision, every stage solve
+// refined in FP64 (FP32 factors that cannot deliver a solve switch the zone to FP64 factors,
+// which redo that solve).
+template <int NB>
+struct BlockBDSolver {
+    const KArgs &a;
+    const BKArgs &b;
+    char *slot;
+    NUPPN_LDS char *lds;
+    ZoneState &s;
+    const double *v;
+    const double *ycur;   // abundances of the current stage (the correction test's scale)
+
+    __device__ double *arr(int i) const { return (double *) (slot + b.off[i]); }
+
+    __device__ void fluxes(const double *y) {
+        const DevNet &N = a.N;
+        molar(N, y, arr(B_XN));
+        wave_sync();
+        reaction_terms(N, arr(B_XN), v, arr(B_VX), arr(B_ST), arr(B_ST) + N.nre);
+        wave_sync();
+    }
+
+    __device__ int jacobian(const double *y, double dt) {
+        (void) dt;   // min_rate_ov_dt is -1e99 on the static network
+        const DevBlock &B = b.B;
+        fluxes(y);
+        const DevSeg &J = B.jac;
+        const double *stem = arr(B_ST);
+        double *part = arr(B_PART), *jac = arr(B_JAC);
+        seg_chunks(J, part, [&](int e) { return J.e_coef[e] * stem[J.e_idx[e]]; });
+        int bad = 0;
+        for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+            const double jv = 1.0e-99 + seg_total(J, part, k);
+            jac[k] = jv;
+            if (!B.csr_isdiag[k] && jv == 0.0) bad = 1;   // calculate_jacobian stops
+        }
+        wave_sync();
+        return ballot(bad) ? E_BAD_STEP : 0;
+    }
+
+    __device__ int level(double h) {
+        const DevBlock &B = b.B;
+        const double *jac = arr(B_JAC);
+        double *Aval = arr(B_AVAL);
+        for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+            double val = jac[k] * (-h);
+            if (B.csr_isdiag[k]) val = val + 1.0;
+            Aval[k] = val;
+        }
+        wave_sync();
+        if (s.pad0 == 0 && block_factor_zone<float, NB>(b, Aval, slot, lds, s.st)) s.pad0 = 1;
+        if (s.pad0 == 1 && block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
+        return 0;
+    }
+
+    __device__ void dxdt(const double *y, double *out) {
+        const DevNet &N = a.N;
+        fluxes(y);
+        const DevSeg &D = b.B.dxdt;
+        const double *vx = arr(B_VX);
+        double *part = arr(B_PART);
+        seg_chunks(D, part, [&](int e) { return D.e_coef[e] * vx[D.e_idx[e]]; });
+        for (int r = lane_id(); r < N.n; r += WAVE) out[r] = seg_total(D, part, r);
+        wave_sync();
+    }
+
+    __device__ int solve(const double *rhs, double *x) {
+        const DevNet &N = a.N;
+        const double *Aval = arr(B_AVAL);
+        if (s.pad0 == 0) {
+            const int rc = block_refine<float, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
+                                                   arr(B_PART), slot, lds, s.st);
+            if (rc != 1) return rc;
+            s.pad0 = 1;
+            if (block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
+        }
+        // a non-finite FP64 solution is returned as it is: the step then ends in GOT_NAN, as
+        // the reference's does when its solve overflows
+        (void) block_refine<double, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
+                                        arr(B_PART), slot, lds, s.st);
+        return 0;
+    }
+};
+
+template <int NB>
+__device__ void zone_bd_block(const KArgs &a, const BKArgs &b, char *slot, NUPPN_LDS char *lds,
+                              int z) {
+    const ZoneData &Z = a.Z;
+    ZoneState s = Z.state[z];
+    if (s.phase != PH_NEWTON) return;
+    BDState bd = Z.bd[z];
+    auto arr = [&](int i) { return (double *) (slot + b.off[i]); };
+    BlockBDSolver<NB> sv{a, b, slot, lds, s, zrow(Z.v, Z.nre_stride, z), arr(B_YK)};
+    BDWork W{arr(B_YK), arr(B_YEXT), arr(B_YSCAL), arr(B_DELTA), arr(B_RHS), arr(B_X),
+             arr(B_QCOL), arr(B_YERR), arr(B_SORT)};
+    bd_step(a, z, s, bd, W, sv);
+    s.phase = PH_AFTER_NEWTON;
+    if (lane_id() == 0) {
+        Z.state[z] = s;
+        Z.bd[z] = bd;
+    }
+    wave_sync();
+}
+
 // every zone in PH_NEWTON; each wavefront owns one scratch slot (arrays and factors) and
 // strides over the zones
 template <int NB>
@@ -239,7 +373,10 @@ __global__ void __launch_bounds__(64) zone_newton_block_kernel(KArgs a, BKArgs b
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton_block<NB>(a, b, slot, lds, z);
+    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) {
+        if (a.K.integration_method == 1) zone_bd_block<NB>(a, b, slot, lds, z);
+        else zone_newton_block<NB>(a, b, slot, lds, z);
+    }
 }
 
 }  // namespace nuppn

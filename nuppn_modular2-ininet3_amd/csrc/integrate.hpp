@@ -113,10 +113,17 @@ __device__ inline void zone_rates(const DevNet &N, double t9, double rho, double
 // -------------------------------------------------------------------------------------------
 // Bader-Deuflhard (bader_deuflhard.F90:139-488).  The reduced matrix of level k is
 // I - h J with J taken once per step; every level factors once and solves nseq(k)+1 times.
+// The reference keeps this state between calls: frst is integrate_network's SAVE'd local
+// (solver.F90:75), kopt a module variable (bader_deuflhard.F90:32), reduce and the error terms
+// locals and module variables that later steps read before setting (sumerr when abuerr < 1e-3,
+// reduce when no branch sets it).
 struct BDState {
-    int frst, reduce, kopt;
+    int frst, reduce, kopt, pad;
     double errmax, abuerr, negerr, sumerr;
-    double err[8], xcol[8];
 };
+
+// errors.F90:5-16
+enum BDErr { E_SLV_CNVRG = 1, E_BAD_STEP = 3, E_NEG_ABUNDS = 4, E_HUGE_ABUNDS = 5, E_SUM_X = 6,
+             E_GOT_NAN = 7 };
 
 }  // namespace nuppn

@@ -67,13 +67,16 @@ __global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) { zone_init(a, b
 
 __global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) { zone_control(a, blockIdx.x); }
 
-// one Newton iteration for every zone in PH_NEWTON; each wavefront owns one scratch slot and
-// strides over the zones
+// one Newton iteration (or Bader-Deuflhard step) for every zone in PH_NEWTON; each wavefront
+// owns one scratch slot and strides over the zones
 __global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const LUWork w = make_work(smem, a.N.n);
     const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
-    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) zone_newton(a, S, w, z);
+    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) {
+        if (a.K.integration_method == 1) zone_bd(a, S, w, z);
+        else zone_newton(a, S, w, z);
+    }
 }
 
 __global__ void flag_reset_kernel(int *flag) { flag[0] = 0; }
@@ -175,6 +178,9 @@ struct nuppn_ctx {
     double *d_yps_old = nullptr, *d_y_old = nullptr, *d_y_cur = nullptr, *d_y_new = nullptr;
     double *d_work = nullptr;
     ZoneState *d_state = nullptr;
+    BDState *d_bd = nullptr;      // [nz] per-zone Bader-Deuflhard state of the running call
+    BDTables bdt{};               // bader_deuflhard_init's tables
+    BDState bd{};                 // the reference's SAVE'd BD state, carried between calls
     size_t work_stride = 0;
     // staging for host-array calls
     int nz_cap = 0, nz_vcap = 0;
@@ -394,7 +400,7 @@ static int ensure_rate_buffer(nuppn_ctx *c, int nz) {
 static int ensure_zone_state(nuppn_ctx *c, int nz) {
     if (nz <= c->nz_state_cap) return 0;
     for (void *p : {(void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
-                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state})
+                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd})
         dev_free(p);
     const size_t nsp = c->net.nsp;
     size_t pow2 = 1;
@@ -406,6 +412,7 @@ static int ensure_zone_state(nuppn_ctx *c, int nz) {
     HIP_OK(hipMalloc((void **) &c->d_y_new, sizeof(double) * nsp * nz));
     HIP_OK(hipMalloc((void **) &c->d_work, sizeof(double) * c->work_stride * nz));
     HIP_OK(hipMalloc((void **) &c->d_state, sizeof(ZoneState) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_bd, sizeof(BDState) * nz));
     c->nz_state_cap = nz;
     return 0;
 }
@@ -419,11 +426,45 @@ static int upload_order(nuppn_ctx *c) {
     return 0;
 }
 
+// bader_deuflhard_init (bader_deuflhard.F90:66-124) with the reference's settings (47-59):
+// use_weights, cf = 1, cj = 0, cs = 10, clr = cj, safe1 = 1/4.  nseq(1:3) = 2, 6, 10, then
+// nint(nseq(i-1) / (5/7)); work a(1) = cj + clr + (nseq(1)+1)(cf+cs), a(i) = a(i-1) +
+// nseq(i)(cf+cs) + clr + cs for i < 9 (a(9) stays 0); correction factors alf(k, iq) =
+// (dgrd safe1)^((a(k+1)-a(iq+1)) / ((a(iq+1)-a(1)+1)(2k+1))); kopt is the first row with
+// a(kopt+1) > a(kopt) alf(kopt-1, kopt) (kmax = kopt; kmax stays 0 if none).
+static void bd_tables(double dgrd, BDTables &T, BDState &bd) {
+    const double cf = 1.0, cj = 0.0, cs = 10.0, clr = 0.0, safe1 = 0.25;
+    memset(&T, 0, sizeof(T));
+    T.nseq[0] = 2;
+    T.nseq[1] = 6;
+    T.nseq[2] = 10;
+    for (int i = 3; i < 9; ++i) T.nseq[i] = (int) std::lround((double) T.nseq[i - 1] / (5.0 / 7.0));
+    T.a[0] = cj + clr + (T.nseq[0] + 1) * (cf + cs);
+    for (int i = 1; i < 8; ++i) T.a[i] = T.a[i - 1] + T.nseq[i] * (cf + cs) + clr + cs;
+    for (int iq = 2; iq <= 8; ++iq)
+        for (int k = 1; k <= iq - 1; ++k)
+            T.alf[k - 1][iq - 1] =
+                std::pow(dgrd * safe1, (T.a[k] - T.a[iq]) / ((T.a[iq] - T.a[0] + 1.0) * (2.0 * k + 1)));
+    int ko;
+    T.kmax = 0;
+    for (ko = 2; ko <= 7; ++ko)
+        if (T.a[ko] > T.a[ko - 1] * T.alf[ko - 2][ko - 1]) {
+            T.kmax = ko;
+            break;
+        }
+    T.kopt0 = ko;
+    memset(&bd, 0, sizeof(bd));
+    bd.frst = 1;
+    bd.reduce = 0;
+    bd.kopt = ko;
+}
+
 static KArgs kargs(nuppn_ctx *c) {
     KArgs a;
     memset(&a, 0, sizeof(a));
     a.N = dev_net(c);
     a.K = dev_knobs(c->kn);
+    a.T = c->bdt;
     a.slot_base = c->d_slots;
     a.slot_bytes = c->layout.bytes;
     for (int i = 0; i < S_COUNT; ++i) a.off[i] = c->layout.off[i];
@@ -471,13 +512,14 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         *status = NUPPN_ERR_UNSUPPORTED;
         return nullptr;
     }
-    if (kn.integration_method != 0) {
+    if (kn.integration_method != 0 && kn.integration_method != 1) {
         *status = NUPPN_ERR_UNSUPPORTED;
         return nullptr;
     }
     kn.min_rate_ov_dt = -1.0e99;   // solver_knobs.F90:66-69 for cyminformal < 1e-99
     auto c = std::make_unique<nuppn_ctx>();
     c->kn = kn;
+    bd_tables(kn.dgrd, c->bdt, c->bd);
     HostNetwork &h = c->net;
     h.nsp = net->nsp;
     h.nre = net->nre;
@@ -557,7 +599,7 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_iperm_c, (void *) c->d_isdiag, (void *) c->d_coef,
                     (void *) c->d_aw, (void *) c->d_zw, (void *) c->d_slots, (void *) c->d_flag,
                     (void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
-                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state,
+                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd,
                     (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
                     (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
                     (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
@@ -750,6 +792,8 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     Z.ierr = ierr;
     Z.stats = (ZoneStats *) stats;
     Z.state = c->d_state;
+    Z.bd = c->d_bd;
+    Z.bd0 = c->bd;
     Z.flag = c->d_flag;
     Z.nz = nz;
     Z.init_prec = (block && bo->factor_precision == 64) ? 1 : 0;
@@ -812,6 +856,13 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     }
     c->last_passes = pass;
     HIP_OK(hipEventRecord(c->ev[3], s));
+    if (c->kn.integration_method == 1) {
+        // the state the reference would carry into its next call: the last zone's (exact for
+        // one zone per call; a batch runs its zones from the same entry state)
+        HIP_OK(hipMemcpyAsync(&c->bd, c->d_bd + (nz - 1), sizeof(BDState), hipMemcpyDeviceToHost,
+                              s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
     return NUPPN_OK;
 }
 

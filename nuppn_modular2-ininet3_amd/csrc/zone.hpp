@@ -43,8 +43,10 @@ inline SlotLayout slot_layout(int nsp, int nre, int n, int nnzA) {
 // global memory between launches (solver.F90:65-107 locals, backward_euler.F90:31-53 locals).
 enum ZonePhase { PH_SUBSTEP = 0, PH_NEWTON = 1, PH_AFTER_NEWTON = 2, PH_DONE = 3 };
 
+// pad0: factor precision of the batched path (0 FP32, 1 FP64); step_ierr: the ierr of the last
+// Bader-Deuflhard step (its dtrcmd goes to dtrcmd)
 struct ZoneState {
-    int phase, i, iter, ierr, lib_err, solved, pad0, pad1;
+    int phase, i, iter, ierr, lib_err, solved, pad0, step_ierr;
     double dtdone, dttry, dtrcmd, my_t9_last, my_rho_last, grd, ye_entry, pad2;
     ZoneStats st;
 };
@@ -58,7 +60,9 @@ struct ZoneData {
     int *ierr;
     ZoneStats *stats;   // may be null
     ZoneState *state;
-    int *flag;          // [0] set when some zone needs a Newton iteration
+    BDState *bd;        // [nz] Bader-Deuflhard state carried between steps
+    BDState bd0;        // the state every zone starts from (the context's)
+    int *flag;          // [0] set when some zone needs a Newton iteration or a BD step
     int nz;
     int init_prec;      // factor precision a zone starts with on the batched path (0 FP32, 1 FP64)
 };
@@ -66,6 +70,7 @@ struct ZoneData {
 struct KArgs {
     DevNet N;
     DevKnobs K;
+    BDTables T;
     ZoneData Z;
     char *slot_base;
     size_t slot_bytes;
@@ -163,6 +168,7 @@ __device__ inline void zone_init(const KArgs &a, int z) {
         s.my_rho_last = 0.5 * (rho_0 + rho_1);
         s.ye_entry = ye;
         Z.state[z] = s;
+        Z.bd[z] = Z.bd0;
     }
     wave_sync();
 }
@@ -173,7 +179,12 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
     const ZoneData &Z = a.Z;
     double *yps = zrow(Z.yps, Z.nsp_stride, z);
     if (loop_done) {
-        for (int j = lane_id(); j < N.nsp; j += WAVE) yps[j] = fmax(yps[j], 1.0e-99);
+        // DEUFL renormalises by the plain sum (solver.F90:229-235), then the final clip
+        const double sm = a.K.integration_method == 1 ? seq_sum(yps, N.nsp) : 1.0;
+        for (int j = lane_id(); j < N.nsp; j += WAVE) {
+            const double y = a.K.integration_method == 1 ? yps[j] / sm : yps[j];
+            yps[j] = fmax(y, 1.0e-99);
+        }
         wave_sync();
     }
     double ye = s.ye_entry;
@@ -213,7 +224,16 @@ __device__ inline void zone_control(const KArgs &a, int z) {
     int ierr = 0;
     bool step_ended = false;   // a backward-Euler step just returned (ierr set)
 
-    if (s.phase == PH_AFTER_NEWTON) {
+    if (s.phase == PH_AFTER_NEWTON && K.integration_method == 1) {
+        // a Bader-Deuflhard step returned (do_one_time_step, solver.F90:164-179)
+        if (s.lib_err || s.step_ierr == E_BAD_STEP) {
+            zone_finish(a, z, s, E_BAD_STEP, false);
+            if (lane_id() == 0) Z.state[z] = s;
+            return;
+        }
+        ierr = s.step_ierr;
+        step_ended = true;
+    } else if (s.phase == PH_AFTER_NEWTON) {
         if (s.lib_err) {   // the reference stops here (zero in the matrix, singular factor)
             zone_finish(a, z, s, 3, false);
             if (lane_id() == 0) Z.state[z] = s;
@@ -271,6 +291,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
         }
         // next sub-step (solver.F90:108-164)
         s.i += 1;
+        if (s.i > 1 && lane_id() == 0) Z.bd[z].frst = 0;   // solver.F90:109
         if (s.i > K.max_sub_steps) {   // loop ran out through the halving branch
             s.st.nsubt = s.i;
             zone_finish(a, z, s, ierr, true);
@@ -305,6 +326,12 @@ __device__ inline void zone_control(const KArgs &a, int z) {
             s.my_t9_last = my_t9;
         }
         ierr = 0;
+        if (my_t9 >= K.min_t9_for_slv && K.integration_method == 1) {
+            s.solved = 1;
+            s.phase = PH_NEWTON;   // the worker runs bader_deuflhard_integration
+            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+            return;
+        }
         if (my_t9 >= K.min_t9_for_slv) {
             // backward_euler_newton set-up (backward_euler.F90:49-53)
             copy_vec(y_old, yps, nsp);
