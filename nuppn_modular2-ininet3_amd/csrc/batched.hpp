@@ -102,7 +102,7 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
 // Factors of the assembled matrix (A in CSR order) in precision T into the factor area fac.
 // Returns 0, or 1 when a pivot or the border Schur complement is not finite or zero.
 template <class T, int NB>
-__device__ int block_factor_zone(const BKArgs &b, const double *Aval, char *fac,
+__device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *Aval, char *fac,
                                  NUPPN_LDS char *lds, ZoneStats &st) {
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, b.B.nip);
@@ -117,7 +117,7 @@ __device__ int block_factor_zone(const BKArgs &b, const double *Aval, char *fac,
 // cannot deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64
 // factors: the reference's singular-matrix outcome).
 template <class T, int NB>
-__device__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
+__device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
                             const double *rhs, const double *y, double *x, double *res,
                             double *d, double *part, char *fac, NUPPN_LDS char *lds,
                             ZoneStats &st) {
@@ -126,9 +126,12 @@ __device__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval
     const int n = B.n;
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
+    NUPPN_GLB double *xg = gptr(x), *resg = gptr(res);
+    const NUPPN_GLB double *dg = gptr(d), *yg = gptr(y), *rhsg = gptr(rhs);
+    const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     for (int i = lane_id(); i < n; i += WAVE) {
-        x[i] = 0.0;
-        res[i] = rhs[i];
+        xg[i] = 0.0;
+        resg[i] = rhsg[i];
     }
     wave_sync();
     double eta_prev = 0.0;
@@ -138,10 +141,11 @@ __device__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval
         double eta = 0.0;
         bool fin = true;
         for (int i = lane_id(); i < n; i += WAVE) {
-            const double xi = x[i] + d[i];
-            x[i] = xi;
-            const double scale = fabs(xi) + fabs(y[N.red2sp[i]]) + kAbundanceFloor;
-            eta = fmax(eta, fabs(d[i]) / scale);
+            const double di = dg[i];
+            const double xi = xg[i] + di;
+            xg[i] = xi;
+            const double scale = fabs(xi) + fabs(yg[red2sp[i]]) + kAbundanceFloor;
+            eta = fmax(eta, fabs(di) / scale);
             fin = fin && isfinite(xi);
         }
         eta = wave_max(eta);
@@ -167,7 +171,7 @@ __device__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval
 // Refined solve with fresh factors (the backward-Euler Newton iteration: one solve per
 // factorisation).  Same return codes as block_refine.
 template <class T, int NB>
-__device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
+__device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
                                    const double *rhs, const double *y, double *x, double *res,
                                    double *d, double *part, char *fac, NUPPN_LDS char *lds,
                                    ZoneStats &st) {
@@ -177,7 +181,7 @@ __device__ int block_refined_solve(const DevNet &N, const BKArgs &b, const doubl
 
 // the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order) and the
 // Newton rhs y_old - y_cur + dt dX/dt as chunked sums over precomputed coefficient lists
-__device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double dt,
+__device__ __forceinline__ void block_assemble(const DevNet &N, const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
                                       double *rhs, double *part) {
@@ -186,19 +190,32 @@ __device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double
     reaction_terms(N, xn, v, vx, stem, stem + N.nre);
     wave_sync();
     const DevSeg &J = B.jac;
-    seg_chunks(J, part, [&](int e) { return J.e_coef[e] * stem[J.e_idx[e]]; });
+    {
+        const NUPPN_GLB double *coef = gptr(J.e_coef), *st = gptr(stem);
+        const NUPPN_GLB int *idx = gptr(J.e_idx);
+        seg_chunks(J, part, [&](int e) { return coef[e] * st[idx[e]]; });
+    }
+    const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
+    NUPPN_GLB double *Ag = gptr(Aval);
     for (int k = lane_id(); k < B.nnzA; k += WAVE) {
         const double jv = 1.0e-99 + seg_total(J, part, k);
         double val = jv * (-dt);
-        if (B.csr_isdiag[k]) val = val + 1.0;
-        Aval[k] = val;
+        if (isdiag[k]) val = val + 1.0;
+        Ag[k] = val;
     }
     wave_sync();
     const DevSeg &D = B.dxdt;
-    seg_chunks(D, part, [&](int e) { return D.e_coef[e] * vx[D.e_idx[e]]; });
+    {
+        const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vx);
+        const NUPPN_GLB int *idx = gptr(D.e_idx);
+        seg_chunks(D, part, [&](int e) { return coef[e] * vxg[idx[e]]; });
+    }
+    const NUPPN_GLB int *red2sp = gptr(N.red2sp);
+    const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur);
+    NUPPN_GLB double *rg = gptr(rhs);
     for (int r = lane_id(); r < N.n; r += WAVE) {
-        const int sp = N.red2sp[r];
-        rhs[r] = y_old[sp] - y_cur[sp] + dt * seg_total(D, part, r);
+        const int sp = red2sp[r];
+        rg[r] = yo[sp] - yc[sp] + dt * seg_total(D, part, r);
     }
     wave_sync();
 }
@@ -208,7 +225,7 @@ __device__ inline void block_assemble(const DevNet &N, const DevBlock &B, double
 // FP64 factors when that fails (the zone then stays on FP64 for the rest of the call) or when
 // the zone is on FP64 already.  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON.
 template <int NB>
-__device__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
+__device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
                                   NUPPN_LDS char *lds, int z) {
     const DevNet &N = a.N;
     const DevKnobs &K = a.K;
@@ -366,17 +383,28 @@ __device__ void zone_bd_block(const KArgs &a, const BKArgs &b, char *slot, NUPPN
     wave_sync();
 }
 
-// every zone in PH_NEWTON; each wavefront owns one scratch slot (arrays and factors) and
-// strides over the zones
-template <int NB>
-__global__ void __launch_bounds__(64) zone_newton_block_kernel(KArgs a, BKArgs b) {
+// every zone in PH_NEWTON (the control pass's work list); each wavefront owns one scratch slot
+// (arrays and factors) and strides over the list.  W = resident wavefronts per SIMD the registers are held to: W = 2
+// doubles the zones in flight when many need an iteration; W = 1 leaves the compiler the full
+// register file and the shortest iteration, for passes with few zones.
+template <int NB, int W>
+__global__ void __launch_bounds__(64, W) zone_newton_block_kernel(KArgs a, BKArgs b) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) {
-        if (a.K.integration_method == 1) zone_bd_block<NB>(a, b, slot, lds, z);
-        else zone_newton_block<NB>(a, b, slot, lds, z);
-    }
+    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+        zone_newton_block<NB>(a, b, slot, lds, a.Z.active[i]);
+}
+
+// every zone in PH_NEWTON under integration_method 1: one Bader-Deuflhard step each (a kernel
+// of its own: the step's code would otherwise raise the Newton kernel's register use)
+template <int NB>
+__global__ void __launch_bounds__(64) zone_bd_block_kernel(KArgs a, BKArgs b) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
+    char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
+    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+        zone_bd_block<NB>(a, b, slot, lds, a.Z.active[i]);
 }
 
 }  // namespace nuppn

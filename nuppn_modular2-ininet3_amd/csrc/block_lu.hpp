@@ -66,16 +66,26 @@ struct BlockCfg {
 
 // LDS of one wavefront: three NB x LD operand buffers, pivot row / column, the solve vector
 template <class T>
+// LDS of one wavefront: the two block-product operands X1, X2 of the factorisation loop, which
+// the solve vector vec overlays (the loop and the solves never run at the same time), the
+// Gauss-Jordan pivot column and row, and the border scratch.  About 19 KB with FP64 factors
+// (NB = 32, 1,100 species), so eight wavefronts share a CU's 160 KB.
 struct BlockLds {
-    NUPPN_LDS T *X1, *X2, *WL, *colbuf, *rowbuf, *vec;
+    NUPPN_LDS T *X1, *X2, *colbuf, *rowbuf, *vec;
     NUPPN_LDS double *bscr;   // [16] border solve scratch
 };
 
 template <class T, int NB>
+__host__ __device__ constexpr size_t block_lds_span(int nip) {
+    return 2 * (size_t) NB * BlockCfg<T, NB>::LD > (size_t) nip + 8
+               ? 2 * (size_t) NB * BlockCfg<T, NB>::LD
+               : (size_t) nip + 8;
+}
+
+template <class T, int NB>
 __host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd) {
     (void) nbd;   // nbd <= 8: the border entries fit the 8 slots after nip
-    return sizeof(T) * (3 * (size_t) NB * BlockCfg<T, NB>::LD + 2 * NB + (size_t) nip + 8) + 16 +
-           16 * sizeof(double);
+    return sizeof(T) * (block_lds_span<T, NB>(nip) + 2 * NB) + 16 + 16 * sizeof(double);
 }
 
 template <class T, int NB>
@@ -84,17 +94,14 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
     BlockLds<T> L;
     NUPPN_LDS T *p = (NUPPN_LDS T *) base;
     L.X1 = p;
-    p += NB * C::LD;
-    L.X2 = p;
-    p += NB * C::LD;
-    L.WL = p;
-    p += NB * C::LD;
+    L.X2 = p + NB * C::LD;
+    L.vec = p;
+    p += block_lds_span<T, NB>(nip);
     L.colbuf = p;
     p += NB;
     L.rowbuf = p;
     p += NB;
-    L.vec = p;
-    NUPPN_LDS char *q = (NUPPN_LDS char *) (p + nip + 8);
+    NUPPN_LDS char *q = (NUPPN_LDS char *) p;
     q = (NUPPN_LDS char *) (((uintptr_t) q + 15) & ~(uintptr_t) 15);
     L.bscr = (NUPPN_LDS double *) q;
     return L;
@@ -150,16 +157,18 @@ __device__ __forceinline__ void lds_store_rm(NUPPN_LDS T *X, const Regs<T, NB> &
 }
 
 template <class T, int NB>
-__device__ __forceinline__ void glb_load_cm(const T *G, Regs<T, NB> &a) {
+__device__ __forceinline__ void glb_load_cm(const T *Gp, Regs<T, NB> &a) {
     using C = BlockCfg<T, NB>;
+    const NUPPN_GLB T *G = gptr(Gp);
     const int c = lane_id() % NB, h = lane_id() / NB;
 #pragma unroll
     for (int t = 0; t < C::R; ++t) a.v[t] = G[c * NB + h * C::R + t];
 }
 
 template <class T, int NB>
-__device__ __forceinline__ void glb_store_cm(T *G, const Regs<T, NB> &a) {
+__device__ __forceinline__ void glb_store_cm(T *Gp, const Regs<T, NB> &a) {
     using C = BlockCfg<T, NB>;
+    NUPPN_GLB T *G = gptr(Gp);
     const int c = lane_id() % NB, h = lane_id() / NB;
 #pragma unroll
     for (int t = 0; t < C::R; ++t) G[c * NB + h * C::R + t] = a.v[t];
@@ -167,7 +176,7 @@ __device__ __forceinline__ void glb_store_cm(T *G, const Regs<T, NB> &a) {
 
 // In-place inverse of the NB x NB block held in lane layout, Gauss-Jordan without pivoting.
 template <class T, int NB>
-__device__ inline void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
+__device__ __forceinline__ void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
                                   NUPPN_LDS T *rowbuf) {
     using C = BlockCfg<T, NB>;
     const int c = lane_id() % NB, h = lane_id() / NB;
@@ -209,7 +218,7 @@ __device__ inline void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
 
 // acc = A * B with A column-major and B row-major in LDS (NB x NB), result in lane layout
 template <class T, int NB>
-__device__ inline void gemm_lds(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
+__device__ __forceinline__ void gemm_lds(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
                                 Regs<T, NB> &acc) {
     using C = BlockCfg<T, NB>;
     const int c = lane_id() % NB, h = lane_id() / NB;
@@ -226,7 +235,7 @@ __device__ inline void gemm_lds(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
 
 // acc -= A * B (same layouts as gemm_lds)
 template <class T, int NB>
-__device__ inline void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
+__device__ __forceinline__ void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
                                     Regs<T, NB> &acc) {
     using C = BlockCfg<T, NB>;
     const int c = lane_id() % NB, h = lane_id() / NB;
@@ -242,13 +251,14 @@ __device__ inline void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN_LDS T *B
 // scaled value of CSC entry e of a block list: molar units, column-scaled
 __device__ __forceinline__ double block_entry(const DevBlock &B, const double *Aval,
                                               const double *cs, int e) {
-    return Aval[B.ent_a[e]] * B.ent_ys[e] * cs[B.ent_col[e]];
+    return gptr(Aval)[gptr(B.ent_a)[e]] * gptr(B.ent_ys)[e] * gptr(cs)[gptr(B.ent_col)[e]];
 }
 
 // lane (rr = lane % NB, h = lane / NB) part of an NB x NB column-major block for matvec_regs
 template <class T, int NB>
-__device__ __forceinline__ void load_mv(const T *__restrict__ M, T (&m)[NB / BlockCfg<T, NB>::H]) {
+__device__ __forceinline__ void load_mv(const T *__restrict__ Mp, T (&m)[NB / BlockCfg<T, NB>::H]) {
     constexpr int CPL = NB / BlockCfg<T, NB>::H;
+    const NUPPN_GLB T *M = gptr(Mp);
     const int rr = lane_id() % NB, h = lane_id() / NB;
 #pragma unroll
     for (int j = 0; j < CPL; ++j) m[j] = M[(h * CPL + j) * NB + rr];
@@ -271,7 +281,7 @@ __device__ __forceinline__ T matvec_regs(const T (&m)[NB / BlockCfg<T, NB>::H],
 // matrices do not depend on v, so each sweep loads the next block's matrices before it works on
 // the current block: the global-memory latency overlaps the dependent chain of mat-vecs.
 template <class T, int NB>
-__device__ inline void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
+__device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
                                     NUPPN_LDS T *v) {
     constexpr int CPL = NB / BlockCfg<T, NB>::H;
     const int rr = lane_id() % NB, h = lane_id() / NB;
@@ -330,7 +340,7 @@ __device__ inline double wave_sum(double v) {
 // Factor the reduced matrix (FP64 CSC values Aval) into F.  Returns 0, or 1 when a pivot or
 // the border Schur complement is not finite / zero (the caller falls back or reports).
 template <class T, int NB>
-__device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFactors<T> &F,
+__device__ __forceinline__ int block_factor(const DevBlock &B, const double *Aval, const BlockFactors<T> &F,
                             const BlockLds<T> &L) {
     using C = BlockCfg<T, NB>;
     constexpr size_t BS = (size_t) NB * NB;
@@ -338,42 +348,44 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
     const int nblk = B.nblk;
     bool bad = false;
     for (int i = lane; i < B.n; i += WAVE) {
-        const double d = Aval[B.diag_a[i]];
+        const double d = gptr(Aval)[gptr(B.diag_a)[i]];
         if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
-        F.cs[i] = 1.0 / d;
+        gptr(F.cs)[i] = 1.0 / d;
     }
     if (ballot(bad)) return 1;
     wave_sync();
     // scalings by padded position (interior blocks, then border): padding rows/columns stay 1
     for (int p = B.ni + lane; p < B.nip; p += WAVE) {
-        F.colsc[p] = 1.0;
-        F.rowinv[p] = 1.0;
+        gptr(F.colsc)[p] = 1.0;
+        gptr(F.rowinv)[p] = 1.0;
     }
     for (int i = lane; i < B.n; i += WAVE) {
-        const int p = B.vpos[i];
-        F.colsc[p] = B.mass[i] * F.cs[i];
-        F.rowinv[p] = 1.0 / B.mass[i];
+        const int p = gptr(B.vpos)[i];
+        const double ms = gptr(B.mass)[i];
+        gptr(F.colsc)[p] = ms * gptr(F.cs)[i];
+        gptr(F.rowinv)[p] = 1.0 / ms;
     }
     wave_sync();
     // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
     // where W_i goes, B_i in place (the loop below then only streams them, prefetched)
     {
         const size_t area = (size_t) nblk * BS;
+        NUPPN_GLB T *S0 = gptr(F.Sinv), *W0 = gptr(F.W), *B0 = gptr(F.Bm);
         for (size_t e = lane; e < area; e += WAVE) {
-            F.Sinv[e] = T(0);
-            F.W[e] = T(0);
-            F.Bm[e] = T(0);
+            S0[e] = T(0);
+            W0[e] = T(0);
+            B0[e] = T(0);
         }
         wave_sync();
-        const int nent = B.ent_ptr[3 * nblk - 2];
+        const int nent = gptr(B.ent_ptr)[3 * nblk - 2];
         for (int e = lane; e < nent; e += WAVE) {
-            const int dst = B.ent_dst[e];
+            const int dst = gptr(B.ent_dst)[e];
             const int ar = (int) (dst / area);
-            T *base = ar == 0 ? F.Sinv : (ar == 1 ? F.W : F.Bm);
+            NUPPN_GLB T *base = ar == 0 ? S0 : (ar == 1 ? W0 : B0);
             base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
         }
         const int pr = lane, last = nblk - 1;   // identity on the padded diagonal positions
-        if (pr < NB && last * NB + pr >= B.ni) F.Sinv[(size_t) last * BS + pr * NB + pr] = T(1);
+        if (pr < NB && last * NB + pr >= B.ni) S0[(size_t) last * BS + pr * NB + pr] = T(1);
         wave_sync();
     }
     Regs<T, NB> a, cb, dn, bb;
@@ -399,33 +411,36 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
         gemm_lds<T, NB>(L.X1, L.X2, a);                                  // W_i = S_i^-1 C_i
         glb_store_cm<T, NB>(F.W + i * BS, a);
         lds_fence();
-        lds_store_rm<T, NB>(L.WL, a);
+        lds_store_rm<T, NB>(L.X2, a);
         lds_store_cm<T, NB>(L.X1, bb);
         lds_fence();
 #pragma unroll
         for (int t = 0; t < C::R; ++t) a.v[t] = dn.v[t];
-        gemm_lds_sub<T, NB>(L.X1, L.WL, a);                              // D - B_i W_i
+        gemm_lds_sub<T, NB>(L.X1, L.X2, a);                              // D - B_i W_i
     }
     wave_sync();
     // border: E, X_F = T^-1 F, G' = G - E X_F and its LU with partial pivoting
     const int nbd = B.nbd, nip = B.nip;
     if (nbd > 0) {
         const int LE = 3 * nblk - 2, LF = LE + 1, LG = LE + 2;
+        NUPPN_GLB T *Eg = gptr(F.E), *XFg = gptr(F.XF);
+        NUPPN_GLB double *Gg = gptr(F.G);
+        const NUPPN_GLB int *eptr = gptr(B.ent_ptr), *eloc = gptr(B.ent_loc);
         for (int i = lane; i < nbd * nip; i += WAVE) {
-            F.E[i] = T(0);
-            F.XF[i] = T(0);
+            Eg[i] = T(0);
+            XFg[i] = T(0);
         }
-        for (int i = lane; i < nbd * nbd; i += WAVE) F.G[i] = 0.0;
+        for (int i = lane; i < nbd * nbd; i += WAVE) Gg[i] = 0.0;
         wave_sync();
-        for (int e = B.ent_ptr[LE] + lane; e < B.ent_ptr[LE + 1]; e += WAVE)
-            F.E[B.ent_loc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = B.ent_ptr[LF] + lane; e < B.ent_ptr[LF + 1]; e += WAVE)
-            F.XF[B.ent_loc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = B.ent_ptr[LG] + lane; e < B.ent_ptr[LG + 1]; e += WAVE)
-            F.G[B.ent_loc[e]] = block_entry(B, Aval, F.cs, e);
+        for (int e = eptr[LE] + lane; e < eptr[LE + 1]; e += WAVE)
+            Eg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
+        for (int e = eptr[LF] + lane; e < eptr[LF + 1]; e += WAVE)
+            XFg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
+        for (int e = eptr[LG] + lane; e < eptr[LG + 1]; e += WAVE)
+            Gg[eloc[e]] = block_entry(B, Aval, F.cs, e);
         wave_sync();
         for (int b = 0; b < nbd; ++b) {
-            T *xf = F.XF + (size_t) b * nip;
+            NUPPN_GLB T *xf = XFg + (size_t) b * nip;
             for (int j = lane; j < nip; j += WAVE) L.vec[j] = xf[j];
             lds_fence();
             thomas_solve<T, NB>(B, F, L.vec);
@@ -437,7 +452,7 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
             for (int b1 = 0; b1 < nbd; ++b1) {
                 double s = 0.0;
                 for (int j = lane; j < nip; j += WAVE)
-                    s += (double) F.E[(size_t) b1 * nip + j] * (double) F.XF[(size_t) b2 * nip + j];
+                    s += (double) Eg[(size_t) b1 * nip + j] * (double) XFg[(size_t) b2 * nip + j];
                 s = wave_sum(s);
                 if (lane == 0) F.G[b1 + b2 * nbd] -= s;
             }
@@ -475,27 +490,31 @@ __device__ int block_factor(const DevBlock &B, const double *Aval, const BlockFa
 
 // x = A^-1 b with the factors F (b, x: FP64 [n], reduced order, mass-fraction units)
 template <class T, int NB>
-__device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const BlockLds<T> &L,
-                            const double *b, double *x) {
+__device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const BlockLds<T> &L,
+                            const double *bp, double *xp) {
+    const NUPPN_GLB double *b = gptr(bp), *rowinv = gptr(F.rowinv), *colsc = gptr(F.colsc);
+    const NUPPN_GLB int *vpos = gptr(B.vpos);
+    NUPPN_GLB double *x = gptr(xp);
+    const NUPPN_GLB T *Eg = gptr(F.E), *XFg = gptr(F.XF);
     const int lane = lane_id(), nip = B.nip, nbd = B.nbd;
     NUPPN_LDS T *v = L.vec;
     // an exact power of two brings the largest entry to [1, 2): FP32 then resolves entries down
     // to 2^-126 of the largest one rather than 2^-126 absolute (refinement residuals shrink far
     // below the FP32 range while the abundances they correct stay meaningful)
     double m = 0.0;
-    for (int c = lane; c < B.n; c += WAVE) m = fmax(m, fabs(b[c] * F.rowinv[B.vpos[c]]));
+    for (int c = lane; c < B.n; c += WAVE) m = fmax(m, fabs(b[c] * rowinv[vpos[c]]));
     m = wave_max(m);
     const int ex = (m > 0.0 && isfinite(m)) ? ilogb(m) : 0;
     const double sc = ldexp(1.0, -ex), unsc = ldexp(1.0, ex);
     for (int j = B.ni + lane; j < nip; j += WAVE) v[j] = T(0);
-    for (int c = lane; c < B.n; c += WAVE) v[B.vpos[c]] = (T) (b[c] * F.rowinv[B.vpos[c]] * sc);
+    for (int c = lane; c < B.n; c += WAVE) v[vpos[c]] = (T) (b[c] * rowinv[vpos[c]] * sc);
     lds_fence();
     thomas_solve<T, NB>(B, F, v);
     if (nbd > 0) {
         NUPPN_LDS double *tb = L.bscr;
         for (int bb = 0; bb < nbd; ++bb) {
             double s = 0.0;
-            for (int j = lane; j < nip; j += WAVE) s += (double) F.E[(size_t) bb * nip + j] * (double) v[j];
+            for (int j = lane; j < nip; j += WAVE) s += (double) Eg[(size_t) bb * nip + j] * (double) v[j];
             s = wave_sum(s);
             if (lane == 0) tb[bb] = (double) v[nip + bb] - s;
         }
@@ -517,20 +536,22 @@ __device__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const B
         lds_fence();
         for (int j = lane; j < nip; j += WAVE) {
             T s = v[j];
-            for (int bb = 0; bb < nbd; ++bb) s = fma_t(-F.XF[(size_t) bb * nip + j], v[nip + bb], s);
+            for (int bb = 0; bb < nbd; ++bb) s = fma_t(-XFg[(size_t) bb * nip + j], v[nip + bb], s);
             v[j] = s;
         }
         lds_fence();
     }
-    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[B.vpos[c]] * F.colsc[B.vpos[c]] * unsc;
+    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[vpos[c]] * colsc[vpos[c]] * unsc;
     wave_sync();
 }
 
 // chunk pass of a segmented sum: part[c] = sum over the chunk's elements of val(e)
 template <class F>
-__device__ inline void seg_chunks(const DevSeg &S, double *part, F val) {
+__device__ __forceinline__ void seg_chunks(const DevSeg &S, double *partp, F val) {
+    const NUPPN_GLB int *ch_beg = gptr(S.ch_beg);
+    NUPPN_GLB double *part = gptr(partp);
     for (int c = lane_id(); c < S.nch; c += WAVE) {
-        const int e0 = S.ch_beg[c], e1 = S.ch_beg[c + 1];
+        const int e0 = ch_beg[c], e1 = ch_beg[c + 1];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         int e = e0;
         for (; e + 3 < e1; e += 4) {
@@ -546,9 +567,11 @@ __device__ inline void seg_chunks(const DevSeg &S, double *part, F val) {
 }
 
 // segment total of the chunk partials
-__device__ __forceinline__ double seg_total(const DevSeg &S, const double *part, int s) {
+__device__ __forceinline__ double seg_total(const DevSeg &S, const double *partp, int s) {
+    const NUPPN_GLB int *seg_ch = gptr(S.seg_ch);
+    const NUPPN_GLB double *part = gptr(partp);
     double t = 0.0;
-    for (int c = S.seg_ch[s]; c < S.seg_ch[s + 1]; ++c) t += part[c];
+    for (int c = seg_ch[s]; c < seg_ch[s + 1]; ++c) t += part[c];
     return t;
 }
 
@@ -556,25 +579,29 @@ __device__ __forceinline__ double seg_total(const DevSeg &S, const double *part,
 // max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager): x solves a system whose
 // every entry is within that relative distance of A and rhs, ignoring rows whose whole
 // magnitude is below the abundance floor.  part holds 2 x nch partials.
-__device__ inline double block_residual(const DevBlock &B, const double *Acsr, const double *rhs,
-                                        const double *x, double *r, double *part) {
+__device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp, const double *rhsp,
+                                        const double *xp, double *rp, double *partp) {
     const DevSeg &S = B.resid;
-    double *pabs = part + S.nch;
+    const NUPPN_GLB double *Acsr = gptr(Acsrp), *rhs = gptr(rhsp), *x = gptr(xp);
+    const NUPPN_GLB int *ch_beg = gptr(S.ch_beg), *e_idx = gptr(S.e_idx), *seg_ch = gptr(S.seg_ch);
+    const NUPPN_GLB int *csr_col = gptr(B.csr_col);
+    NUPPN_GLB double *r = gptr(rp), *part = gptr(partp);
+    NUPPN_GLB double *pabs = part + S.nch;
     for (int c = lane_id(); c < S.nch; c += WAVE) {
-        const int e0 = S.ch_beg[c], e1 = S.ch_beg[c + 1];
+        const int e0 = ch_beg[c], e1 = ch_beg[c + 1];
         double s0 = 0.0, s1 = 0.0, a0 = 0.0, a1 = 0.0;
         int e = e0;
         for (; e + 1 < e1; e += 2) {
-            const int k0 = S.e_idx[e], k1 = S.e_idx[e + 1];
-            const double t0 = Acsr[k0] * x[B.csr_col[k0]], t1 = Acsr[k1] * x[B.csr_col[k1]];
+            const int k0 = e_idx[e], k1 = e_idx[e + 1];
+            const double t0 = Acsr[k0] * x[csr_col[k0]], t1 = Acsr[k1] * x[csr_col[k1]];
             s0 += t0;
             s1 += t1;
             a0 += fabs(t0);
             a1 += fabs(t1);
         }
         if (e < e1) {
-            const int k0 = S.e_idx[e];
-            const double t0 = Acsr[k0] * x[B.csr_col[k0]];
+            const int k0 = e_idx[e];
+            const double t0 = Acsr[k0] * x[csr_col[k0]];
             s0 += t0;
             a0 += fabs(t0);
         }
@@ -585,7 +612,7 @@ __device__ inline double block_residual(const DevBlock &B, const double *Acsr, c
     double om = 0.0;
     for (int i = lane_id(); i < B.n; i += WAVE) {
         double t = 0.0, a = 0.0;
-        for (int c = S.seg_ch[i]; c < S.seg_ch[i + 1]; ++c) {
+        for (int c = seg_ch[i]; c < seg_ch[i + 1]; ++c) {
             t += part[c];
             a += pabs[c];
         }

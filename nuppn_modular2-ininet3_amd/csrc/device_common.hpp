@@ -137,20 +137,40 @@ struct DevNet {
     const int *iperm_c;     // [n]
 };
 
+// Pointers reach the device code as generic pointers (struct fields, arguments of functions
+// that are not inlined), so the compiler emits flat memory instructions for them.  A flat load
+// counts against the LDS counter as well: every LDS wait then also waits for the outstanding
+// global loads, and prefetches issued ahead of LDS-bound work stop overlapping it.  The hot
+// helpers therefore access memory through global-address-space pointers; every pointer given to
+// a kernel here is device (global) memory.
+#define NUPPN_GLB __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ NUPPN_GLB T *gptr(T *p) {
+    return (NUPPN_GLB T *) p;
+}
+template <class T>
+__device__ __forceinline__ const NUPPN_GLB T *gptr(const T *p) {
+    return (const NUPPN_GLB T *) p;
+}
 // xn = X/A for considered isotopes (A > 0), xn(gamma) = 1 (jac_rhs.F90:235-240, 312-317)
-__device__ inline void molar(const DevNet &N, const double *x, double *xn) {
+__device__ inline void molar(const DevNet &N, const double *xp, double *xnp) {
+    const NUPPN_GLB double *x = gptr(xp), *aw = gptr(N.aw);
+    NUPPN_GLB double *xn = gptr(xnp);
     for (int i = lane_id(); i < N.nsp; i += WAVE)
-        xn[i] = (i != N.igamma && N.aw[i] > 0.0) ? x[i] / N.aw[i] : 0.0;
+        xn[i] = (i != N.igamma && aw[i] > 0.0) ? x[i] / aw[i] : 0.0;
     if (lane_id() == 0) xn[N.igamma] = 1.0;
 }
 
 // per-reaction quantities: vx (calculate_dxdt flux, jac_rhs.F90:245) and the two Jacobian stems
 // (jac_rhs.F90:355, 359)
-__device__ inline void reaction_terms(const DevNet &N, const double *xn, const double *v,
-                                      double *vx, double *st1, double *st2) {
+__device__ inline void reaction_terms(const DevNet &N, const double *xnp, const double *vp,
+                                      double *vxp, double *st1p, double *st2p) {
     const int nre = N.nre;
+    const NUPPN_GLB int *k = gptr(N.k);
+    const NUPPN_GLB double *xn = gptr(xnp), *v = gptr(vp);
+    NUPPN_GLB double *vx = gptr(vxp), *st1 = gptr(st1p), *st2 = gptr(st2p);
     for (int l = lane_id(); l < nre; l += WAVE) {
-        const int in1 = N.k[l], n1 = N.k[nre + l], in2 = N.k[2 * nre + l], n2 = N.k[3 * nre + l];
+        const int in1 = k[l], n1 = k[nre + l], in2 = k[2 * nre + l], n2 = k[3 * nre + l];
         const double y1 = xn[in1], y2 = xn[in2], vl = v[l];
         vx[l] = kFact[n1] * vl * powi(y1, n1) * y2;
         st1[l] = (double) n1 * kFact[n1] * kFact[n2] * powi(y2, n2) * powi(y1, n1 - 1) * vl;

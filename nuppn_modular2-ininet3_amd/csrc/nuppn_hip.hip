@@ -73,17 +73,29 @@ __global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const LUWork w = make_work(smem, a.N.n);
     const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
-    for (int z = blockIdx.x; z < a.Z.nz; z += gridDim.x) {
-        if (a.K.integration_method == 1) zone_bd(a, S, w, z);
-        else zone_newton(a, S, w, z);
-    }
+    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x) zone_newton(a, S, w, a.Z.active[i]);
 }
 
-__global__ void flag_reset_kernel(int *flag) { flag[0] = 0; }
+// one Bader-Deuflhard step for every zone in PH_NEWTON (integration_method 1)
+__global__ void __launch_bounds__(64) zone_bd_kernel(KArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const LUWork w = make_work(smem, a.N.n);
+    const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
+    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x) zone_bd(a, S, w, a.Z.active[i]);
+}
+
+__global__ void flag_reset_kernel(int *flag) {
+    flag[0] = 0;
+    flag[1] = 0;
+}
 
 namespace nuppn {
-template __global__ void zone_newton_block_kernel<32>(KArgs a, BKArgs b);
-template __global__ void zone_newton_block_kernel<64>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<32, 1>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<32, 2>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<64, 1>(KArgs a, BKArgs b);
+template __global__ void zone_newton_block_kernel<64, 2>(KArgs a, BKArgs b);
+template __global__ void zone_bd_block_kernel<32>(KArgs a, BKArgs b);
+template __global__ void zone_bd_block_kernel<64>(KArgs a, BKArgs b);
 }  // namespace nuppn
 
 // single-state assembly entries (test hooks): mode 0 dX/dt, mode 1 reduced matrix values
@@ -179,6 +191,7 @@ struct nuppn_ctx {
     double *d_work = nullptr;
     ZoneState *d_state = nullptr;
     BDState *d_bd = nullptr;      // [nz] per-zone Bader-Deuflhard state of the running call
+    int *d_active = nullptr;      // [nz] the zones a worker pass takes
     BDTables bdt{};               // bader_deuflhard_init's tables
     BDState bd{};                 // the reference's SAVE'd BD state, carried between calls
     size_t work_stride = 0;
@@ -201,6 +214,7 @@ struct nuppn_ctx {
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
     int bmax_slots = 0, bslots = 0;
+    int bgrid[3] = {0, 0, 0};   // resident wavefronts of the worker held to 1 / 2 per SIMD
     char *d_bslots = nullptr;
     hipEvent_t ev[4] = {};
     hipEvent_t evn[2] = {}, evc[2] = {};
@@ -283,9 +297,17 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     return B;
 }
 
-static const void *block_kernel_ptr(int nb) {
-    return nb == 64 ? (const void *) zone_newton_block_kernel<64>
-                    : (const void *) zone_newton_block_kernel<32>;
+// the batched path's worker: BD step (method 1) or Newton iteration, the latter held to one or
+// two resident wavefronts per SIMD (waves)
+static const void *block_kernel_ptr(int nb, int method, int waves) {
+    if (method == 1)
+        return nb == 64 ? (const void *) zone_bd_block_kernel<64>
+                        : (const void *) zone_bd_block_kernel<32>;
+    if (nb == 64)
+        return waves == 2 ? (const void *) zone_newton_block_kernel<64, 2>
+                          : (const void *) zone_newton_block_kernel<64, 1>;
+    return waves == 2 ? (const void *) zone_newton_block_kernel<32, 2>
+                      : (const void *) zone_newton_block_kernel<32, 1>;
 }
 
 // resident wavefronts per CU of a kernel with `lds` bytes of dynamic LDS
@@ -334,8 +356,11 @@ static int setup_batched(nuppn_ctx *c) {
                              : block_lds_bytes<double, 32>(b.nip, b.nbd));
     if (c->blds64 > 160 * 1024) return 0;
     // one kernel runs both factor precisions: it takes the FP64 LDS size
-    const int occ = kernel_occupancy(block_kernel_ptr(nb), c->blds64);
-    if (occ <= 0) return 0;
+    const int method = c->kn.integration_method;
+    const int occ1 = kernel_occupancy(block_kernel_ptr(nb, method, 1), c->blds64);
+    const int occ2 = kernel_occupancy(block_kernel_ptr(nb, method, 2), c->blds64);
+    if (occ1 <= 0 || occ2 <= 0) return 0;
+    const int occ = std::max(occ1, occ2);
     int dev = 0, ncu = 0;
     (void) hipGetDevice(&dev);
     (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -343,6 +368,8 @@ static int setup_batched(nuppn_ctx *c) {
     (void) hipMemGetInfo(&free_b, &total_b);
     const int by_mem = (int) std::max<size_t>(1, (free_b / 4) / c->blayout.bytes);
     c->bmax_slots = std::max(1, std::min(ncu * occ, by_mem));
+    c->bgrid[1] = std::max(1, std::min(ncu * occ1, c->bmax_slots));
+    c->bgrid[2] = std::max(1, std::min(ncu * occ2, c->bmax_slots));
     c->bnb = nb;
     return 0;
 }
@@ -400,7 +427,8 @@ static int ensure_rate_buffer(nuppn_ctx *c, int nz) {
 static int ensure_zone_state(nuppn_ctx *c, int nz) {
     if (nz <= c->nz_state_cap) return 0;
     for (void *p : {(void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
-                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd})
+                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd,
+                    (void *) c->d_active})
         dev_free(p);
     const size_t nsp = c->net.nsp;
     size_t pow2 = 1;
@@ -413,6 +441,7 @@ static int ensure_zone_state(nuppn_ctx *c, int nz) {
     HIP_OK(hipMalloc((void **) &c->d_work, sizeof(double) * c->work_stride * nz));
     HIP_OK(hipMalloc((void **) &c->d_state, sizeof(ZoneState) * nz));
     HIP_OK(hipMalloc((void **) &c->d_bd, sizeof(BDState) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_active, sizeof(int) * nz));
     c->nz_state_cap = nz;
     return 0;
 }
@@ -547,7 +576,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         dev_upload(&c->d_isdiag, s.A_isdiag) ||
         dev_upload(&c->d_perm_c, (const int *) nullptr, (size_t) s.n) ||
         dev_upload(&c->d_iperm_c, (const int *) nullptr, (size_t) s.n) ||
-        dev_upload(&c->d_flag, (const int *) nullptr, 1) ||
+        dev_upload(&c->d_flag, (const int *) nullptr, 2) ||
         dev_upload(&c->d_status, (const int *) nullptr, 4))
         return nullptr;
     if (upload_order(c.get())) return nullptr;
@@ -557,9 +586,11 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         *status = NUPPN_ERR_UNSUPPORTED;
         return nullptr;
     }
+    const void *plain_kernel = kn.integration_method == 1 ? (const void *) zone_bd_kernel
+                                                          : (const void *) zone_newton_kernel;
     if (c->lds > 64 * 1024 &&
-        hipFuncSetAttribute((const void *) zone_newton_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int) c->lds) != hipSuccess)
+        hipFuncSetAttribute(plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int) c->lds) != hipSuccess)
         return nullptr;
     if (c->lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) lu_kernel,
@@ -567,7 +598,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     int dev = 0, ncu = 0, per_cu = 0;
     (void) hipGetDevice(&dev);
     (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *) zone_newton_kernel, 64,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, plain_kernel, 64,
                                                      c->lds) != hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
@@ -582,7 +613,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
     for (auto &e : c->evc)
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    if (hipHostMalloc((void **) &c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **) &c->h_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return nullptr;
     if (ensure_slots(c.get(), 1)) return nullptr;
     if (setup_batched(c.get())) return nullptr;
@@ -600,7 +631,7 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
                     (void *) c->d_aw, (void *) c->d_zw, (void *) c->d_slots, (void *) c->d_flag,
                     (void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
                     (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd,
-                    (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
+                    (void *) c->d_active, (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
                     (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
                     (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
                     (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagk,
@@ -793,6 +824,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     Z.stats = (ZoneStats *) stats;
     Z.state = c->d_state;
     Z.bd = c->d_bd;
+    Z.active = c->d_active;
     Z.bd0 = c->bd;
     Z.flag = c->d_flag;
     Z.nz = nz;
@@ -827,7 +859,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         hipLaunchKernelGGL(zone_control_kernel, dim3(nz), dim3(64), 0, s, a);
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evc[1], s));
-        HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         float ms = 0.f;
         if (newton_pending) {
@@ -840,14 +872,22 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         if (c->h_flag[0] == 0) break;
         if (pass >= max_pass) return NUPPN_ERR_HIP;
         HIP_OK(hipEventRecord(c->evn[0], s));
+        const bool bd = c->kn.integration_method == 1;
+        const int active = std::min(c->h_flag[1], nz);   // the control pass's work list
+        a.Z.nact = active;
         if (!block) {
-            hipLaunchKernelGGL(zone_newton_kernel, dim3(grid_n), dim3(64), c->lds, s, a);
-        } else if (c->bnb == 64) {
-            hipLaunchKernelGGL((zone_newton_block_kernel<64>), dim3(grid_n), dim3(64), c->blds64,
-                               s, a, b);
+            void *args[] = {&a};
+            HIP_OK(hipLaunchKernel(bd ? (const void *) zone_bd_kernel
+                                      : (const void *) zone_newton_kernel,
+                                   dim3(std::min(grid_n, active)), dim3(64), args, c->lds, s));
         } else {
-            hipLaunchKernelGGL((zone_newton_block_kernel<32>), dim3(grid_n), dim3(64), c->blds64,
-                               s, a, b);
+            // the register-lean two-per-SIMD variant only when the single-wavefront one cannot
+            // give every zone of the list its own wavefront
+            const int waves = (!bd && active > c->bgrid[1]) ? 2 : 1;
+            const int grid = std::min({grid_n, c->bgrid[waves], active});
+            void *args[] = {&a, &b};
+            HIP_OK(hipLaunchKernel(block_kernel_ptr(c->bnb, bd ? 1 : 0, waves), dim3(grid),
+                                   dim3(64), args, c->blds64, s));
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evn[1], s));

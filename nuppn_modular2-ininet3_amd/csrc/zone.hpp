@@ -62,7 +62,10 @@ struct ZoneData {
     ZoneState *state;
     BDState *bd;        // [nz] Bader-Deuflhard state carried between steps
     BDState bd0;        // the state every zone starts from (the context's)
-    int *flag;          // [0] set when some zone needs a Newton iteration or a BD step
+    int *flag;          // [0] set when some zone needs a Newton iteration or a BD step,
+                        // [1] the number of such zones
+    int *active;        // [nz] those zones, in no particular order (the workers' work list)
+    int nact;           // entries of active the worker launch takes (set per pass)
     int nz;
     int init_prec;      // factor precision a zone starts with on the batched path (0 FP32, 1 FP64)
 };
@@ -262,7 +265,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
                 step_ended = true;
             } else {
                 s.phase = PH_NEWTON;
-                if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+                if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; Z.active[atomicAdd(&Z.flag[1], 1)] = z; }
                 return;
             }
         }
@@ -329,7 +332,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
         if (my_t9 >= K.min_t9_for_slv && K.integration_method == 1) {
             s.solved = 1;
             s.phase = PH_NEWTON;   // the worker runs bader_deuflhard_integration
-            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; Z.active[atomicAdd(&Z.flag[1], 1)] = z; }
             return;
         }
         if (my_t9 >= K.min_t9_for_slv) {
@@ -354,7 +357,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
                 continue;
             }
             s.phase = PH_NEWTON;
-            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; }
+            if (lane_id() == 0) { Z.state[z] = s; Z.flag[0] = 1; Z.active[atomicAdd(&Z.flag[1], 1)] = z; }
             return;
         }
         // below min_t9_for_slv: no nucleosynthesis, the step counts as done (solver.F90:180-183)
