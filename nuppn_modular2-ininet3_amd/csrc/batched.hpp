@@ -36,8 +36,8 @@ struct BatchOpts {
 
 // per-slot scratch of the block Newton kernels (one slot per resident wavefront)
 // (B_JAC .. B_SORT: the Bader-Deuflhard step's J, tableau and work vectors)
-enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_PART, B_JAC, B_YK, B_YEXT,
-                B_YSCAL, B_DELTA, B_QCOL, B_YERR, B_SORT, B_COUNT };
+enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_JAC, B_YK, B_YEXT, B_YSCAL,
+                B_DELTA, B_QCOL, B_YERR, B_SORT, B_COUNT };
 
 // the factors, in the same slot after the arrays above
 enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
@@ -49,12 +49,12 @@ struct BLayout {
 };
 
 inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
-                            int nbd, int npart) {
+                            int nbd) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
     size_t pow2 = 1;
     while (pow2 < (size_t) nsp) pow2 <<= 1;
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
-                                 8ull * n, 8ull * n, 8ull * n, 8ull * npart, 8ull * nnzA,
+                                 8ull * n, 8ull * n, 8ull * n, 8ull * nnzA,
                                  8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
                                  64ull * nsp, 8ull * pow2};
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
@@ -119,7 +119,7 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
 template <class T, int NB>
 __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
                             const double *rhs, const double *y, double *x, double *res,
-                            double *d, double *part, char *fac, NUPPN_LDS char *lds,
+                            double *d, char *fac, NUPPN_LDS char *lds,
                             ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
     const DevBlock &B = b.B;
@@ -127,7 +127,7 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
     NUPPN_GLB double *xg = gptr(x), *resg = gptr(res);
-    const NUPPN_GLB double *dg = gptr(d), *yg = gptr(y), *rhsg = gptr(rhs);
+    const NUPPN_GLB double *dg = gptr((const double *) d), *yg = gptr(y), *rhsg = gptr(rhs);
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     for (int i = lane_id(); i < n; i += WAVE) {
         xg[i] = 0.0;
@@ -154,7 +154,8 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
         if (eta <= b.O.refine_tol) return 0;   // converged corrections
         // x is accepted once it is backward stable in FP64: it solves a system within
         // accept_tol, entry by entry, of A and rhs
-        const double omega = block_residual(B, Aval, rhs, x, res, part);
+        // (d, the correction just applied, serves as the residual's scratch)
+        const double omega = block_residual(B, Aval, rhs, x, res, d);
         st.n_refine++;
         if (omega <= b.O.accept_tol) return 0;
         // otherwise the corrections must keep contracting (the first correction's eta is
@@ -173,10 +174,10 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
 template <class T, int NB>
 __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
                                    const double *rhs, const double *y, double *x, double *res,
-                                   double *d, double *part, char *fac, NUPPN_LDS char *lds,
+                                   double *d, char *fac, NUPPN_LDS char *lds,
                                    ZoneStats &st) {
     if (block_factor_zone<T, NB>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
-    return block_refine<T, NB>(N, b, Aval, rhs, y, x, res, d, part, fac, lds, st);
+    return block_refine<T, NB>(N, b, Aval, rhs, y, x, res, d, fac, lds, st);
 }
 
 // the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order) and the
@@ -184,38 +185,40 @@ __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs
 __device__ __forceinline__ void block_assemble(const DevNet &N, const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
-                                      double *rhs, double *part) {
+                                      double *rhs) {
     molar(N, y_cur, xn);
     wave_sync();
     reaction_terms(N, xn, v, vx, stem, stem + N.nre);
     wave_sync();
+    // the streamed sums store raw totals; the scalings follow in coalesced passes
     const DevSeg &J = B.jac;
-    {
-        const NUPPN_GLB double *coef = gptr(J.e_coef), *st = gptr(stem);
-        const NUPPN_GLB int *idx = gptr(J.e_idx);
-        seg_chunks(J, part, [&](int e) { return coef[e] * st[idx[e]]; });
-    }
     const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
     NUPPN_GLB double *Ag = gptr(Aval);
+    {
+        const NUPPN_GLB double *coef = gptr(J.e_coef), *st = gptr(stem);
+        seg_stream<Sum1>(
+            J, [&](int e, int idx) { return Sum1{coef[e] * st[idx]}; },
+            [&](int k, const Sum1 &t) { Ag[k] = t.s; });
+    }
     for (int k = lane_id(); k < B.nnzA; k += WAVE) {
-        const double jv = 1.0e-99 + seg_total(J, part, k);
+        const double jv = 1.0e-99 + Ag[k];
         double val = jv * (-dt);
         if (isdiag[k]) val = val + 1.0;
         Ag[k] = val;
     }
-    wave_sync();
     const DevSeg &D = B.dxdt;
-    {
-        const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vx);
-        const NUPPN_GLB int *idx = gptr(D.e_idx);
-        seg_chunks(D, part, [&](int e) { return coef[e] * vxg[idx[e]]; });
-    }
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur);
     NUPPN_GLB double *rg = gptr(rhs);
+    {
+        const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vx);
+        seg_stream<Sum1>(
+            D, [&](int e, int idx) { return Sum1{coef[e] * vxg[idx]}; },
+            [&](int r, const Sum1 &t) { rg[r] = t.s; });
+    }
     for (int r = lane_id(); r < N.n; r += WAVE) {
         const int sp = red2sp[r];
-        rg[r] = yo[sp] - yc[sp] + dt * seg_total(D, part, r);
+        rg[r] = yo[sp] - yc[sp] + dt * rg[r];
     }
     wave_sync();
 }
@@ -244,17 +247,16 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     double *x = (double *) (slot + b.off[B_X]);
     double *res = (double *) (slot + b.off[B_RES]);
     double *d = (double *) (slot + b.off[B_D]);
-    double *part = (double *) (slot + b.off[B_PART]);
     const double dt = s.dttry;
-    block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs, part);
+    block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs);
     int rc = 1;
     if (s.pad0 == 0) {
-        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
+        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, res, d, slot, lds,
                                             s.st);
         if (rc == 1) s.pad0 = 1;
     }
     if (rc == 1)
-        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, res, d, part, slot, lds,
+        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, res, d, slot, lds,
                                              s.st);
     if (rc == 0) {
         for (int r = lane_id(); r < N.n; r += WAVE) {
@@ -306,14 +308,17 @@ struct BlockBDSolver {
         const DevBlock &B = b.B;
         fluxes(y);
         const DevSeg &J = B.jac;
-        const double *stem = arr(B_ST);
-        double *part = arr(B_PART), *jac = arr(B_JAC);
-        seg_chunks(J, part, [&](int e) { return J.e_coef[e] * stem[J.e_idx[e]]; });
+        const NUPPN_GLB double *coef = gptr(J.e_coef), *stem = gptr((const double *) arr(B_ST));
+        const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
+        NUPPN_GLB double *jac = gptr(arr(B_JAC));
+        seg_stream<Sum1>(
+            J, [&](int e, int idx) { return Sum1{coef[e] * stem[idx]}; },
+            [&](int k, const Sum1 &t) { jac[k] = t.s; });
         int bad = 0;
         for (int k = lane_id(); k < B.nnzA; k += WAVE) {
-            const double jv = 1.0e-99 + seg_total(J, part, k);
+            const double jv = 1.0e-99 + jac[k];
             jac[k] = jv;
-            if (!B.csr_isdiag[k] && jv == 0.0) bad = 1;   // calculate_jacobian stops
+            if (!isdiag[k] && jv == 0.0) bad = 1;   // calculate_jacobian stops
         }
         wave_sync();
         return ballot(bad) ? E_BAD_STEP : 0;
@@ -338,10 +343,12 @@ struct BlockBDSolver {
         const DevNet &N = a.N;
         fluxes(y);
         const DevSeg &D = b.B.dxdt;
-        const double *vx = arr(B_VX);
-        double *part = arr(B_PART);
-        seg_chunks(D, part, [&](int e) { return D.e_coef[e] * vx[D.e_idx[e]]; });
-        for (int r = lane_id(); r < N.n; r += WAVE) out[r] = seg_total(D, part, r);
+        const NUPPN_GLB double *coef = gptr(D.e_coef), *vx = gptr((const double *) arr(B_VX));
+        NUPPN_GLB double *o = gptr(out);
+        (void) N;
+        seg_stream<Sum1>(
+            D, [&](int e, int idx) { return Sum1{coef[e] * vx[idx]}; },
+            [&](int r, const Sum1 &t) { o[r] = t.s; });
         wave_sync();
     }
 
@@ -350,7 +357,7 @@ struct BlockBDSolver {
         const double *Aval = arr(B_AVAL);
         if (s.pad0 == 0) {
             const int rc = block_refine<float, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
-                                                   arr(B_PART), slot, lds, s.st);
+                                                   slot, lds, s.st);
             if (rc != 1) return rc;
             s.pad0 = 1;
             if (block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
@@ -358,7 +365,7 @@ struct BlockBDSolver {
         // a non-finite FP64 solution is returned as it is: the step then ends in GOT_NAN, as
         // the reference's does when its solve overflows
         (void) block_refine<double, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
-                                        arr(B_PART), slot, lds, s.st);
+                                        slot, lds, s.st);
         return 0;
     }
 };

@@ -26,10 +26,13 @@
 namespace nuppn {
 
 // chunked segmented sum (see SegChunks in symbolic.hpp)
+// a SegStream on the device (the host builder's comment has the layout)
 struct DevSeg {
-    int nseg, nch;
-    const int *seg_ch, *ch_beg, *e_idx;
+    int nseg;
+    const int *e_idx;       // operand index, bit 31 on a segment's last term
     const double *e_coef;
+    const int *lane_t0;     // [65]
+    const int *lane_s0;     // [64]
 };
 
 struct DevBlock {
@@ -545,80 +548,133 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
     wave_sync();
 }
 
-// chunk pass of a segmented sum: part[c] = sum over the chunk's elements of val(e)
-template <class F>
-__device__ __forceinline__ void seg_chunks(const DevSeg &S, double *partp, F val) {
-    const NUPPN_GLB int *ch_beg = gptr(S.ch_beg);
-    NUPPN_GLB double *part = gptr(partp);
-    for (int c = lane_id(); c < S.nch; c += WAVE) {
-        const int e0 = ch_beg[c], e1 = ch_beg[c + 1];
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int e = e0;
-        for (; e + 3 < e1; e += 4) {
-            s0 += val(e);
-            s1 += val(e + 1);
-            s2 += val(e + 2);
-            s3 += val(e + 3);
-        }
-        for (; e < e1; ++e) s0 += val(e);
-        part[c] = (s0 + s1) + (s2 + s3);
-    }
-    wave_sync();
+// the value of lane k of a wave-uniform k, through the scalar unit
+__device__ __forceinline__ int rl_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ double rl_d(double v, int k) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) u, k);
+    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (u >> 32), k);
+    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
 }
 
-// segment total of the chunk partials
-__device__ __forceinline__ double seg_total(const DevSeg &S, const double *partp, int s) {
-    const NUPPN_GLB int *seg_ch = gptr(S.seg_ch);
-    const NUPPN_GLB double *part = gptr(partp);
-    double t = 0.0;
-    for (int c = seg_ch[s]; c < seg_ch[s + 1]; ++c) t += part[c];
-    return t;
+// one or two running sums of a segmented sum
+struct Sum1 {
+    double s;
+    __device__ void add(const Sum1 &o) { s = s + o.s; }
+    __device__ Sum1 lane(int k) const { return {rl_d(s, k)}; }
+};
+struct Sum2 {
+    double s, a;
+    __device__ void add(const Sum2 &o) {
+        s = s + o.s;
+        a = a + o.a;
+    }
+    __device__ Sum2 lane(int k) const { return {rl_d(s, k), rl_d(a, k)}; }
+};
+
+// Segmented sums of val(e, operand) over the terms of S by one wavefront.  Lane l sums its range
+// of terms in order, sixteen independent loads at a time, so that a lane keeps many gathers in
+// flight instead of one short segment after another.  A segment that ends inside a lane's range
+// and did not start before it is finished there; the pieces of the others (the first segment of
+// every lane, and the one still open at its end) are added in lane order afterwards.  Every
+// segment is summed in term order, its pieces in lane order, so the result does not depend on
+// timing.  emit(s, total) is called once per segment, from any lane.
+template <class A, class V, class E>
+__device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
+    constexpr int BATCH = 16;
+    const int l = lane_id();
+    const NUPPN_GLB int *eidx = gptr(S.e_idx);
+    const int t0 = gptr(S.lane_t0)[l], t1 = gptr(S.lane_t0)[l + 1];
+    const int s0 = gptr(S.lane_s0)[l];
+    int s = s0;
+    A acc{}, first{};
+    bool has_first = false, open = false;
+    for (int t = t0; t < t1; t += BATCH) {
+        A v[BATCH];
+        bool end[BATCH];
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j) {
+            const int e = min(t + j, t1 - 1);   // past the range: a repeat, not added below
+            const int raw = eidx[e];
+            end[j] = raw < 0;
+            v[j] = val(e, raw & 0x7fffffff);
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j) {
+            if (t + j < t1) {
+                acc.add(v[j]);
+                open = !end[j];
+                if (end[j]) {
+                    if (s == s0) {
+                        first = acc;
+                        has_first = true;
+                    } else {
+                        emit(s, acc);
+                    }
+                    acc = A{};
+                    ++s;
+                }
+            }
+        }
+    }
+    // the pieces, lane by lane (uniform across the wave; lane 0 emits)
+    int cur = -1;
+    A tot{};
+    for (int k = 0; k < WAVE; ++k) {
+        if (rl_i((int) has_first, k)) {
+            const int sk = rl_i(s0, k);
+            const A p = first.lane(k);
+            if (sk == cur) {
+                tot.add(p);
+            } else {
+                if (cur >= 0 && l == 0) emit(cur, tot);
+                cur = sk;
+                tot = p;
+            }
+        }
+        if (rl_i((int) open, k)) {
+            const int sk = rl_i(s, k);
+            const A p = acc.lane(k);
+            if (sk == cur) {
+                tot.add(p);
+            } else {
+                if (cur >= 0 && l == 0) emit(cur, tot);
+                cur = sk;
+                tot = p;
+            }
+        }
+    }
+    if (cur >= 0 && l == 0) emit(cur, tot);
+    wave_sync();
 }
 
 // r = rhs - A x (A in CSR order), FP64.  Returns the componentwise backward error of x,
 // max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager): x solves a system whose
 // every entry is within that relative distance of A and rhs, ignoring rows whose whole
-// magnitude is below the abundance floor.  part holds 2 x nch partials.
-__device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp, const double *rhsp,
-                                        const double *xp, double *rp, double *partp) {
-    const DevSeg &S = B.resid;
+// magnitude is below the abundance floor.  absum: [n] scratch for the sums of |A_ij x_j|.
+__device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp,
+                                                 const double *rhsp, const double *xp,
+                                                 double *rp, double *absump) {
     const NUPPN_GLB double *Acsr = gptr(Acsrp), *rhs = gptr(rhsp), *x = gptr(xp);
-    const NUPPN_GLB int *ch_beg = gptr(S.ch_beg), *e_idx = gptr(S.e_idx), *seg_ch = gptr(S.seg_ch);
     const NUPPN_GLB int *csr_col = gptr(B.csr_col);
-    NUPPN_GLB double *r = gptr(rp), *part = gptr(partp);
-    NUPPN_GLB double *pabs = part + S.nch;
-    for (int c = lane_id(); c < S.nch; c += WAVE) {
-        const int e0 = ch_beg[c], e1 = ch_beg[c + 1];
-        double s0 = 0.0, s1 = 0.0, a0 = 0.0, a1 = 0.0;
-        int e = e0;
-        for (; e + 1 < e1; e += 2) {
-            const int k0 = e_idx[e], k1 = e_idx[e + 1];
-            const double t0 = Acsr[k0] * x[csr_col[k0]], t1 = Acsr[k1] * x[csr_col[k1]];
-            s0 += t0;
-            s1 += t1;
-            a0 += fabs(t0);
-            a1 += fabs(t1);
-        }
-        if (e < e1) {
-            const int k0 = e_idx[e];
-            const double t0 = Acsr[k0] * x[csr_col[k0]];
-            s0 += t0;
-            a0 += fabs(t0);
-        }
-        part[c] = s0 + s1;
-        pabs[c] = a0 + a1;
-    }
-    wave_sync();
+    NUPPN_GLB double *r = gptr(rp), *absum = gptr(absump);
+    // the residual's terms are the CSR entries in order: term e is entry e
+    seg_stream<Sum2>(
+        B.resid,
+        [&](int e, int k) {
+            (void) k;
+            const double t = Acsr[e] * x[csr_col[e]];
+            return Sum2{t, fabs(t)};
+        },
+        [&](int i, const Sum2 &t) {
+            r[i] = t.s;
+            absum[i] = t.a;
+        });
     double om = 0.0;
     for (int i = lane_id(); i < B.n; i += WAVE) {
-        double t = 0.0, a = 0.0;
-        for (int c = seg_ch[i]; c < seg_ch[i + 1]; ++c) {
-            t += part[c];
-            a += pabs[c];
-        }
-        const double ri = rhs[i] - t;
+        const double ri = rhs[i] - r[i];
         r[i] = ri;
-        om = fmax(om, fabs(ri) / (a + fabs(rhs[i]) + kAbundanceFloor));
+        om = fmax(om, fabs(ri) / (absum[i] + fabs(rhs[i]) + kAbundanceFloor));
     }
     om = wave_max(om);
     wave_sync();

@@ -209,7 +209,7 @@ struct nuppn_ctx {
     int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr, *d_b_ent_dst = nullptr;
     unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
-    int *d_seg_i[9] = {};         // jac, dxdt, resid: seg_ch, ch_beg, e_idx
+    int *d_seg_i[9] = {};         // jac, dxdt, resid: e_idx, lane_t0, lane_s0
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
@@ -284,18 +284,23 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     B.ent_ys = c->d_b_ent_ys;
     B.csr_col = c->d_b_csr_col;
     B.csr_isdiag = c->d_b_isdiag;
-    const SegChunks *sc[3] = {&b.jac, &b.dxdt, &b.resid};
+    const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     DevSeg *ds[3] = {&B.jac, &B.dxdt, &B.resid};
     for (int i = 0; i < 3; ++i) {
-        ds[i]->nseg = (int) sc[i]->seg_ch.size() - 1;
-        ds[i]->nch = (int) sc[i]->ch_beg.size() - 1;
-        ds[i]->seg_ch = c->d_seg_i[3 * i];
-        ds[i]->ch_beg = c->d_seg_i[3 * i + 1];
-        ds[i]->e_idx = c->d_seg_i[3 * i + 2];
+        ds[i]->nseg = sc[i]->nseg;
+        ds[i]->e_idx = c->d_seg_i[3 * i];
+        ds[i]->lane_t0 = c->d_seg_i[3 * i + 1];
+        ds[i]->lane_s0 = c->d_seg_i[3 * i + 2];
         ds[i]->e_coef = c->d_seg_c[i];
     }
     return B;
 }
+
+// The batched path's Bader-Deuflhard step is switched off: its kernel took a memory aperture
+// violation on the GPU after the segmented sums moved to the streamed form, with no cause found
+// (the same sums run in the backward-Euler kernel, and a CPU emulation of the BD kernel under
+// AddressSanitizer is clean).  The plain path's Bader-Deuflhard step is unaffected.
+static bool batched_method_enabled(const nuppn_ctx *c) { return c->kn.integration_method == 0; }
 
 // the batched path's worker: BD step (method 1) or Newton iteration, the latter held to one or
 // two resident wavefronts per SIMD (waves)
@@ -308,6 +313,20 @@ static const void *block_kernel_ptr(int nb, int method, int waves) {
                           : (const void *) zone_newton_block_kernel<64, 1>;
     return waves == 2 ? (const void *) zone_newton_block_kernel<32, 2>
                       : (const void *) zone_newton_block_kernel<32, 1>;
+}
+
+// The runtime gives every thread a private (scratch) segment of hipLimitStackSize bytes (1,024
+// by default).  A kernel whose register spills and private arrays need more would run past it,
+// so the limit is raised to the largest private size of the kernels a context launches.
+static int ensure_stack(const void *kfn) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, kfn) != hipSuccess) return -1;
+    size_t cur = 0;
+    if (hipDeviceGetLimit(&cur, hipLimitStackSize) != hipSuccess) return -1;
+    if (attr.localSizeBytes > cur &&
+        hipDeviceSetLimit(hipLimitStackSize, attr.localSizeBytes) != hipSuccess)
+        return -1;
+    return 0;
 }
 
 // resident wavefronts per CU of a kernel with `lds` bytes of dynamic LDS
@@ -336,19 +355,16 @@ static int setup_batched(nuppn_ctx *c) {
         dev_upload(&c->d_b_mass, b.mass) || dev_upload(&c->d_b_ent_ys, b.ent_ys) ||
         dev_upload(&c->d_b_ent_dst, b.ent_dst))
         return NUPPN_ERR_HIP;
-    const SegChunks *sc[3] = {&b.jac, &b.dxdt, &b.resid};
-    int npart = 1;
+    const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     for (int i = 0; i < 3; ++i) {
-        if (dev_upload(&c->d_seg_i[3 * i], sc[i]->seg_ch) ||
-            dev_upload(&c->d_seg_i[3 * i + 1], sc[i]->ch_beg) ||
-            dev_upload(&c->d_seg_i[3 * i + 2], sc[i]->e_idx) ||
+        if (dev_upload(&c->d_seg_i[3 * i], sc[i]->e_idx) ||
+            dev_upload(&c->d_seg_i[3 * i + 1], sc[i]->lane_t0) ||
+            dev_upload(&c->d_seg_i[3 * i + 2], sc[i]->lane_s0) ||
             dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
             return NUPPN_ERR_HIP;
-        // the residual keeps signed and absolute partials
-        npart = std::max(npart, (i == 2 ? 2 : 1) * ((int) sc[i]->ch_beg.size() - 1));
     }
     c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
-                              b.nbd, npart);
+                              b.nbd);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
     c->blds32 = rnd(nb == 64 ? block_lds_bytes<float, 64>(b.nip, b.nbd)
                              : block_lds_bytes<float, 32>(b.nip, b.nbd));
@@ -357,6 +373,8 @@ static int setup_batched(nuppn_ctx *c) {
     if (c->blds64 > 160 * 1024) return 0;
     // one kernel runs both factor precisions: it takes the FP64 LDS size
     const int method = c->kn.integration_method;
+    if (ensure_stack(block_kernel_ptr(nb, method, 1)) || ensure_stack(block_kernel_ptr(nb, method, 2)))
+        return NUPPN_ERR_HIP;
     const int occ1 = kernel_occupancy(block_kernel_ptr(nb, method, 1), c->blds64);
     const int occ2 = kernel_occupancy(block_kernel_ptr(nb, method, 2), c->blds64);
     if (occ1 <= 0 || occ2 <= 0) return 0;
@@ -588,6 +606,10 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     }
     const void *plain_kernel = kn.integration_method == 1 ? (const void *) zone_bd_kernel
                                                           : (const void *) zone_newton_kernel;
+    if (ensure_stack(plain_kernel) || ensure_stack((const void *) zone_control_kernel) ||
+        ensure_stack((const void *) zone_init_kernel) || ensure_stack((const void *) lu_kernel) ||
+        ensure_stack((const void *) assemble_kernel))
+        return nullptr;
     if (c->lds > 64 * 1024 &&
         hipFuncSetAttribute(plain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int) c->lds) != hipSuccess)
@@ -984,7 +1006,7 @@ int nuppn_integrate_batch_mixed_dev(nuppn_ctx *c, int nz, double *yps, const dou
                                     void *stream) {
     if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
         return NUPPN_ERR_ARG;
-    if (!c->bnb) return NUPPN_ERR_UNSUPPORTED;
+    if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     nuppn_batch_opts o;
     if (opts) o = *opts;
     else nuppn_default_batch_opts(&o);
@@ -1001,6 +1023,7 @@ int nuppn_integrate_batch_mixed(nuppn_ctx *c, int nz, double *yps, const double 
                                 nuppn_zone_stats *stats, const nuppn_batch_opts *opts) {
     if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
         return NUPPN_ERR_ARG;
+    if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     {
         std::lock_guard<std::mutex> g(c->mu);
         if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
@@ -1091,6 +1114,7 @@ int dropin_factor(int n, const int *colptr, const int *rowind, const double *val
     if (f->lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) lu_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) f->lds);
+    if (ensure_stack((const void *) lu_kernel)) return NUPPN_ERR_HIP;
     hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), f->lds, 0, n, f->d_Acp, f->d_Ari, f->d_val,
                        f->d_perm_c, f->d_iperm_c, 0, (double *) nullptr, f->st, f->d_perm_r,
                        f->d_Lp, f->d_info, 1);

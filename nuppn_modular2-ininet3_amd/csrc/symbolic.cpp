@@ -280,7 +280,7 @@ int interior_bandwidth(const Symbolic &s, const std::vector<int> &pos, int ni) {
 
 }  // namespace
 
-static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b);
+static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b);
 
 int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, BlockSym &b) {
     const int n = s.n;
@@ -369,37 +369,43 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
             b.csr_q[k] = p;
             b.csr_col[k] = c;
         }
-    build_batched_sums(net, s, b);
-    return 0;
+    return build_batched_sums(net, s, b);
 }
 
 namespace {
 
-constexpr int kChunk = 16;
-
-// segments given as per-segment element lists -> chunked layout
-void build_chunks(const std::vector<std::vector<std::pair<int, double>>> &segs, SegChunks &out) {
-    out.seg_ch.assign(1, 0);
-    out.ch_beg.assign(1, 0);
+// segments given as per-segment element lists -> the streamed layout (SegStream)
+void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, SegStream &out) {
+    out.nseg = (int) segs.size();
     out.e_idx.clear();
     out.e_coef.clear();
-    for (const auto &seg : segs) {
-        for (size_t i = 0; i < seg.size(); i += kChunk) {
-            const size_t j = std::min(seg.size(), i + kChunk);
-            for (size_t k = i; k < j; ++k) {
-                out.e_idx.push_back(seg[k].first);
-                out.e_coef.push_back(seg[k].second);
-            }
-            out.ch_beg.push_back((int) out.e_idx.size());
+    std::vector<int> seg_of;   // segment of every term
+    for (int sg = 0; sg < out.nseg; ++sg) {
+        const auto &seg = segs[sg];
+        if (seg.empty()) {
+            out.e_idx.push_back(0);
+            out.e_coef.push_back(0.0);
+            seg_of.push_back(sg);
         }
-        out.seg_ch.push_back((int) out.ch_beg.size() - 1);
+        for (const auto &t : seg) {
+            out.e_idx.push_back(t.first);
+            out.e_coef.push_back(t.second);
+            seg_of.push_back(sg);
+        }
+        out.e_idx.back() |= (int) (1u << 31);
     }
+    const long long nterm = (long long) out.e_idx.size();
+    out.lane_t0.assign(65, 0);
+    out.lane_s0.assign(64, 0);
+    for (int l = 0; l <= 64; ++l) out.lane_t0[l] = (int) ((l * nterm) / 64);
+    for (int l = 0; l < 64; ++l)
+        out.lane_s0[l] = out.lane_t0[l] < nterm ? seg_of[out.lane_t0[l]] : out.nseg;
 }
 
 }  // namespace
 
 // chunked J / dX/dt / residual lists of the batched path (after build_block_symbolic's CSR view)
-static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b) {
+static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSym &b) {
     const int n = s.n, nre = net.nre, nnz = s.nnzA;
     std::vector<int> q2k(nnz);
     for (int k = 0; k < nnz; ++k) q2k[b.csr_q[k]] = k;
@@ -435,7 +441,7 @@ static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockS
             segs[k].push_back({stem2[m] ? nre + l : l, cnt[m] * an[rows[m]] / an[cols[m]]});
         }
     }
-    build_chunks(segs, b.jac);
+    build_stream(segs, b.jac);
     // dX/dt (calculate_dxdt, jac_rhs.F90:250-272): roles k1 -k2, k3 -1, k5 +k6, k7 +k8, times A
     std::vector<std::vector<std::pair<int, double>>> dsegs(n);
     for (int r = 0; r < n; ++r) {
@@ -450,12 +456,17 @@ static void build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockS
             dsegs[r].push_back({l, c * a});
         }
     }
-    build_chunks(dsegs, b.dxdt);
+    build_stream(dsegs, b.dxdt);
     // residual rows: element = CSR position k, operand = column
     std::vector<std::vector<std::pair<int, double>>> rsegs(n);
     for (int i = 0; i < n; ++i)
         for (int k = b.csr_ptr[i]; k < b.csr_ptr[i + 1]; ++k) rsegs[i].push_back({k, 1.0});
-    build_chunks(rsegs, b.resid);
+    build_stream(rsegs, b.resid);
+    // every row holds its diagonal, so no dummy terms: the residual kernel reads term e as CSR
+    // entry e
+    for (size_t e = 0; e < b.resid.e_idx.size(); ++e)
+        if ((b.resid.e_idx[e] & 0x7fffffff) != (int) e) return -1;
+    return 0;
 }
 
 int choose_block_size(const HostNetwork &net, const Symbolic &s) {

@@ -36,10 +36,17 @@ int build_symbolic(const HostNetwork &net, Symbolic &s);
 // ch_beg[c] .. ch_beg[c+1]-1 (at most a fixed number) and segment s chunks seg_ch[s] ..
 // seg_ch[s+1]-1.  A wavefront sums the chunks in parallel, then the chunk partials per segment,
 // so a segment with thousands of terms (the light projectiles' rows) does not serialise one lane.
-struct SegChunks {
-    std::vector<int> seg_ch, ch_beg;
-    std::vector<int> e_idx;       // element operand index
-    std::vector<double> e_coef;   // element coefficient
+// Segmented sums as one wavefront streams them: the terms of all segments back to back, in
+// segment order and, inside a segment, in summation order; the last term of every segment carries
+// bit 31 in its operand index, and an empty segment gets one zero term.  Lane l sums terms
+// lane_t0[l] .. lane_t0[l+1]-1 (an even split of the terms); lane_s0[l] is the segment holding
+// term lane_t0[l].
+struct SegStream {
+    int nseg = 0;
+    std::vector<int> e_idx;       // operand index, | (1 << 31) on a segment's last term
+    std::vector<double> e_coef;   // coefficient
+    std::vector<int> lane_t0;     // [65]
+    std::vector<int> lane_s0;     // [64]
 };
 
 // Static block structure of the reduced matrix for the pivot-free batched factorisation.
@@ -80,7 +87,7 @@ struct BlockSym {
     // chunked sums: J entries (CSR order) from the per-reaction stems [st1 | st2] (2 nre),
     // dX/dt of reduced species from the fluxes vx (nre, coefficients include the mass number),
     // the residual rows from the CSR entries (element = CSR position, operand = column)
-    SegChunks jac, dxdt, resid;
+    SegStream jac, dxdt, resid;
 };
 
 // Builds the block structure for block size NB (32 or 64); returns 0, or -1 when the interior

@@ -64,6 +64,19 @@ def test_bd_trajectory_c0(pkg, oracle, hip):
         ctx.close()
 
 
+def test_bd_batched_path_unsupported(pkg, hip):
+    """The batched path refuses integration_method 1 (switched off) with NUPPN_ERR_UNSUPPORTED and
+    leaves the abundances untouched."""
+    w = pkg.network.config(1, nz=8)
+    sel = np.array([0])
+    ctx = hip.Context(w.net, hip.default_knobs(integration_method=1))
+    try:
+        with pytest.raises(RuntimeError, match=str(hip.ERR_UNSUPPORTED)):
+            ctx.integrate_batch_mixed(w.x0[sel], *_args(w, sel))
+    finally:
+        ctx.close()
+
+
 def test_bd_batch_matches_oracle_c1(pkg, oracle, hip):
     """1,100-species zones (T9 0.08 .. 0.43) in one batch call, every zone from the entry BD
     state; the oracle per zone from the same state (reset before each zone).  (Zones with more
@@ -92,6 +105,10 @@ def test_bd_batch_matches_oracle_c1(pkg, oracle, hip):
         ctx.close()
 
 
+@pytest.mark.skip(reason="batched-path Bader-Deuflhard switched off: its kernel took a GPU memory "
+                         "aperture violation after the streamed segmented sums, cause not found; "
+                         "nuppn_integrate_batch_mixed returns NUPPN_ERR_UNSUPPORTED for "
+                         "integration_method 1")
 @pytest.mark.parametrize("prec", [32, 64])
 def test_bd_batched_matches_plain(pkg, hip, prec):
     """The batched path's BD (block factors in FP32 or FP64, every stage solve refined to
