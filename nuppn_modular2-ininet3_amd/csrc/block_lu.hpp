@@ -548,20 +548,11 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
     wave_sync();
 }
 
-// the value of lane k of a wave-uniform k, through the scalar unit
-__device__ __forceinline__ int rl_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
-__device__ __forceinline__ double rl_d(double v, int k) {
-    const unsigned long long u = __double_as_longlong(v);
-    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) u, k);
-    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (u >> 32), k);
-    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
-}
-
 // one or two running sums of a segmented sum
 struct Sum1 {
     double s;
     __device__ void add(const Sum1 &o) { s = s + o.s; }
-    __device__ Sum1 lane(int k) const { return {rl_d(s, k)}; }
+    __device__ Sum1 lane(int k) const { return {readlane_d(s, k)}; }
 };
 struct Sum2 {
     double s, a;
@@ -569,7 +560,7 @@ struct Sum2 {
         s = s + o.s;
         a = a + o.a;
     }
-    __device__ Sum2 lane(int k) const { return {rl_d(s, k), rl_d(a, k)}; }
+    __device__ Sum2 lane(int k) const { return {readlane_d(s, k), readlane_d(a, k)}; }
 };
 
 // Segmented sums of val(e, operand) over the terms of S by one wavefront.  Lane l sums its range
@@ -621,8 +612,8 @@ __device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
     int cur = -1;
     A tot{};
     for (int k = 0; k < WAVE; ++k) {
-        if (rl_i((int) has_first, k)) {
-            const int sk = rl_i(s0, k);
+        if (readlane_i((int) has_first, k)) {
+            const int sk = readlane_i(s0, k);
             const A p = first.lane(k);
             if (sk == cur) {
                 tot.add(p);
@@ -632,8 +623,8 @@ __device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
                 tot = p;
             }
         }
-        if (rl_i((int) open, k)) {
-            const int sk = rl_i(s, k);
+        if (readlane_i((int) open, k)) {
+            const int sk = readlane_i(s, k);
             const A p = acc.lane(k);
             if (sk == cur) {
                 tot.add(p);

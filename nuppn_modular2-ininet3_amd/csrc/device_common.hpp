@@ -24,9 +24,15 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));
 }
 
-__device__ __forceinline__ int readlane_i(int v, int l) { return __shfl(v, l, WAVE); }
+// value of lane l, for a lane index that is the same on every lane (through the scalar unit)
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-__device__ __forceinline__ double readlane_d(double v, int l) { return __shfl(v, l, WAVE); }
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const unsigned long long u = (unsigned long long) __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) u, l);
+    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (u >> 32), l);
+    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
 
 // LDS address space: pointers into the workgroup's LDS compile to ds_* instructions
 #define NUPPN_LDS __attribute__((address_space(3)))
@@ -55,7 +61,8 @@ __device__ __forceinline__ int wave_max_i(int v) {
 }
 
 // s = ((0 + a[0]) + a[1]) + ... : the sequential order of Fortran's SUM (every lane gets s)
-__device__ inline double seq_sum(const double *a, int n) {
+template <class P>
+__device__ inline double seq_sum(P a, int n) {
     double s = 0.0;
     for (int base = 0; base < n; base += WAVE) {
         int i = base + lane_id();

@@ -46,8 +46,10 @@ constexpr double SUM_TOL = 1.0e-6;
 constexpr double SMALL_C = 1.0e-30;
 
 // -------------------------------------------------------------------------------------------
-// bitonic sort (ascending) of a global buffer of length m (power of two) by one wavefront
-__device__ inline void wave_bitonic_sort(double *a, int m) {
+// bitonic sort (ascending) of a buffer (global or LDS) of length m (power of two) by one
+// wavefront
+template <class P>
+__device__ inline void wave_bitonic_sort(P a, int m) {
     for (int k = 2; k <= m; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = lane_id(); i < m; i += WAVE) {
@@ -64,7 +66,8 @@ __device__ inline void wave_bitonic_sort(double *a, int m) {
 }
 
 // sortedsum (utils.F90:481-501): sum in ascending order.  buf holds >= pow2(n) doubles.
-__device__ inline double sorted_sum(const double *vals, int n, double *buf) {
+template <class P>
+__device__ inline double sorted_sum(const double *vals, int n, P buf) {
     int m = 1;
     while (m < n) m <<= 1;
     for (int i = lane_id(); i < m; i += WAVE) buf[i] = (i < n) ? vals[i] : __builtin_inf();
@@ -74,7 +77,8 @@ __device__ inline double sorted_sum(const double *vals, int n, double *buf) {
 }
 
 // calculate_ye (utils.F90:537-553)
-__device__ inline double calculate_ye(const DevNet &N, const double *x, double *eta, double *buf) {
+template <class P>
+__device__ inline double calculate_ye(const DevNet &N, const double *x, double *eta, P buf) {
     for (int i = lane_id(); i < N.nsp; i += WAVE)
         eta[i] = (i != N.igamma) ? (N.aw[i] - 2.0 * N.zw[i]) * x[i] / N.aw[i] : 0.0;
     wave_sync();

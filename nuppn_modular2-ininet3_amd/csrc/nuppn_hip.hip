@@ -63,9 +63,19 @@ __global__ void __launch_bounds__(256) rates_kernel(const double *__restrict__ c
 
 
 
-__global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) { zone_init(a, blockIdx.x); }
+// dynamic LDS of both: pow2(nsp) doubles, the sort buffer of the sorted sums
+__global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    zone_init(a, blockIdx.x, (NUPPN_LDS double *) smem);
+}
 
-__global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) { zone_control(a, blockIdx.x); }
+// one controller pass over the zones of the previous pass's work list (every zone at first: the
+// zones not on it are done)
+__global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    for (int i = blockIdx.x; i < a.Z.nctl; i += gridDim.x)
+        zone_control(a, a.Z.ctl ? a.Z.ctl[i] : i, (NUPPN_LDS double *) smem);
+}
 
 // one Newton iteration (or Bader-Deuflhard step) for every zone in PH_NEWTON; each wavefront
 // owns one scratch slot and strides over the zones
@@ -191,7 +201,8 @@ struct nuppn_ctx {
     double *d_work = nullptr;
     ZoneState *d_state = nullptr;
     BDState *d_bd = nullptr;      // [nz] per-zone Bader-Deuflhard state of the running call
-    int *d_active = nullptr;      // [nz] the zones a worker pass takes
+    int *d_active = nullptr;      // [2][nz] the work lists of consecutive passes
+    size_t sort_lds = 0;          // LDS of the control and init kernels (sorted-sum buffer)
     BDTables bdt{};               // bader_deuflhard_init's tables
     BDState bd{};                 // the reference's SAVE'd BD state, carried between calls
     size_t work_stride = 0;
@@ -459,7 +470,7 @@ static int ensure_zone_state(nuppn_ctx *c, int nz) {
     HIP_OK(hipMalloc((void **) &c->d_work, sizeof(double) * c->work_stride * nz));
     HIP_OK(hipMalloc((void **) &c->d_state, sizeof(ZoneState) * nz));
     HIP_OK(hipMalloc((void **) &c->d_bd, sizeof(BDState) * nz));
-    HIP_OK(hipMalloc((void **) &c->d_active, sizeof(int) * nz));
+    HIP_OK(hipMalloc((void **) &c->d_active, 2 * sizeof(int) * nz));
     c->nz_state_cap = nz;
     return 0;
 }
@@ -599,6 +610,23 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         return nullptr;
     if (upload_order(c.get())) return nullptr;
     c->layout = slot_layout(h.nsp, h.nre, s.n, s.nnzA);
+    {
+        size_t pow2 = 1;
+        while (pow2 < (size_t) h.nsp) pow2 <<= 1;
+        c->sort_lds = 8 * pow2;
+        if (c->sort_lds > 160 * 1024) {
+            *status = NUPPN_ERR_UNSUPPORTED;
+            return nullptr;
+        }
+        if (c->sort_lds > 64 * 1024 &&
+            (hipFuncSetAttribute((const void *) zone_control_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) c->sort_lds) != hipSuccess ||
+             hipFuncSetAttribute((const void *) zone_init_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) c->sort_lds) != hipSuccess))
+            return nullptr;
+    }
     c->lds = lds_bytes(s.n);
     if (c->lds > 160 * 1024) {
         *status = NUPPN_ERR_UNSUPPORTED;
@@ -847,6 +875,8 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     Z.state = c->d_state;
     Z.bd = c->d_bd;
     Z.active = c->d_active;
+    Z.ctl = nullptr;
+    Z.nctl = nz;
     Z.bd0 = c->bd;
     Z.flag = c->d_flag;
     Z.nz = nz;
@@ -866,7 +896,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     }
     const int grid_n = std::min(nz, block ? c->bslots : c->nslots);
     HIP_OK(hipEventRecord(c->ev[2], s));
-    hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), c->sort_lds, s, a);
     HIP_OK(hipGetLastError());
     // each pass advances every zone by at least one Newton iteration or finishes it
     const long long max_pass = (long long) c->kn.max_sub_steps * (c->kn.ittd + 2) + 16;
@@ -876,9 +906,12 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     c->control_ms = 0.0;
     c->newton_launches = 0;
     for (;; ++pass) {
+        // this pass builds work list pass % 2; its control pass takes the previous list
+        a.Z.active = c->d_active + (size_t) (pass & 1) * nz;
         hipLaunchKernelGGL(flag_reset_kernel, dim3(1), dim3(1), 0, s, c->d_flag);
         HIP_OK(hipEventRecord(c->evc[0], s));
-        hipLaunchKernelGGL(zone_control_kernel, dim3(nz), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(zone_control_kernel, dim3(std::max(1, a.Z.nctl)), dim3(64), c->sort_lds,
+                           s, a);
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evc[1], s));
         HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -897,6 +930,9 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         const bool bd = c->kn.integration_method == 1;
         const int active = std::min(c->h_flag[1], nz);   // the control pass's work list
         a.Z.nact = active;
+        // the next control pass takes this list
+        a.Z.ctl = a.Z.active;
+        a.Z.nctl = active;
         if (!block) {
             void *args[] = {&a};
             HIP_OK(hipLaunchKernel(bd ? (const void *) zone_bd_kernel

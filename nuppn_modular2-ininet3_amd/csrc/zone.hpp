@@ -66,6 +66,8 @@ struct ZoneData {
                         // [1] the number of such zones
     int *active;        // [nz] those zones, in no particular order (the workers' work list)
     int nact;           // entries of active the worker launch takes (set per pass)
+    const int *ctl;     // the zones the control pass takes: the previous pass's work list, or
+    int nctl;           // every zone (ctl null) in the first pass
     int nz;
     int init_prec;      // factor precision a zone starts with on the batched path (0 FP32, 1 FP64)
 };
@@ -153,13 +155,14 @@ __device__ inline double interp2(double x1, double y0, double y1, double xw) {
 __device__ inline double *zrow(double *base, size_t stride, int z) { return base + (size_t) z * stride; }
 
 // set-up of a zone at the start of a batch: yps_old = yps, entry Ye, phase SUBSTEP
-__device__ inline void zone_init(const KArgs &a, int z) {
+// sb: LDS of pow2(nsp) doubles for the sorted sums
+__device__ inline void zone_init(const KArgs &a, int z, NUPPN_LDS double *sb) {
     const DevNet &N = a.N;
     const ZoneData &Z = a.Z;
     double *yps = zrow(Z.yps, Z.nsp_stride, z);
     copy_vec(zrow(Z.yps_old, Z.nsp_stride, z), yps, N.nsp);
     double *work = zrow(Z.work, Z.work_stride, z);
-    const double ye = calculate_ye(N, yps, work, work + Z.nsp_stride);
+    const double ye = calculate_ye(N, yps, work, sb);
     if (lane_id() == 0) {
         ZoneState s = {};
         s.phase = PH_SUBSTEP;
@@ -177,7 +180,8 @@ __device__ inline void zone_init(const KArgs &a, int z) {
 }
 
 // final bookkeeping of a zone (solver.F90:229-237): clip, Ye, outputs
-__device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr, bool loop_done) {
+__device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr, bool loop_done,
+                                   NUPPN_LDS double *sb) {
     const DevNet &N = a.N;
     const ZoneData &Z = a.Z;
     double *yps = zrow(Z.yps, Z.nsp_stride, z);
@@ -193,7 +197,7 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
     double ye = s.ye_entry;
     if (s.solved) {
         double *work = zrow(Z.work, Z.work_stride, z);
-        ye = calculate_ye(N, yps, work, work + Z.nsp_stride);
+        ye = calculate_ye(N, yps, work, sb);
     }
     s.phase = PH_DONE;
     s.ierr = ierr;
@@ -207,7 +211,7 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
 
 // one pass of the controller for zone z: consume the result of a Newton iteration if there is
 // one, then advance sub-steps until the zone needs another Newton iteration or is done.
-__device__ inline void zone_control(const KArgs &a, int z) {
+__device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb) {
     const DevNet &N = a.N;
     const DevKnobs &K = a.K;
     const ZoneData &Z = a.Z;
@@ -230,7 +234,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
     if (s.phase == PH_AFTER_NEWTON && K.integration_method == 1) {
         // a Bader-Deuflhard step returned (do_one_time_step, solver.F90:164-179)
         if (s.lib_err || s.step_ierr == E_BAD_STEP) {
-            zone_finish(a, z, s, E_BAD_STEP, false);
+            zone_finish(a, z, s, E_BAD_STEP, false, sb);
             if (lane_id() == 0) Z.state[z] = s;
             return;
         }
@@ -238,7 +242,7 @@ __device__ inline void zone_control(const KArgs &a, int z) {
         step_ended = true;
     } else if (s.phase == PH_AFTER_NEWTON) {
         if (s.lib_err) {   // the reference stops here (zero in the matrix, singular factor)
-            zone_finish(a, z, s, 3, false);
+            zone_finish(a, z, s, 3, false, sb);
             if (lane_id() == 0) Z.state[z] = s;
             return;
         }
@@ -275,14 +279,14 @@ __device__ inline void zone_control(const KArgs &a, int z) {
         if (step_ended) {
             step_ended = false;
             // back in integrate_network after do_one_time_step (solver.F90:165-225)
-            if (s.dttry < SMALL_C) { zone_finish(a, z, s, 11, false); break; }
+            if (s.dttry < SMALL_C) { zone_finish(a, z, s, 11, false, sb); break; }
             if (ierr == 0) {
                 s.dtdone = s.dtdone + s.dttry;
                 for (int j = lane_id(); j < nsp; j += WAVE) yps[j] = fmax(yps[j], 1.0e-99);
                 wave_sync();
                 copy_vec(yps_old, yps, nsp);
                 s.st.nsucc++;
-                if (s.dtdone >= dtwant) { s.st.nsubt = s.i; zone_finish(a, z, s, 0, true); break; }
+                if (s.dtdone >= dtwant) { s.st.nsubt = s.i; zone_finish(a, z, s, 0, true, sb); break; }
             } else {
                 copy_vec(yps, yps_old, nsp);
                 s.st.nfail++;
@@ -290,14 +294,14 @@ __device__ inline void zone_control(const KArgs &a, int z) {
             s.dttry = fmin(s.dtrcmd, dtwant - s.dtdone);
             // MAX_SUBSTEPS and SMALL_DT return before nsubt is assigned (solver.F90:174-178,
             // 218-225): the per-call count stays 0
-            if (s.i == K.max_sub_steps) { zone_finish(a, z, s, 2, false); break; }
+            if (s.i == K.max_sub_steps) { zone_finish(a, z, s, 2, false, sb); break; }
         }
         // next sub-step (solver.F90:108-164)
         s.i += 1;
         if (s.i > 1 && lane_id() == 0) Z.bd[z].frst = 0;   // solver.F90:109
         if (s.i > K.max_sub_steps) {   // loop ran out through the halving branch
             s.st.nsubt = s.i;
-            zone_finish(a, z, s, ierr, true);
+            zone_finish(a, z, s, ierr, true, sb);
             break;
         }
         double my_t9, my_rho, my_t9_0 = 0.0, my_t9_1 = 0.0, my_rho_0 = 0.0, my_rho_1 = 0.0;
