@@ -43,6 +43,20 @@ def _args():
     return p.parse_args()
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "zone_newton_block_kernel_2048x1100_backward_euler_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the Newton kernel from the committed rocprofv3 counter passes
+    (FETCH_SIZE and WRITE_SIZE, calibrated against a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            return float(json.load(f)["hbm_bytes_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -237,8 +251,9 @@ def main():
                                "FP32 + FP64 refinement",
                    "zones": nz, "species": net.nsp, "reactions": net.nre, "path": "mixed"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "zone_newton_block_kernel<32> (one launch per pass)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
+                     "kernel": "zone_newton_block_kernel<32, 1 or 2> (one launch per pass)",
                      "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s},
         "cpu_baseline": cpu,
         "detail": {
