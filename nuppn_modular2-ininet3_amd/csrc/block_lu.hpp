@@ -67,7 +67,6 @@ struct BlockCfg {
     static constexpr int LD = NB + 4;     // LDS leading dimension (16-byte aligned, conflict-free)
 };
 
-// LDS of one wavefront: three NB x LD operand buffers, pivot row / column, the solve vector
 template <class T>
 // LDS of one wavefront: the two block-product operands X1, X2 of the factorisation loop, which
 // the solve vector vec overlays (the loop and the solves never run at the same time), the
@@ -251,6 +250,96 @@ __device__ __forceinline__ void gemm_lds_sub(const NUPPN_LDS T *Acm, const NUPPN
     }
 }
 
+// --- 32 x 32 block products on the matrix cores (NB = 32): 2 x 2 tiles of 16 x 16, k in steps of
+// 4 (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32).  Lane l supplies A[16 ti + l % 16][4 kk + l / 16]
+// and B[4 kk + l / 16][16 tj + l % 16]; result register r of lane l is column l % 16 of the tile,
+// row l / 16 + 4 r with FP64 and 4 (l / 16) + r with FP32 (the two instructions' layouts, checked
+// on the MI355X).
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+struct MVec;
+template <>
+struct MVec<double> {
+    typedef mf64x4 type;
+};
+template <>
+struct MVec<float> {
+    typedef mf32x4 type;
+};
+
+// the four result tiles (ti, tj) of a 32 x 32 product, tile 2 ti + tj
+template <class T>
+struct MTiles {
+    typename MVec<T>::type t[4];
+};
+
+__device__ __forceinline__ mf64x4 mfma16(double a, double b, mf64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mf32x4 mfma16(float a, float b, mf32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class T>
+__device__ __forceinline__ int mrow(int l, int r) {
+    return sizeof(T) == 8 ? (l >> 4) + 4 * r : 4 * (l >> 4) + r;
+}
+
+// f(tile, r, row, col) for every result register of the lane
+template <class T, class F>
+__device__ __forceinline__ void mtile_for(F f) {
+    const int l = lane_id();
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) f(2 * ti + tj, r, 16 * ti + mrow<T>(l, r), 16 * tj + (l & 15));
+}
+
+// acc += sgn A B with A column-major and B row-major in LDS (leading dimension LD), sgn = -1
+// when negate
+template <class T, int LD>
+__device__ __forceinline__ void mfma_gemm32(const NUPPN_LDS T *Acm, const NUPPN_LDS T *Brm,
+                                            MTiles<T> &acc, bool negate) {
+    const int l = lane_id(), i = l & 15, k = l >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        const int row = (4 * kk + k) * LD;
+        T a0 = Acm[row + i], a1 = Acm[row + 16 + i];
+        const T b0 = Brm[row + i], b1 = Brm[row + 16 + i];
+        if (negate) {
+            a0 = -a0;
+            a1 = -a1;
+        }
+        acc.t[0] = mfma16(a0, b0, acc.t[0]);
+        acc.t[1] = mfma16(a0, b1, acc.t[1]);
+        acc.t[2] = mfma16(a1, b0, acc.t[2]);
+        acc.t[3] = mfma16(a1, b1, acc.t[3]);
+    }
+}
+
+// a 32 x 32 column-major block in global memory to / from the tile layout
+template <class T>
+__device__ __forceinline__ void mtile_load_glb(const T *Gp, MTiles<T> &M) {
+    const NUPPN_GLB T *G = gptr(Gp);
+    mtile_for<T>([&](int q, int r, int row, int col) { M.t[q][r] = G[col * 32 + row]; });
+}
+template <class T>
+__device__ __forceinline__ void mtile_store_glb(T *Gp, const MTiles<T> &M) {
+    NUPPN_GLB T *G = gptr(Gp);
+    mtile_for<T>([&](int q, int r, int row, int col) { G[col * 32 + row] = M.t[q][r]; });
+}
+template <class T, int LD>
+__device__ __forceinline__ void mtile_store_lds_cm(NUPPN_LDS T *X, const MTiles<T> &M) {
+    mtile_for<T>([&](int q, int r, int row, int col) { X[col * LD + row] = M.t[q][r]; });
+}
+template <class T, int LD>
+__device__ __forceinline__ void mtile_store_lds_rm(NUPPN_LDS T *X, const MTiles<T> &M) {
+    mtile_for<T>([&](int q, int r, int row, int col) { X[row * LD + col] = M.t[q][r]; });
+}
+
 // scaled value of CSC entry e of a block list: molar units, column-scaled
 __device__ __forceinline__ double block_entry(const DevBlock &B, const double *Aval,
                                               const double *cs, int e) {
@@ -392,13 +481,15 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         wave_sync();
     }
     Regs<T, NB> a, cb, dn, bb;
+    MTiles<T> dnt;   // NB = 32: D_{i+1} in the matrix-core tile layout
     glb_load_cm<T, NB>(F.Sinv, a);
     for (int i = 0; i < nblk; ++i) {
         const bool more = i + 1 < nblk;
         if (more) {   // operands of this step's products, independent of the inversion
             glb_load_cm<T, NB>(F.W + i * BS, cb);
             glb_load_cm<T, NB>(F.Bm + i * BS, bb);
-            glb_load_cm<T, NB>(F.Sinv + (i + 1) * BS, dn);
+            if constexpr (NB == 32) mtile_load_glb<T>(F.Sinv + (i + 1) * BS, dnt);
+            else glb_load_cm<T, NB>(F.Sinv + (i + 1) * BS, dn);
         }
         gj_inverse<T, NB>(a, L.colbuf, L.rowbuf);
         glb_store_cm<T, NB>(F.Sinv + i * BS, a);
@@ -411,15 +502,32 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         lds_store_cm<T, NB>(L.X1, a);
         lds_store_rm<T, NB>(L.X2, cb);
         lds_fence();
-        gemm_lds<T, NB>(L.X1, L.X2, a);                                  // W_i = S_i^-1 C_i
-        glb_store_cm<T, NB>(F.W + i * BS, a);
-        lds_fence();
-        lds_store_rm<T, NB>(L.X2, a);
-        lds_store_cm<T, NB>(L.X1, bb);
-        lds_fence();
+        if constexpr (NB == 32) {
+            MTiles<T> w;   // W_i = S_i^-1 C_i
 #pragma unroll
-        for (int t = 0; t < C::R; ++t) a.v[t] = dn.v[t];
-        gemm_lds_sub<T, NB>(L.X1, L.X2, a);                              // D - B_i W_i
+            for (int q = 0; q < 4; ++q) w.t[q] = typename MVec<T>::type{0, 0, 0, 0};
+            mfma_gemm32<T, C::LD>(L.X1, L.X2, w, false);
+            mtile_store_glb<T>(F.W + i * BS, w);
+            lds_fence();
+            mtile_store_lds_rm<T, C::LD>(L.X2, w);
+            lds_store_cm<T, NB>(L.X1, bb);
+            lds_fence();
+            mfma_gemm32<T, C::LD>(L.X1, L.X2, dnt, true);                  // D - B_i W_i
+            lds_fence();
+            mtile_store_lds_cm<T, C::LD>(L.X1, dnt);
+            lds_fence();
+            lds_load_cm<T, NB>(L.X1, a);
+        } else {
+            gemm_lds<T, NB>(L.X1, L.X2, a);                              // W_i = S_i^-1 C_i
+            glb_store_cm<T, NB>(F.W + i * BS, a);
+            lds_fence();
+            lds_store_rm<T, NB>(L.X2, a);
+            lds_store_cm<T, NB>(L.X1, bb);
+            lds_fence();
+#pragma unroll
+            for (int t = 0; t < C::R; ++t) a.v[t] = dn.v[t];
+            gemm_lds_sub<T, NB>(L.X1, L.X2, a);                          // D - B_i W_i
+        }
     }
     wave_sync();
     // border: E, X_F = T^-1 F, G' = G - E X_F and its LU with partial pivoting
