@@ -67,13 +67,14 @@ struct BlockCfg {
     static constexpr int LD = NB + 4;     // LDS leading dimension (16-byte aligned, conflict-free)
 };
 
+// LDS of one wavefront: the two block-product operands X1, X2 of the factorisation loop (X2
+// also holds the Gauss-Jordan pivot columns and rows between the products), which the solve
+// vector vec overlays (the loop and the solves never run at the same time), and the border
+// scratch.  About 18.6 KB with FP64 factors (NB = 32, 1,100 species), so eight wavefronts share
+// a CU's 160 KB.
 template <class T>
-// LDS of one wavefront: the two block-product operands X1, X2 of the factorisation loop, which
-// the solve vector vec overlays (the loop and the solves never run at the same time), the
-// Gauss-Jordan pivot column and row, and the border scratch.  About 19 KB with FP64 factors
-// (NB = 32, 1,100 species), so eight wavefronts share a CU's 160 KB.
 struct BlockLds {
-    NUPPN_LDS T *X1, *X2, *colbuf, *rowbuf, *vec;
+    NUPPN_LDS T *X1, *X2, *vec;
     NUPPN_LDS double *bscr;   // [16] border solve scratch
 };
 
@@ -87,7 +88,7 @@ __host__ __device__ constexpr size_t block_lds_span(int nip) {
 template <class T, int NB>
 __host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd) {
     (void) nbd;   // nbd <= 8: the border entries fit the 8 slots after nip
-    return sizeof(T) * (block_lds_span<T, NB>(nip) + 2 * NB) + 16 + 16 * sizeof(double);
+    return sizeof(T) * block_lds_span<T, NB>(nip) + 16 + 16 * sizeof(double);
 }
 
 template <class T, int NB>
@@ -99,10 +100,6 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
     L.X2 = p + NB * C::LD;
     L.vec = p;
     p += block_lds_span<T, NB>(nip);
-    L.colbuf = p;
-    p += NB;
-    L.rowbuf = p;
-    p += NB;
     NUPPN_LDS char *q = (NUPPN_LDS char *) p;
     q = (NUPPN_LDS char *) (((uintptr_t) q + 15) & ~(uintptr_t) 15);
     L.bscr = (NUPPN_LDS double *) q;
@@ -176,43 +173,69 @@ __device__ __forceinline__ void glb_store_cm(T *Gp, const Regs<T, NB> &a) {
     for (int t = 0; t < C::R; ++t) G[c * NB + h * C::R + t] = a.v[t];
 }
 
-// In-place inverse of the NB x NB block held in lane layout, Gauss-Jordan without pivoting.
+// In-place inverse of the NB x NB block held in lane layout: Gauss-Jordan without pivoting, on
+// 2 x 2 pivot blocks K = {k, k+1} (the leading 2 x 2 minors of these diagonally dominant blocks
+// are far from singular).  One step, with P = a[K][K]:
+//     a[i][j] -= a[i][K] P^-1 a[K][j]  (i, j not in K),  a[i][K] = -a[i][K] P^-1,
+//     a[K][j] = P^-1 a[K][j],  a[K][K] = P^-1,
+// the same as two scalar steps in exact arithmetic, with half the LDS round trips.  buf: 4 NB
+// entries of LDS (the pivot columns and rows).
 template <class T, int NB>
-__device__ __forceinline__ void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *colbuf,
-                                  NUPPN_LDS T *rowbuf) {
+__device__ __forceinline__ void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *buf) {
     using C = BlockCfg<T, NB>;
     const int c = lane_id() % NB, h = lane_id() / NB;
-    constexpr int KU = NB <= 32 ? NB : 2;
+    NUPPN_LDS T *col0 = buf, *col1 = buf + NB, *row0 = buf + 2 * NB, *row1 = buf + 3 * NB;
+    constexpr int KU = NB <= 32 ? NB / 2 : 2;
 #pragma unroll KU
-    for (int k = 0; k < NB; ++k) {
-        const int kh = k / C::R, kt = k % C::R;
-        if (c == k) {
+    for (int k = 0; k < NB; k += 2) {
+        const int kh = k / C::R, kt = k % C::R;   // rows k, k + 1 lie in the same lane half
+        if (c == k || c == k + 1) {
+            NUPPN_LDS T *cb = c == k ? col0 : col1;
 #pragma unroll
-            for (int t = 0; t < C::R; ++t) colbuf[h * C::R + t] = a.v[t];
+            for (int t = 0; t < C::R; ++t) cb[h * C::R + t] = a.v[t];
         }
         if (h == kh) {
-            T rv = a.v[0];
+            T r0 = a.v[0], r1 = a.v[1];
 #pragma unroll
-            for (int t = 1; t < C::R; ++t)
-                if (t == kt) rv = a.v[t];
-            rowbuf[c] = rv;
+            for (int t = 2; t < C::R; t += 2)
+                if (t == kt) {
+                    r0 = a.v[t];
+                    r1 = a.v[t + 1];
+                }
+            row0[c] = r0;
+            row1[c] = r1;
         }
         lds_fence();
-        const T pinv = pivot_rcp(colbuf[k]);
-        const T mkc = rowbuf[c] * pinv;
-        // column k becomes -col/p, the others a - col * (row k / p); row k is set below
-        const bool pc = c == k;
-        const T f = pc ? pinv : mkc;
+        // P^-1 = [[p11, -p01], [-p10, p00]] / det
+        const T p00 = col0[k], p10 = col0[k + 1], p01 = col1[k], p11 = col1[k + 1];
+        const T dinv = pivot_rcp(p00 * p11 - p01 * p10);
+        const T q00 = p11 * dinv, q01 = -p01 * dinv, q10 = -p10 * dinv, q11 = p00 * dinv;
+        const T u0 = row0[c], u1 = row1[c];
+        // (f0, f1): the new rows K of column c; columns outside K also take the rank-2 update
+        const bool inK = c == k || c == k + 1;
+        T f0, f1;
+        if (c == k) {
+            f0 = q00;
+            f1 = q10;
+        } else if (c == k + 1) {
+            f0 = q01;
+            f1 = q11;
+        } else {
+            f0 = fma_t(q00, u0, q01 * u1);
+            f1 = fma_t(q10, u0, q11 * u1);
+        }
 #pragma unroll
         for (int t = 0; t < C::R; ++t) {
-            const T col = colbuf[h * C::R + t];
-            a.v[t] = fma_t(-col, f, pc ? T(0) : a.v[t]);
+            const T x0 = col0[h * C::R + t], x1 = col1[h * C::R + t];
+            a.v[t] = fma_t(-x0, f0, fma_t(-x1, f1, inK ? T(0) : a.v[t]));
         }
         if (h == kh) {
-            const T rv = pc ? pinv : mkc;
 #pragma unroll
-            for (int t = 0; t < C::R; ++t)
-                if (t == kt) a.v[t] = rv;
+            for (int t = 0; t < C::R; t += 2)
+                if (t == kt) {
+                    a.v[t] = f0;
+                    a.v[t + 1] = f1;
+                }
         }
         lds_fence();   // every read of this step precedes the next step's writes
     }
@@ -326,11 +349,6 @@ __device__ __forceinline__ void mtile_load_glb(const T *Gp, MTiles<T> &M) {
     const NUPPN_GLB T *G = gptr(Gp);
     mtile_for<T>([&](int q, int r, int row, int col) { M.t[q][r] = G[col * 32 + row]; });
 }
-template <class T>
-__device__ __forceinline__ void mtile_store_glb(T *Gp, const MTiles<T> &M) {
-    NUPPN_GLB T *G = gptr(Gp);
-    mtile_for<T>([&](int q, int r, int row, int col) { G[col * 32 + row] = M.t[q][r]; });
-}
 template <class T, int LD>
 __device__ __forceinline__ void mtile_store_lds_cm(NUPPN_LDS T *X, const MTiles<T> &M) {
     mtile_for<T>([&](int q, int r, int row, int col) { X[col * LD + row] = M.t[q][r]; });
@@ -369,12 +387,13 @@ __device__ __forceinline__ T matvec_regs(const T (&m)[NB / BlockCfg<T, NB>::H],
     return acc;
 }
 
-// v <- T^-1 v for the interior block-tridiagonal part (v in LDS, nip entries).  The block
-// matrices do not depend on v, so each sweep loads the next block's matrices before it works on
-// the current block: the global-memory latency overlaps the dependent chain of mat-vecs.
-template <class T, int NB>
+// v_q <- T^-1 v_q (q < NV) for the interior block-tridiagonal part (v_q in LDS at v + q vs, nip
+// entries each).  The block matrices do not depend on v, so each sweep loads the next block's
+// matrices before it works on the current block: the global-memory latency overlaps the
+// dependent chain of mat-vecs; with NV = 2 each block's matrices serve both vectors.
+template <class T, int NB, int NV = 1>
 __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFactors<T> &F,
-                                    NUPPN_LDS T *v) {
+                                             NUPPN_LDS T *v, int vs = 0) {
     constexpr int CPL = NB / BlockCfg<T, NB>::H;
     const int rr = lane_id() % NB, h = lane_id() / NB;
     constexpr size_t BS = (size_t) NB * NB;
@@ -388,16 +407,23 @@ __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFacto
             load_mv<T, NB>(F.Bm + (size_t) i * BS, nB);
             load_mv<T, NB>(F.Sinv + (size_t) (i + 1) * BS, nS);
         }
-        NUPPN_LDS T *vi = v + i * NB;
         if (i > 0) {
-            const T acc = matvec_regs<T, NB>(mB, v + (i - 1) * NB);
+            T acc[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) acc[q] = matvec_regs<T, NB>(mB, v + q * vs + (i - 1) * NB);
             lds_fence();
-            if (h == 0) vi[rr] = vi[rr] - acc;
+            if (h == 0)
+#pragma unroll
+                for (int q = 0; q < NV; ++q) v[q * vs + i * NB + rr] = v[q * vs + i * NB + rr] - acc[q];
             lds_fence();
         }
-        const T acc = matvec_regs<T, NB>(mS, vi);
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = matvec_regs<T, NB>(mS, v + q * vs + i * NB);
         lds_fence();
-        if (h == 0) vi[rr] = acc;
+        if (h == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[q * vs + i * NB + rr] = acc[q];
         lds_fence();
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
@@ -409,10 +435,13 @@ __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFacto
     load_mv<T, NB>(F.W + (size_t) (nblk - 2) * BS, mS);
     for (int i = nblk - 2; i >= 0; --i) {
         if (i > 0) load_mv<T, NB>(F.W + (size_t) (i - 1) * BS, nS);
-        NUPPN_LDS T *vi = v + i * NB;
-        const T acc = matvec_regs<T, NB>(mS, v + (i + 1) * NB);
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = matvec_regs<T, NB>(mS, v + q * vs + (i + 1) * NB);
         lds_fence();
-        if (h == 0) vi[rr] = vi[rr] - acc;
+        if (h == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[q * vs + i * NB + rr] = v[q * vs + i * NB + rr] - acc[q];
         lds_fence();
 #pragma unroll
         for (int j = 0; j < CPL; ++j) mS[j] = nS[j];
@@ -491,7 +520,7 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
             if constexpr (NB == 32) mtile_load_glb<T>(F.Sinv + (i + 1) * BS, dnt);
             else glb_load_cm<T, NB>(F.Sinv + (i + 1) * BS, dn);
         }
-        gj_inverse<T, NB>(a, L.colbuf, L.rowbuf);
+        gj_inverse<T, NB>(a, L.X2);   // X2 is free between the products
         glb_store_cm<T, NB>(F.Sinv + i * BS, a);
         bool fin = true;
 #pragma unroll
@@ -507,9 +536,15 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
 #pragma unroll
             for (int q = 0; q < 4; ++q) w.t[q] = typename MVec<T>::type{0, 0, 0, 0};
             mfma_gemm32<T, C::LD>(L.X1, L.X2, w, false);
-            mtile_store_glb<T>(F.W + i * BS, w);
+            // W_i to global column-major through LDS (the tile layout scatters a lane's rows),
+            // and row-major into X2 for the second product
             lds_fence();
+            mtile_store_lds_cm<T, C::LD>(L.X1, w);
             mtile_store_lds_rm<T, C::LD>(L.X2, w);
+            lds_fence();
+            lds_load_cm<T, NB>(L.X1, a);
+            glb_store_cm<T, NB>(F.W + i * BS, a);
+            lds_fence();
             lds_store_cm<T, NB>(L.X1, bb);
             lds_fence();
             mfma_gemm32<T, C::LD>(L.X1, L.X2, dnt, true);                  // D - B_i W_i
@@ -550,13 +585,24 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         for (int e = eptr[LG] + lane; e < eptr[LG + 1]; e += WAVE)
             Gg[eloc[e]] = block_entry(B, Aval, F.cs, e);
         wave_sync();
-        for (int b = 0; b < nbd; ++b) {
-            NUPPN_GLB T *xf = XFg + (size_t) b * nip;
-            for (int j = lane; j < nip; j += WAVE) L.vec[j] = xf[j];
+        // X_F = T^-1 F, two columns per sweep where two vectors fit the LDS span
+        const int vs = nip + 8;
+        const bool two = 2 * (size_t) vs <= block_lds_span<T, NB>(nip);
+        for (int b = 0; b < nbd;) {
+            const int nv = (two && b + 1 < nbd) ? 2 : 1;
+            for (int q = 0; q < nv; ++q) {
+                NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
+                for (int j = lane; j < nip; j += WAVE) L.vec[q * vs + j] = xf[j];
+            }
             lds_fence();
-            thomas_solve<T, NB>(B, F, L.vec);
-            for (int j = lane; j < nip; j += WAVE) xf[j] = L.vec[j];
+            if (nv == 2) thomas_solve<T, NB, 2>(B, F, L.vec, vs);
+            else thomas_solve<T, NB>(B, F, L.vec);
+            for (int q = 0; q < nv; ++q) {
+                NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
+                for (int j = lane; j < nip; j += WAVE) xf[j] = L.vec[q * vs + j];
+            }
             lds_fence();
+            b += nv;
         }
         wave_sync();
         for (int b2 = 0; b2 < nbd; ++b2)
