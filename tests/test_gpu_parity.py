@@ -43,6 +43,16 @@ def c1(pkg, oracle, hip):
     ctx.close()
 
 
+@pytest.fixture(scope="module")
+def c3(pkg, oracle, hip):
+    """The 5,000-species network (34,055 reactions, n = 4,999 reduced species)."""
+    w = pkg.network.config(3, nz=4)
+    ctx = hip.Context(w.net)
+    o = oracle.Oracle(w.net, oracle.default_knobs(cyminformal=-1.0))
+    yield w, ctx, o
+    ctx.close()
+
+
 def _rate_bound(coef, t9):
     t = max(t9, 0.01)
     terms = np.abs(coef * np.array([1, 1 / t, t ** (-1 / 3), t ** (1 / 3), t, t ** (5 / 3),
@@ -80,7 +90,7 @@ def test_rates_parity(c1, hip, pkg):
     cg.close()
 
 
-@pytest.mark.parametrize("which", ["c0", "c1"])
+@pytest.mark.parametrize("which", ["c0", "c1", "c3"])
 def test_dxdt_and_jacobian_bitwise(which, request, oracle):
     w, ctx, o = request.getfixturevalue(which)
     rng = np.random.default_rng(11)
@@ -102,13 +112,15 @@ def test_dxdt_and_jacobian_bitwise(which, request, oracle):
             np.testing.assert_array_equal(jv, cv)
 
 
-def test_lu_pivots_and_solution(c1, oracle):
+@pytest.mark.parametrize("which,nzones", [("c1", 6), ("c3", 2)])
+def test_lu_pivots_and_solution(which, nzones, request, oracle):
     """Device GEPP on I - dt J of stiff and mild zones: row pivots equal the oracle's and the
-    reference SuperLU's, nnz(L)/nnz(U) equal the oracle's, solution bit-equal to the oracle's."""
-    w, ctx, o = c1
+    reference SuperLU's, nnz(L)/nnz(U) equal the oracle's, solution bit-equal to the oracle's.
+    On the 5,000-species network COLAMD leaves ~2.6 M entries in L + U."""
+    w, ctx, o = request.getfixturevalue(which)
     perm_c = ctx.column_order()
     ref = oracle.ref_available()
-    for z in range(6):
+    for z in range(nzones):
         v = o.rates(w.t9[z, 0], w.rho[z, 0])
         n, rp, ci, va, mask = o.reduced_matrix(w.dt[z], w.x0[z], v)
         cp, ri, cv = oracle.csr_to_csc(n, rp, ci, va)
@@ -268,3 +280,18 @@ def test_c_fortran_dgssv_dropin(hip, oracle, c1):
         assert np.all(b[n:ldb] == 0)
         info, h3 = hip.c_fortran_dgssv(3, n, int(cp[n]), 2, vals, rif, cpf, b, ldb, h)
         assert info == 0 and h3 == 0
+
+
+def test_c3_batched_path_unavailable(c3, hip):
+    """The 5,000-species network's interior coupling (alpha captures two Z rows of up to 51
+    species apart: band 102) exceeds the 64-species blocks, so the batched path reports itself
+    unavailable and nuppn_integrate_batch_mixed refuses with NUPPN_ERR_UNSUPPORTED, leaving the
+    abundances untouched; the plain path serves the network."""
+    w, ctx, o = c3
+    assert not ctx.block_info()["available"]
+    sel = np.array([0])
+    y0 = w.x0[sel].copy()
+    with pytest.raises(RuntimeError, match=str(hip.ERR_UNSUPPORTED)):
+        ctx.integrate_batch_mixed(y0, *[a[sel] for a in (w.t9[:, 0], w.t9[:, 1], w.rho[:, 0],
+                                                            w.rho[:, 1], w.dt)])
+    np.testing.assert_array_equal(y0, w.x0[sel])
