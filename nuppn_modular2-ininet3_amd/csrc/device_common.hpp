@@ -73,6 +73,32 @@ __device__ inline double seq_sum(P a, int n) {
     return s;
 }
 
+// The same sum over an LDS buffer: every lane folds the values itself from broadcast reads (the
+// reads of a group are independent and issue together; only the adds form the chain), which
+// avoids the cross-lane moves of the register form above.
+__device__ inline double seq_sum(NUPPN_LDS double *a, int n) {
+    double s = 0.0;
+    int i = 0;
+    for (; i + 16 <= n; i += 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = a[i + k];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s = s + v[k];
+    }
+    for (; i < n; ++i) s = s + a[i];
+    return s;
+}
+
+// seq_sum of a global array through an LDS buffer of >= n doubles
+__device__ inline double seq_sum_staged(const double *a, int n, NUPPN_LDS double *buf) {
+    for (int i = lane_id(); i < n; i += WAVE) buf[i] = a[i];
+    wave_sync();
+    const double s = seq_sum(buf, n);
+    wave_sync();
+    return s;
+}
+
 // real ** integer as gfortran evaluates it (libgcc __powidf2: square-and-multiply)
 __device__ __forceinline__ double powi(double x, int m) {
     unsigned int n = m < 0 ? -(unsigned int) m : (unsigned int) m;

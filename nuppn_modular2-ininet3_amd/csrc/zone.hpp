@@ -187,7 +187,7 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
     double *yps = zrow(Z.yps, Z.nsp_stride, z);
     if (loop_done) {
         // DEUFL renormalises by the plain sum (solver.F90:229-235), then the final clip
-        const double sm = a.K.integration_method == 1 ? seq_sum(yps, N.nsp) : 1.0;
+        const double sm = a.K.integration_method == 1 ? seq_sum_staged(yps, N.nsp, sb) : 1.0;
         for (int j = lane_id(); j < N.nsp; j += WAVE) {
             const double y = a.K.integration_method == 1 ? yps[j] / sm : yps[j];
             yps[j] = fmax(y, 1.0e-99);
@@ -251,7 +251,7 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
         s.st.iter++;
         s.st.n_newton++;
         const double mneg = wave_maxneg(y_new, nsp);
-        const double sm = seq_sum(y_new, nsp);
+        const double sm = seq_sum_staged(y_new, nsp, sb);
         if (s.grd <= dgrd && mneg < sum_check && fabs(1.0 - sm) <= SUM_TOL) {
             copy_vec(yps, y_new, nsp);
             s.dtrcmd = 2.0 * fmax(fmin(sqrt(dgrd / fmax(s.grd, SMALL_C)), rmax), rmin) * s.dttry;
@@ -351,7 +351,7 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
             s.solved = 1;
             if (0 > K.ittd) {
                 const double mneg = wave_maxneg(y_new, nsp);
-                const double sm = seq_sum(y_new, nsp);
+                const double sm = seq_sum_staged(y_new, nsp, sb);
                 double err, tol;
                 if (mneg >= sum_check) { err = mneg; tol = sum_check; ierr = 4; }
                 else if (fabs(1.0 - sm) > SUM_TOL) { err = fabs(1.0 - sm); tol = SUM_TOL; ierr = 6; }
