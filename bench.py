@@ -115,6 +115,16 @@ def _cpu_init():
     _CPU["o"] = oracle_lib.Oracle(w.net, oracle_lib.default_knobs(cyminformal=-1.0), ls)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _cpu_zone(z):
     w, o = _CPU["w"], _CPU["o"]
     ierr, y, ye, st = o.integrate_zone(w.x0[z], w.t9[z, 0], w.t9[z, 1], w.rho[z, 0], w.rho[z, 1],
@@ -222,11 +232,16 @@ def main():
         log(f"plain sample: {elp * 1e3:.1f} ms vs mixed {elb * 1e3:.1f} ms on {len(sel)} zones")
 
     cpu = None
+    cpu_one = None
     if not a.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         idx = np.linspace(0, nz - 1, min(a.cpu_sample, nz)).round().astype(int)
         cpu = cpu_baseline(idx, cores)
+        cpu["sample"] += f"; host CPU: {_cpu_model()}"
         log(f"cpu baseline: {cpu['value']:.2f} zone-timesteps/s on {cores} cores")
+        # the same sample's first quarter on one core
+        cpu_one = cpu_baseline(idx[::4], 1)
+        log(f"cpu baseline, one core: {cpu_one['value']:.2f} zone-timesteps/s on {len(idx[::4])} zones")
 
     if a.save:
         np.savez(a.save, y_mixed=y_mixed, y_fp64=y64, nsubt_mixed=st_mixed["nsubt"],
@@ -276,6 +291,8 @@ def main():
             "newton_ms": tm["newton_ms"], "control_ms": tm["control_ms"],
             "passes": tm["passes"], "algorithmic_bytes_per_step": bytes_step,
             "plain_sample": plain,
+            "cpu_one_core": None if cpu_one is None else {
+                "value": cpu_one["value"], "unit": cpu_one["unit"], "sample": cpu_one["sample"]},
         },
     }
     print(json.dumps(out))
