@@ -13,10 +13,14 @@ difference of pow() between device and host libm moves dtrcmd, not the decisions
 path (pivot-free block factors, FP64 refinement) against the plain path within the FP64
 rounding spread of the solve, as the backward-Euler batched tests.
 """
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.fixture(scope="module")
@@ -60,6 +64,33 @@ def test_bd_trajectory_c0(pkg, oracle, hip):
             assert e <= 1e-10
             # DEUFL renormalises by the plain sum (solver.F90:234)
             assert abs(y_g.sum() - 1.0) <= 1e-14
+    finally:
+        ctx.close()
+
+
+def test_bd_stiff_zone_c4_golden(pkg, hip):
+    """The very stiff regime of the 4,096-zone Bader-Deuflhard workload (T9 1..4, rho 1e5..1e9
+    g/cc, dt 1e-3..1e2 s): zone 7 (T9 1.65, 43 sub-steps, 176 factorisations) from a fresh
+    context, each side with its own mid-point rates, against the oracle's stored result
+    (tests/golden/make_bd_stiff_golden.py; the oracle needs about 90 s for it): identical error
+    code, sub-steps, stage and step counts, and <= 1e-10 relative on every X > 1e-20."""
+    g = np.load(os.path.join(GOLD, "bd_stiff_c4_zone7.npz"))
+    w = pkg.network.config(4, nz=int(g["nz"]))
+    z = int(g["zone"])
+    ctx = hip.Context(w.net, hip.default_knobs(integration_method=1))
+    try:
+        ie, y, ye, st = ctx.integrate_network(w.x0[z].copy(), w.t9[z, 0], w.t9[z, 1], w.rho[z, 0],
+                                              w.rho[z, 1], w.dt[z])
+        e = _maxrel(y, g["y"])
+        print(f"zone {z}: ierr {ie}/{int(g['ierr'])} nsubt {st.nsubt}/{int(g['nsubt'])} iter "
+              f"{st.iter}/{int(g['iter'])} succ/fail {st.nsucc},{st.nfail}/{int(g['nsucc'])},"
+              f"{int(g['nfail'])} factor {st.n_factor}/{int(g['n_factor'])} relerr {e:.2e}",
+              flush=True)
+        assert ie == int(g["ierr"]) == 0
+        assert (st.nsubt, st.iter, st.nsucc, st.nfail, st.n_factor, st.n_solve) == \
+            tuple(int(g[k]) for k in ("nsubt", "iter", "nsucc", "nfail", "n_factor", "n_solve"))
+        assert e <= 1e-10
+        assert abs(y.sum() - 1.0) <= 1e-14
     finally:
         ctx.close()
 
