@@ -272,7 +272,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
                      "kernel": "zone_newton_block_kernel<32, 1 or 2> (one launch per pass)",
-                     "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s},
+                     "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s,
+                     # time-weighted: the same algorithmic bytes over the whole timed step
+                     "pipeline_frac": bytes_step / (elapsed / a.steps) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "detail": {
             "fp64_factor_path_zone_timesteps_per_s": nz / el64,
