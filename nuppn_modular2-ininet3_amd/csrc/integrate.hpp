@@ -96,16 +96,13 @@ __device__ inline double wave_maxneg(const double *y, int n) {
     return wave_max(m);
 }
 
-// ypsplus (utils.F90:726-740)
-__device__ inline void ypsplus(double *y, int n) {
-    for (int i = lane_id(); i < n; i += WAVE) {
-        double t = y[i];
-        if (t < -1.0e-27) t = 1.0e-99;
-        if (t < 1.0e-27) t = 1.0e-99;
-        y[i] = t;
-    }
-    wave_sync();
+// ypsplus (utils.F90:726-740) of one value
+__device__ __forceinline__ double ypsplus_value(double t) {
+    if (t < -1.0e-27) t = 1.0e-99;
+    if (t < 1.0e-27) t = 1.0e-99;
+    return t;
 }
+
 
 // rates of one state into v (one wavefront)
 __device__ inline void zone_rates(const DevNet &N, double t9, double rho, double *v) {

@@ -250,16 +250,27 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
         s.st.titers++;
         s.st.iter++;
         s.st.n_newton++;
-        const double mneg = wave_maxneg(y_new, nsp);
-        const double sm = seq_sum_staged(y_new, nsp, sb);
+        // y_new is read from HBM once, into the LDS sort buffer: the negative-abundance check
+        // on the way, then the mass sum and the copy below from LDS
+        double mloc = -__builtin_huge_val();
+        for (int i = lane_id(); i < nsp; i += WAVE) {
+            const double y = y_new[i];
+            sb[i] = y;
+            mloc = fmax(mloc, -y);
+        }
+        const double mneg = wave_max(mloc);
+        wave_sync();
+        const double sm = seq_sum(sb, nsp);
         if (s.grd <= dgrd && mneg < sum_check && fabs(1.0 - sm) <= SUM_TOL) {
-            copy_vec(yps, y_new, nsp);
+            // yps = y_new, then ypsplus
+            for (int i = lane_id(); i < nsp; i += WAVE) yps[i] = ypsplus_value(sb[i]);
+            wave_sync();
             s.dtrcmd = 2.0 * fmax(fmin(sqrt(dgrd / fmax(s.grd, SMALL_C)), rmax), rmin) * s.dttry;
-            ypsplus(yps, nsp);
             ierr = 0;
             step_ended = true;
         } else {
-            copy_vec(y_cur, y_new, nsp);
+            for (int i = lane_id(); i < nsp; i += WAVE) y_cur[i] = sb[i];
+            wave_sync();
             if (s.iter > K.ittd) {
                 double err, tol;
                 if (mneg >= sum_check) { err = mneg; tol = sum_check; ierr = 4; }
