@@ -293,9 +293,13 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
             if (s.dttry < SMALL_C) { zone_finish(a, z, s, 11, false, sb); break; }
             if (ierr == 0) {
                 s.dtdone = s.dtdone + s.dttry;
-                for (int j = lane_id(); j < nsp; j += WAVE) yps[j] = fmax(yps[j], 1.0e-99);
+                // yps = max(yps, 1e-99), then yps_old = yps (one read of yps)
+                for (int j = lane_id(); j < nsp; j += WAVE) {
+                    const double y = fmax(yps[j], 1.0e-99);
+                    yps[j] = y;
+                    yps_old[j] = y;
+                }
                 wave_sync();
-                copy_vec(yps_old, yps, nsp);
                 s.st.nsucc++;
                 if (s.dtdone >= dtwant) { s.st.nsubt = s.i; zone_finish(a, z, s, 0, true, sb); break; }
             } else {
@@ -352,9 +356,13 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
         }
         if (my_t9 >= K.min_t9_for_slv) {
             // backward_euler_newton set-up (backward_euler.F90:49-53)
-            copy_vec(y_old, yps, nsp);
-            copy_vec(y_cur, yps, nsp);
-            copy_vec(y_new, yps, nsp);
+            for (int j = lane_id(); j < nsp; j += WAVE) {
+                const double y = yps[j];
+                y_old[j] = y;
+                y_cur[j] = y;
+                y_new[j] = y;
+            }
+            wave_sync();
             s.iter = 0;
             s.grd = 0.0;
             s.st.titers = 0;
