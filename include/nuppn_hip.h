@@ -28,8 +28,9 @@ extern "C" {
 /* library errors */
 #define NUPPN_ERR_ARG          (-1)   /* bad argument */
 #define NUPPN_ERR_HIP          (-2)   /* HIP runtime error / no device / library not loadable */
-#define NUPPN_ERR_PATTERN      (-3)   /* a matrix entry became exactly 0 or -1: the reference would
-                                         drop it and re-order; not reproduced by the static path */
+#define NUPPN_ERR_PATTERN      (-3)   /* reserved (not returned): entries that reduce_jacobian
+                                         drops (exactly 0 or -1) are dropped, and a zero where
+                                         the reference stops ends the zone with NUPPN_BAD_STEP */
 #define NUPPN_ERR_SINGULAR     (-4)   /* exactly zero pivot (slu_solve returns info > 0 there) */
 #define NUPPN_ERR_UNSUPPORTED  (-5)   /* configuration outside the static-network path */
 
@@ -150,21 +151,23 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
 
 /* Options of the batched path (nuppn_integrate_batch_mixed): the reduced system is factored
  * without pivoting in a block-tridiagonal + border form (molar units, columns scaled by the
- * diagonal), in FP32 (factor_precision 32) or FP64 (64).  Every solve is refined in FP64 against
- * the FP64 matrix, x_{k+1} = x_k + (LU)^-1 (rhs - A x_k), and stops when
+ * diagonal), in FP32 (factor_precision 32) or FP64 (64).  Every solve is refined against the
+ * FP64 matrix, x_{k+1} = x_k + (LU)^-1 (rhs - A x_k), the residual rhs - A x carried in twice the
+ * FP64 precision and rounded once, so the refinement converges to the solution of the FP64
+ * system itself whatever the factor precision.  It stops when
  *   - each correction satisfies |d_i| <= refine_tol * (|x_i| + |y_i| + 1e-30), y the current
  *     abundances, or
- *   - x is backward stable: max_i |rhs - A x|_i / (|A| |x| + |rhs| + 1e-30)_i <= accept_tol (x
- *     solves a system within accept_tol of A and rhs, entry by entry; rows whose magnitude is
- *     below the 1e-30 abundance floor do not count).
+ *   - (accept_tol > 0 only) x is backward stable: max_i |rhs - A x|_i / (|A| |x| + |rhs| +
+ *     1e-30)_i <= accept_tol.  Off by default: a backward-stable x can still be far from the
+ *     solution in ill-conditioned components, differently for each factor precision.
  * When the corrections stop contracting, or max_refine steps pass, with FP32 factors, the zone
  * switches to FP64 factors for the rest of the call and the iteration is redone; with FP64
  * factors the last iterate is kept. */
 typedef struct {
     int factor_precision;   /* 32 (default) or 64 */
-    int max_refine;         /* refinement steps per solve (8) */
-    double refine_tol;      /* 1e-13 */
-    double accept_tol;      /* 1e-12 */
+    int max_refine;         /* refinement steps per solve (12) */
+    double refine_tol;      /* 1e-17: below half an ulp of every new abundance */
+    double accept_tol;      /* 0 (off) */
 } nuppn_batch_opts;
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o);

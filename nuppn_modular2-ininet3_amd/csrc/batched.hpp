@@ -6,20 +6,20 @@
 // zone_control run integrate_network's sub-step and Newton decisions in FP64), so both paths
 // take their decisions with one implementation; only the linear solve differs:
 //     x_0 = 0, r_0 = rhs;  d_k = (LU)^-1 r_k (factors in precision T),  x_{k+1} = x_k + d_k,
-//     r_{k+1} = rhs - A x_{k+1} (FP64, the assembled matrix)
+//     r_{k+1} = rhs - A x_{k+1} (the FP64 assembled matrix; products and sums in twice the FP64
+//     precision, rounded once: block_residual)
 // stopping when every correction is small against its own species, |d_i| <= refine_tol *
 // (|x_i| + |y_i| + 1e-30) with y the current abundances (a normwise test would leave the
 // abundances of rare species, 20 orders of magnitude below the dominant ones, unconverged; the
-// 1e-30 floor ignores species far below any reported abundance), or when x is backward stable
-// entry by entry, max_i |r_i| / (|A| |x| + |rhs|)_i <= accept_tol (Oettli-Prager; the
-// correction test alone cannot stop on components whose condition number is near 1/eps(FP64),
-// where the FP64 residual's own rounding keeps moving them).  The same stopping rule holds for
-// both factor precisions, so FP32-factor and FP64-factor zones converge to the same solution.
+// 1e-30 floor ignores species far below any reported abundance).  With the extra-precise
+// residual the iteration converges to the solution of the FP64 system itself, whatever the
+// precision of the factors, as long as they contract it: FP32-factor and FP64-factor zones then
+// take the same Newton iterates to within a few units of the last place.  (An FP64 residual
+// would stop it at cond(A) eps(FP64) of the solution, different for every factorisation.)
 // When the corrections stop shrinking (a contraction worse than 1/2 per step) or the step limit
 // passes with FP32 factors, cond(A) * eps(FP32) is not small: the zone is marked for FP64
-// factors (for the rest of the call) and left in the Newton phase, and the FP64 kernel,
-// launched after the FP32 one in the same pass, redoes the iteration.  With FP64 factors the
-// last iterate is kept.
+// factors (for the rest of the call) and the same wavefront redoes the iteration with them.
+// With FP64 factors the last iterate is kept.
 #pragma once
 #include "bd.hpp"
 #include "block_lu.hpp"
@@ -31,13 +31,14 @@ struct BatchOpts {
     int fp64_factors;   // 1: FP64 factors from the start (no FP32 stage)
     int max_refine;     // refinement steps per solve
     double refine_tol;  // stop when every |d_i| <= refine_tol * (|x_i| + |y_i| + 1e-30)
-    double accept_tol;  // stop when max_i |r_i| / (|A| |x| + |rhs|)_i <= accept_tol
+    double accept_tol;  // > 0: stop also when max_i |r_i| / (|A| |x| + |rhs|)_i <= accept_tol
 };
 
 // per-slot scratch of the block Newton kernels (one slot per resident wavefront)
 // (B_JAC .. B_SORT: the Bader-Deuflhard step's J, tableau and work vectors)
+// (B_LO: the low parts of the extra-precise residual; B_XLO: those of the solution)
 enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_JAC, B_YK, B_YEXT, B_YSCAL,
-                B_DELTA, B_QCOL, B_YERR, B_SORT, B_COUNT };
+                B_DELTA, B_QCOL, B_YERR, B_SORT, B_LO, B_XLO, B_COUNT };
 
 // the factors, in the same slot after the arrays above
 enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
@@ -56,7 +57,7 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
                                  8ull * n, 8ull * n, 8ull * n, 8ull * nnzA,
                                  8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
-                                 64ull * nsp, 8ull * pow2};
+                                 64ull * nsp, 8ull * pow2, 8ull * n, 8ull * n};
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                   4ull * nbd + 4};
@@ -112,25 +113,30 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
     return fail;
 }
 
-// Refined solve of A x = rhs with the precision-T factors in fac.  y: current abundances
-// (species order) for the correction test.  Returns 0 (x holds the solution), 1 (FP32 factors
-// cannot deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64
-// factors: the reference's singular-matrix outcome).
+// Refined solve of A x = rhs with the precision-T factors (fac: the zone's slot).  y: current
+// abundances (species order).  The solution is kept as x + xlo, an unevaluated sum carried in
+// twice the FP64 precision, so that y + x is accurate relative to itself even for a species
+// that the step burns down by many orders of magnitude (y + x << |x|).  The corrections stop
+// when every |d_i| <= refine_tol (|y_i + x_i| + kRefineFloor): each new abundance has converged
+// relative to its own size.  Returns 0 (x, xlo hold the solution), 1 (FP32 factors cannot
+// deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64 factors: the
+// reference's singular-matrix outcome).
 template <class T, int NB>
 __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
-                            const double *rhs, const double *y, double *x, double *res,
-                            double *d, char *fac, NUPPN_LDS char *lds,
+                            const double *rhs, const double *y, double *x, double *xlo,
+                            double *res, double *d, char *fac, NUPPN_LDS char *lds,
                             ZoneStats &st) {
     constexpr bool fp64 = sizeof(T) == 8;
     const DevBlock &B = b.B;
     const int n = B.n;
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
-    NUPPN_GLB double *xg = gptr(x), *resg = gptr(res);
+    NUPPN_GLB double *xg = gptr(x), *xl = gptr(xlo), *resg = gptr(res);
     const NUPPN_GLB double *dg = gptr((const double *) d), *yg = gptr(y), *rhsg = gptr(rhs);
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     for (int i = lane_id(); i < n; i += WAVE) {
         xg[i] = 0.0;
+        xl[i] = 0.0;
         resg[i] = rhsg[i];
     }
     wave_sync();
@@ -141,23 +147,29 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
         double eta = 0.0;
         bool fin = true;
         for (int i = lane_id(); i < n; i += WAVE) {
-            const double di = dg[i];
-            const double xi = xg[i] + di;
-            xg[i] = xi;
-            const double scale = fabs(xi) + fabs(yg[red2sp[i]]) + kAbundanceFloor;
+            const double di = dg[i], xh = xg[i];
+            // (xh + xlo) + d: TwoSum, the rounding error into the low part
+            const double sh = xh + di;
+            const double bb = sh - xh;
+            const double lo = xl[i] + ((xh - (sh - bb)) + (di - bb));
+            const double hi = sh + lo;
+            xg[i] = hi;
+            xl[i] = lo - (hi - sh);
+            const double scale = fabs(yg[red2sp[i]] + hi) + kRefineFloor;
             eta = fmax(eta, fabs(di) / scale);
-            fin = fin && isfinite(xi);
+            fin = fin && isfinite(hi);
         }
         eta = wave_max(eta);
         wave_sync();
         if (ballot(!fin)) return fp64 ? -4 : 1;
         if (eta <= b.O.refine_tol) return 0;   // converged corrections
-        // x is accepted once it is backward stable in FP64: it solves a system within
-        // accept_tol, entry by entry, of A and rhs
-        // (d, the correction just applied, serves as the residual's scratch)
-        const double omega = block_residual(B, Aval, rhs, x, res, d);
+        // the next residual, extra-precise (d, the correction just applied, serves as its
+        // scratch); with accept_tol > 0, x is also accepted once it is backward stable: it
+        // solves a system within accept_tol, entry by entry, of A and rhs
+        const double omega = block_residual(B, Aval, rhs, x, xlo, res, d,
+                                            (double *) (fac + b.off[B_LO]));
         st.n_refine++;
-        if (omega <= b.O.accept_tol) return 0;
+        if (b.O.accept_tol > 0.0 && omega <= b.O.accept_tol) return 0;
         // otherwise the corrections must keep contracting (the first correction's eta is
         // about 1 by construction, so contraction is judged from the second one on).  A
         // stall or the step limit with FP32 factors means cond(A) * eps(FP32) is not small
@@ -173,54 +185,103 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
 // factorisation).  Same return codes as block_refine.
 template <class T, int NB>
 __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
-                                   const double *rhs, const double *y, double *x, double *res,
-                                   double *d, char *fac, NUPPN_LDS char *lds,
+                                   const double *rhs, const double *y, double *x, double *xlo,
+                                   double *res, double *d, char *fac, NUPPN_LDS char *lds,
                                    ZoneStats &st) {
     if (block_factor_zone<T, NB>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
-    return block_refine<T, NB>(N, b, Aval, rhs, y, x, res, d, fac, lds, st);
+    return block_refine<T, NB>(N, b, Aval, rhs, y, x, xlo, res, d, fac, lds, st);
 }
 
-// the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order) and the
-// Newton rhs y_old - y_cur + dt dX/dt as chunked sums over precomputed coefficient lists
-__device__ __forceinline__ void block_assemble(const DevNet &N, const DevBlock &B, double dt,
+// dX/dt of the reduced species from the per-reaction fluxes vx, each species' sum carried in
+// twice the FP64 precision (products split exactly, Dot2 sums) and left as hi (out) + lo (lo).
+// Every reaction moves integer multiples of its flux between species whose mass numbers weight
+// them to zero, so these sums conserve mass to the rounding of each species' net rate, where
+// an FP64 sum loses the rounding of its gross fluxes (forward and reverse rates of nuclei near
+// equilibrium exceed the net by many orders): the mass drift of a hot zone over a cycle falls
+// from ~1e-11 to the last bits.
+__device__ __forceinline__ void block_dxdt_dd(const DevBlock &B, const double *vxp, double *outp,
+                                              double *lop) {
+    const DevSeg &D = B.dxdt;
+    const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vxp);
+    NUPPN_GLB double *o = gptr(outp), *lo = gptr(lop);
+    seg_stream<SumDD2>(
+        D,
+        [&](int e, int idx) {
+            const double c = coef[e], f = vxg[idx];
+            const double t = c * f;
+            return SumDD2{t, fma(c, f, -t)};
+        },
+        [&](int r, const SumDD2 &t) {
+            o[r] = t.hi;
+            lo[r] = t.lo;
+        });
+}
+
+// The Jacobian entries (CSR order) from the per-reaction stems: every entry the sum of its
+// contributions cnt stem a_row / a_col as real values - the coefficient a double-double from the
+// host, its product with the stem split exactly, the sum in double-double (Dot2) - plus the
+// 1e-99 seed of calculate_jacobian (jac_rhs.F90:309), rounded once.  jlo: [nnzA] scratch.
+__device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const double *stemp,
+                                                  double *jacp, double *jlop) {
+    const DevSeg &J = B.jac;
+    const NUPPN_GLB double *coef = gptr(J.e_coef), *coefl = gptr(J.e_coef_lo), *st = gptr(stemp);
+    NUPPN_GLB double *jac = gptr(jacp), *jlo = gptr(jlop);
+    seg_stream<SumDD2>(
+        J,
+        [&](int e, int idx) {
+            const double s = st[idx], ch = coef[e];
+            const double t = s * ch;
+            return SumDD2{t, fma(s, ch, -t) + s * coefl[e]};
+        },
+        [&](int k, const SumDD2 &t) {
+            jac[k] = t.hi;
+            jlo[k] = t.lo;
+        });
+    for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+        const double hi = jac[k], seed = 1.0e-99;
+        const double s = hi + seed, bb = s - hi;
+        jac[k] = s + (((hi - (s - bb)) + (seed - bb)) + jlo[k]);
+    }
+    wave_sync();
+}
+
+// the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order, with
+// reduce_jacobian's drops) and the Newton rhs y_old - y_cur + dt dX/dt as streamed sums over
+// precomputed coefficient lists (J, dX/dt and the rhs each rounded once from double-double
+// sums).  lo: [n], jlo: [nnzA] scratch.  Returns 1 where the reference stops (reduced_value).
+__device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
-                                      double *rhs) {
+                                      double *rhs, double *lop, double *jlo) {
     molar(N, y_cur, xn);
     wave_sync();
     reaction_terms(N, xn, v, vx, stem, stem + N.nre);
     wave_sync();
     // the streamed sums store raw totals; the scalings follow in coalesced passes
-    const DevSeg &J = B.jac;
     const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
     NUPPN_GLB double *Ag = gptr(Aval);
-    {
-        const NUPPN_GLB double *coef = gptr(J.e_coef), *st = gptr(stem);
-        seg_stream<Sum1>(
-            J, [&](int e, int idx) { return Sum1{coef[e] * st[idx]}; },
-            [&](int k, const Sum1 &t) { Ag[k] = t.s; });
-    }
-    for (int k = lane_id(); k < B.nnzA; k += WAVE) {
-        const double jv = 1.0e-99 + Ag[k];
-        double val = jv * (-dt);
-        if (isdiag[k]) val = val + 1.0;
-        Ag[k] = val;
-    }
-    const DevSeg &D = B.dxdt;
+    block_jacobian_dd(B, stem, Aval, jlo);
+    int stop = 0;
+    for (int k = lane_id(); k < B.nnzA; k += WAVE) Ag[k] = reduced_value(Ag[k], dt, isdiag[k], stop);
+    stop = ballot(stop) ? 1 : 0;
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
-    const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur);
+    const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur), *lo = gptr((const double *) lop);
     NUPPN_GLB double *rg = gptr(rhs);
-    {
-        const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vx);
-        seg_stream<Sum1>(
-            D, [&](int e, int idx) { return Sum1{coef[e] * vxg[idx]}; },
-            [&](int r, const Sum1 &t) { rg[r] = t.s; });
-    }
+    block_dxdt_dd(B, vx, rhs, lop);
     for (int r = lane_id(); r < N.n; r += WAVE) {
+        // y_old - y_cur + dt (hi + lo): each step split exactly (TwoSum, FMA), rounded once
         const int sp = red2sp[r];
-        rg[r] = yo[sp] - yc[sp] + dt * rg[r];
+        const double a = yo[sp], b = -yc[sp], fh = rg[r];
+        const double s1 = a + b, b1 = s1 - a;
+        const double e1 = (a - (s1 - b1)) + (b - b1);
+        const double p = dt * fh;
+        const double e2 = fma(dt, fh, -p) + dt * lo[r];
+        const double s2 = s1 + p, b2 = s2 - s1;
+        const double e3 = (s1 - (s2 - b2)) + (p - b2);
+        rg[r] = s2 + (e1 + (e2 + e3));
     }
     wave_sync();
+    return stop;
 }
 
 // One Newton iteration (newton_iteration, backward_euler.F90:120-176) for zone z in PH_NEWTON:
@@ -245,23 +306,30 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     double *Aval = (double *) (slot + b.off[B_AVAL]);
     double *rhs = (double *) (slot + b.off[B_RHS]);
     double *x = (double *) (slot + b.off[B_X]);
+    double *xlo = (double *) (slot + b.off[B_XLO]);
     double *res = (double *) (slot + b.off[B_RES]);
     double *d = (double *) (slot + b.off[B_D]);
     const double dt = s.dttry;
-    block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs);
-    int rc = 1;
-    if (s.pad0 == 0) {
-        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, res, d, slot, lds,
+    const int stop = block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
+                                    (double *) (slot + b.off[B_LO]),
+                                    (double *) (slot + b.off[B_JAC]));
+    int rc = stop ? 2 : 1;
+    if (!stop && s.pad0 == 0) {
+        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot, lds,
                                             s.st);
         if (rc == 1) s.pad0 = 1;
     }
     if (rc == 1)
-        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, res, d, slot, lds,
+        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot, lds,
                                              s.st);
     if (rc == 0) {
+        // y_new = y_cur + (x + xlo), rounded once
         for (int r = lane_id(); r < N.n; r += WAVE) {
             const int sp = N.red2sp[r];
-            y_new[sp] = y_cur[sp] + x[r];
+            const double yc = y_cur[sp], xh = x[r];
+            const double sh = yc + xh;
+            const double bb = sh - yc;
+            y_new[sp] = sh + (((yc - (sh - bb)) + (xh - bb)) + xlo[r]);
         }
         wave_sync();
         double g = 0.0;
@@ -273,7 +341,8 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
         }
         s.grd = wave_max(g);
     }
-    s.lib_err = rc;
+    s.lib_err = stop ? 0 : rc;
+    s.step_ierr = stop ? E_BAD_STEP : 0;
     s.phase = PH_AFTER_NEWTON;
     if (lane_id() == 0) Z.state[z] = s;
     wave_sync();
@@ -307,19 +376,12 @@ struct BlockBDSolver {
         (void) dt;   // min_rate_ov_dt is -1e99 on the static network
         const DevBlock &B = b.B;
         fluxes(y);
-        const DevSeg &J = B.jac;
-        const NUPPN_GLB double *coef = gptr(J.e_coef), *stem = gptr((const double *) arr(B_ST));
+        block_jacobian_dd(B, arr(B_ST), arr(B_JAC), arr(B_AVAL));   // A is set per level
         const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
-        NUPPN_GLB double *jac = gptr(arr(B_JAC));
-        seg_stream<Sum1>(
-            J, [&](int e, int idx) { return Sum1{coef[e] * stem[idx]}; },
-            [&](int k, const Sum1 &t) { jac[k] = t.s; });
+        const NUPPN_GLB double *jac = gptr((const double *) arr(B_JAC));
         int bad = 0;
-        for (int k = lane_id(); k < B.nnzA; k += WAVE) {
-            const double jv = 1.0e-99 + jac[k];
-            jac[k] = jv;
-            if (!isdiag[k] && jv == 0.0) bad = 1;   // calculate_jacobian stops
-        }
+        for (int k = lane_id(); k < B.nnzA; k += WAVE)
+            if (!isdiag[k] && jac[k] == 0.0) bad = 1;   // calculate_jacobian stops
         wave_sync();
         return ballot(bad) ? E_BAD_STEP : 0;
     }
@@ -328,44 +390,48 @@ struct BlockBDSolver {
         const DevBlock &B = b.B;
         const double *jac = arr(B_JAC);
         double *Aval = arr(B_AVAL);
-        for (int k = lane_id(); k < B.nnzA; k += WAVE) {
-            double val = jac[k] * (-h);
-            if (B.csr_isdiag[k]) val = val + 1.0;
-            Aval[k] = val;
-        }
+        int stop = 0;
+        for (int k = lane_id(); k < B.nnzA; k += WAVE)
+            Aval[k] = reduced_value(jac[k], h, B.csr_isdiag[k], stop);
         wave_sync();
+        if (ballot(stop)) return E_BAD_STEP;
         if (s.pad0 == 0 && block_factor_zone<float, NB>(b, Aval, slot, lds, s.st)) s.pad0 = 1;
         if (s.pad0 == 1 && block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
         return 0;
     }
 
+    // dX/dt summed in twice the FP64 precision (block_dxdt_dd), rounded once
     __device__ void dxdt(const double *y, double *out) {
-        const DevNet &N = a.N;
         fluxes(y);
-        const DevSeg &D = b.B.dxdt;
-        const NUPPN_GLB double *coef = gptr(D.e_coef), *vx = gptr((const double *) arr(B_VX));
-        NUPPN_GLB double *o = gptr(out);
-        (void) N;
-        seg_stream<Sum1>(
-            D, [&](int e, int idx) { return Sum1{coef[e] * vx[idx]}; },
-            [&](int r, const Sum1 &t) { o[r] = t.s; });
+        block_dxdt_dd(b.B, arr(B_VX), out, arr(B_LO));
+        const double *lo = arr(B_LO);
+        for (int r = lane_id(); r < b.B.n; r += WAVE) out[r] = out[r] + lo[r];
         wave_sync();
     }
 
+    // x = the refined solution, rounded to FP64 (the stage updates are FP64, as the reference's)
     __device__ int solve(const double *rhs, double *x) {
         const DevNet &N = a.N;
         const double *Aval = arr(B_AVAL);
+        int rc = 1;
         if (s.pad0 == 0) {
-            const int rc = block_refine<float, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
-                                                   slot, lds, s.st);
-            if (rc != 1) return rc;
-            s.pad0 = 1;
-            if (block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
+            rc = block_refine<float, NB>(N, b, Aval, rhs, ycur, x, arr(B_XLO), arr(B_RES),
+                                         arr(B_D), slot, lds, s.st);
+            if (rc == 1) {
+                s.pad0 = 1;
+                if (block_factor_zone<double, NB>(b, Aval, slot, lds, s.st)) return -4;
+            }
         }
         // a non-finite FP64 solution is returned as it is: the step then ends in GOT_NAN, as
         // the reference's does when its solve overflows
-        (void) block_refine<double, NB>(N, b, Aval, rhs, ycur, x, arr(B_RES), arr(B_D),
-                                        slot, lds, s.st);
+        if (rc == 1)
+            (void) block_refine<double, NB>(N, b, Aval, rhs, ycur, x, arr(B_XLO), arr(B_RES),
+                                            arr(B_D), slot, lds, s.st);
+        else if (rc != 0)
+            return rc;
+        const double *xl = arr(B_XLO);
+        for (int r = lane_id(); r < N.n; r += WAVE) x[r] = x[r] + xl[r];
+        wave_sync();
         return 0;
     }
 };

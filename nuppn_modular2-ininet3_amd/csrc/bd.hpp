@@ -3,7 +3,8 @@
 // (276-316, 410-488), extrapolate (371-407) and compute_error (325-366), for one zone on one
 // wavefront.  The linear algebra comes from a solver object:
 //   int jacobian(y, dt)  J at y once per step (calculate_jacobian); 0 or E_BAD_STEP
-//   int level(h)         I - h J assembled and factored; 0 or a negative library error
+//   int level(h)         I - h J assembled and factored; 0, E_BAD_STEP (a zero where the
+//                        reference stops) or a negative library error
 //   void dxdt(y, out)    reduced dX/dt at y (calculate_dxdt + reduce_rhs_bd)
 //   int solve(rhs, x)    x = (I - h J)^-1 rhs; 0 or a negative library error
 // The plain solver below factors with the reference's GEPP (pivot order of SuperLU); the
@@ -118,6 +119,11 @@ __device__ void bd_step(const KArgs &a, int z, ZoneState &s, BDState &bd, const 
         const double h = dt / (double) ns;
         const double xest = (h / ns) * (h / ns);
         const int e = sv.level(h);
+        if (e > 0) {   // reduce_jacobian's stop (semi_implicit_extrapolation, 287-289)
+            s.step_ierr = E_BAD_STEP;
+            s.dtrcmd = dt * 0.2;
+            return;
+        }
         if (e) {
             s.lib_err = e;
             return;
@@ -256,22 +262,14 @@ struct PlainBDSolver {
         return ballot(bad) ? E_BAD_STEP : 0;
     }
 
+    // I - h J with reduce_jacobian's drops; E_BAD_STEP where the reference stops
     __device__ int level(double h) {
         const DevNet &N = a.N;
-        int bad = 0;
-        for (int q = lane_id(); q < N.nnzA; q += WAVE) {
-            const double jv = S.jac_cpy[q];
-            double val = jv * (-h);
-            if (N.A_isdiag[q]) {
-                val = val + 1.0;
-                if (val == 0.0) bad = 1;
-            } else if (jv == 0.0 || val == 0.0 || val == -1.0) {
-                bad = 1;
-            }
-            S.Aval[q] = val;
-        }
+        int stop = 0;
+        for (int q = lane_id(); q < N.nnzA; q += WAVE)
+            S.Aval[q] = reduced_value(S.jac_cpy[q], h, N.A_isdiag[q], stop);
         wave_sync();
-        if (ballot(bad)) return -3;
+        if (ballot(stop)) return E_BAD_STEP;
         const int info = gepp_factor(N.n, N.A_colptr, N.A_rowind, S.Aval, N.iperm_c, w, S.lu);
         s.st.n_factor++;
         if (info) return -4;

@@ -31,6 +31,7 @@ struct DevSeg {
     int nseg;
     const int *e_idx;       // operand index, bit 31 on a segment's last term
     const double *e_coef;
+    const double *e_coef_lo;   // J: low parts of the coefficients (null for the other streams)
     const int *lane_t0;     // [65]
     const int *lane_s0;     // [64]
 };
@@ -453,6 +454,11 @@ __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFacto
 // species against this floor at least
 constexpr double kAbundanceFloor = 1.0e-30;
 
+// the refinement's absolute floor: new abundances below it (far below the 1e-20 of any
+// reported mass fraction and the 1e-27 under which ypsplus clips) need not converge relative
+// to themselves
+constexpr double kRefineFloor = 1.0e-25;
+
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
@@ -793,33 +799,81 @@ __device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
     wave_sync();
 }
 
-// r = rhs - A x (A in CSR order), FP64.  Returns the componentwise backward error of x,
-// max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager): x solves a system whose
-// every entry is within that relative distance of A and rhs, ignoring rows whose whole
-// magnitude is below the abundance floor.  absum: [n] scratch for the sums of |A_ij x_j|.
+// A running sum carried in twice the working precision (Ogita, Rump and Oishi, "Accurate sum
+// and dot product", SIAM J. Sci. Comput. 26 (2005), algorithm Dot2): hi + lo, where every
+// addition into hi is split exactly by TwoSum (Knuth) and its rounding error collects in lo;
+// a term arrives as an exact product split p + e by one FMA.  The result is as accurate as a
+// sum computed in double-double and rounded once, error <= eps |sum| + O(n eps^2) sum |terms|.
+// Beside it, the plain sum of the terms' magnitudes.
+struct SumDD {
+    double hi, lo, a;
+    __device__ void add(const SumDD &o) {
+        const double s = hi + o.hi;
+        const double bb = s - hi;
+        const double err = (hi - (s - bb)) + (o.hi - bb);
+        hi = s;
+        lo = lo + (o.lo + err);
+        a = a + o.a;
+    }
+    __device__ SumDD lane(int k) const {
+        return {readlane_d(hi, k), readlane_d(lo, k), readlane_d(a, k)};
+    }
+};
+
+// the same without the magnitudes
+struct SumDD2 {
+    double hi, lo;
+    __device__ void add(const SumDD2 &o) {
+        const double s = hi + o.hi;
+        const double bb = s - hi;
+        const double err = (hi - (s - bb)) + (o.hi - bb);
+        hi = s;
+        lo = lo + (o.lo + err);
+    }
+    __device__ SumDD2 lane(int k) const { return {readlane_d(hi, k), readlane_d(lo, k)}; }
+};
+
+// r = rhs - A (x + xlo) (A in CSR order; x + xlo the solution carried in twice the FP64
+// precision) with the products and sums carried in twice the FP64
+// precision, rounded once: the residual of the FP64 system to FP64 accuracy even where the
+// terms cancel by many orders of magnitude, so iterative refinement converges to the solution
+// of the FP64 system itself (its forward error, not only its backward error, reaches the FP64
+// level: Demmel, Hida, Kahan, Li, Mukherjee and Riedy, "Error bounds from extra-precise
+// iterative refinement", ACM TOMS 32 (2006)).  Returns the componentwise backward error
+// max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager).  lo, absum: [n] scratch.
 __device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp,
                                                  const double *rhsp, const double *xp,
-                                                 double *rp, double *absump) {
+                                                 const double *xlop, double *rp,
+                                                 double *absump, double *lop) {
     const NUPPN_GLB double *Acsr = gptr(Acsrp), *rhs = gptr(rhsp), *x = gptr(xp);
+    const NUPPN_GLB double *xlo = gptr(xlop);
     const NUPPN_GLB int *csr_col = gptr(B.csr_col);
-    NUPPN_GLB double *r = gptr(rp), *absum = gptr(absump);
+    NUPPN_GLB double *r = gptr(rp), *absum = gptr(absump), *lo = gptr(lop);
     // the residual's terms are the CSR entries in order: term e is entry e
-    seg_stream<Sum2>(
+    seg_stream<SumDD>(
         B.resid,
         [&](int e, int k) {
             (void) k;
-            const double t = Acsr[e] * x[csr_col[e]];
-            return Sum2{t, fabs(t)};
+            const int c = csr_col[e];
+            const double av = Acsr[e], xv = x[c];
+            const double t = av * xv;
+            return SumDD{t, fma(av, xlo[c], fma(av, xv, -t)), fabs(t)};
         },
-        [&](int i, const Sum2 &t) {
-            r[i] = t.s;
+        [&](int i, const SumDD &t) {
+            r[i] = t.hi;
+            lo[i] = t.lo;
             absum[i] = t.a;
         });
     double om = 0.0;
     for (int i = lane_id(); i < B.n; i += WAVE) {
-        const double ri = rhs[i] - r[i];
+        // rhs - (hi + lo): the difference rhs - hi split exactly, then the low parts
+        const double b = rhs[i], hi = r[i];
+        const double s = b - hi;
+        const double bb = s - b;
+        const double err = (b - (s - bb)) + (-hi - bb);
+        const double ri = s + (err - lo[i]);
         r[i] = ri;
-        om = fmax(om, fabs(ri) / (absum[i] + fabs(rhs[i]) + kAbundanceFloor));
+        om = fmax(om, fabs(ri) / (absum[i] + fabs(b) + kAbundanceFloor));
     }
     om = wave_max(om);
     wave_sync();

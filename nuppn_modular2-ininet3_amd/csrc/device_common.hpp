@@ -259,23 +259,31 @@ __device__ __forceinline__ double jac_entry(const DevNet &N, int q, const double
 }
 
 // Reduced matrix values of I - scale*J (newton_iteration val*(-dt) + reduce_jacobian,
-// jac_rhs.F90:140-161): returns 0, or NUPPN_ERR_PATTERN when an entry would be dropped by the
-// reference (exact 0 or -1 before the identity) or the diagonal ends exactly 0.
+// jac_rhs.F90:140-161).  reduce_jacobian drops every entry of -dt J that is exactly 0 or -1
+// (138-146) before it adds the identity: a dropped off-diagonal entry is stored as an explicit
+// zero here (the static pattern keeps its slot; with the column order fixed, an explicit zero
+// changes no value of the factors or the solve).  The diagonal of a masked species is restored
+// and takes val + 1 whatever it held (151-161).  Returns 1 where the reference stops instead:
+// an off-diagonal Jacobian entry exactly zero (calculate_jacobian, 444-451) or a reduced
+// diagonal exactly zero (177-182).
+__device__ __forceinline__ double reduced_value(double jv, double scale, bool diag, int &stop) {
+    double val = jv * (-scale);
+    if (diag) {
+        val = val + 1.0;
+        if (val == 0.0) stop = 1;
+    } else {
+        if (jv == 0.0) stop = 1;
+        if (fabs(val) == 0.0 || val == -1.0) val = 0.0;
+    }
+    return val;
+}
+
 __device__ inline int assemble_matrix(const DevNet &N, double scale, const double *st1,
                                       const double *st2, double *Aval) {
-    int bad = 0;
-    for (int q = lane_id(); q < N.nnzA; q += WAVE) {
-        double jv = jac_entry(N, q, st1, st2);
-        double val = jv * (-scale);
-        if (N.A_isdiag[q]) {
-            val = val + 1.0;
-            if (val == 0.0) bad = 1;
-        } else {
-            if (jv == 0.0 || fabs(val) == 0.0 || val == -1.0) bad = 1;
-        }
-        Aval[q] = val;
-    }
-    return ballot(bad) ? -3 : 0;
+    int stop = 0;
+    for (int q = lane_id(); q < N.nnzA; q += WAVE)
+        Aval[q] = reduced_value(jac_entry(N, q, st1, st2), scale, N.A_isdiag[q], stop);
+    return ballot(stop) ? 1 : 0;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -222,6 +222,7 @@ struct nuppn_ctx {
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
     int *d_seg_i[9] = {};         // jac, dxdt, resid: e_idx, lane_t0, lane_s0
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
+    double *d_jac_lo = nullptr;   // jac: e_coef_lo
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
     int bmax_slots = 0, bslots = 0;
@@ -303,7 +304,9 @@ static DevBlock dev_block(const nuppn_ctx *c) {
         ds[i]->lane_t0 = c->d_seg_i[3 * i + 1];
         ds[i]->lane_s0 = c->d_seg_i[3 * i + 2];
         ds[i]->e_coef = c->d_seg_c[i];
+        ds[i]->e_coef_lo = nullptr;
     }
+    B.jac.e_coef_lo = c->d_jac_lo;
     return B;
 }
 
@@ -374,6 +377,7 @@ static int setup_batched(nuppn_ctx *c) {
             dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
             return NUPPN_ERR_HIP;
     }
+    if (dev_upload(&c->d_jac_lo, b.jac.e_coef_lo)) return NUPPN_ERR_HIP;
     c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
                               b.nbd);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
@@ -692,6 +696,7 @@ void nuppn_ctx_destroy(nuppn_ctx *c) {
         dev_free(p);
     for (void *p : c->d_seg_i) dev_free(p);
     for (void *p : c->d_seg_c) dev_free(p);
+    dev_free(c->d_jac_lo);
     for (auto &e : c->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e : c->evn)
@@ -1022,9 +1027,9 @@ int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->factor_precision = 32;
-    o->max_refine = 8;
-    o->refine_tol = 1.0e-13;
-    o->accept_tol = 1.0e-12;
+    o->max_refine = 12;
+    o->refine_tol = 1.0e-17;
+    o->accept_tol = 0.0;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {

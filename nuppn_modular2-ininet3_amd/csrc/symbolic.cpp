@@ -1,6 +1,8 @@
 // Host-side symbolic phase (see symbolic.hpp).
 #include "symbolic.hpp"
 
+#include <cmath>
+
 #include <algorithm>
 #include <cstdint>
 #include <tuple>
@@ -426,7 +428,11 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
     const double *an = net.aw.data();
     // J contributions (statement order of calculate_jacobian, jac_rhs.F90:365-392):
     // value = (+-count) * an[row] / an[col] * stem, stem = st1[l] (m in 0,1,3,4) or st2[l]
+    // The coefficient cnt * an[row] / an[col] is kept as a double-double, hi + lo, with hi the
+    // rounded quotient and lo = (num - hi an[col]) / an[col] (the remainder of the division is
+    // exact with an FMA), so that the device can sum each entry's contributions as real values.
     std::vector<std::vector<std::pair<int, double>>> segs(nnz);
+    std::vector<std::vector<double>> seg_lo(nnz);
     for (int k = 0; k < nnz; ++k) {
         const int q = b.csr_q[k];
         for (int e = s.jq_ptr[q]; e < s.jq_ptr[q + 1]; ++e) {
@@ -438,10 +444,19 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
             const int cols[8] = {in1, in1, in2, in1, in1, in2, in2, in2};
             const int cnt[8] = {-n1, -n2, -n1, no1, no2, -n2, no1, no2};
             const bool stem2[8] = {false, false, true, false, false, true, true, true};
-            segs[k].push_back({stem2[m] ? nre + l : l, cnt[m] * an[rows[m]] / an[cols[m]]});
+            const double num = cnt[m] * an[rows[m]], ac = an[cols[m]];
+            const double hi = num / ac;
+            segs[k].push_back({stem2[m] ? nre + l : l, hi});
+            seg_lo[k].push_back(std::fma(-hi, ac, num) / ac);
         }
     }
     build_stream(segs, b.jac);
+    b.jac.e_coef_lo.clear();   // in build_stream's term order (an empty segment: one zero term)
+    for (int k = 0; k < nnz; ++k) {
+        if (seg_lo[k].empty()) b.jac.e_coef_lo.push_back(0.0);
+        for (double lo : seg_lo[k]) b.jac.e_coef_lo.push_back(lo);
+    }
+    if (b.jac.e_coef_lo.size() != b.jac.e_coef.size()) return -1;
     // dX/dt (calculate_dxdt, jac_rhs.F90:250-272): roles k1 -k2, k3 -1, k5 +k6, k7 +k8, times A
     std::vector<std::vector<std::pair<int, double>>> dsegs(n);
     for (int r = 0; r < n; ++r) {

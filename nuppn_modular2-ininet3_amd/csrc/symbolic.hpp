@@ -45,6 +45,7 @@ struct SegStream {
     int nseg = 0;
     std::vector<int> e_idx;       // operand index, | (1 << 31) on a segment's last term
     std::vector<double> e_coef;   // coefficient
+    std::vector<double> e_coef_lo;  // J only: the coefficient's low part (hi + lo its exact value)
     std::vector<int> lane_t0;     // [65]
     std::vector<int> lane_s0;     // [64]
 };

@@ -44,10 +44,12 @@ inline SlotLayout slot_layout(int nsp, int nre, int n, int nnzA) {
 enum ZonePhase { PH_SUBSTEP = 0, PH_NEWTON = 1, PH_AFTER_NEWTON = 2, PH_DONE = 3 };
 
 // pad0: factor precision of the batched path (0 FP32, 1 FP64); step_ierr: the ierr of the last
-// Bader-Deuflhard step (its dtrcmd goes to dtrcmd)
+// Bader-Deuflhard step (its dtrcmd goes to dtrcmd); ye_cur: integrate_network's ye, which
+// do_one_time_step recomputes from every step's raw result (solver.F90:272) and a failed step
+// restores (solver.F90:190,197)
 struct ZoneState {
     int phase, i, iter, ierr, lib_err, solved, pad0, step_ierr;
-    double dtdone, dttry, dtrcmd, my_t9_last, my_rho_last, grd, ye_entry, pad2;
+    double dtdone, dttry, dtrcmd, my_t9_last, my_rho_last, grd, ye_entry, ye_cur;
     ZoneStats st;
 };
 
@@ -173,13 +175,17 @@ __device__ inline void zone_init(const KArgs &a, int z, NUPPN_LDS double *sb) {
         s.my_t9_last = 0.5 * (t9_0 + t9_1);
         s.my_rho_last = 0.5 * (rho_0 + rho_1);
         s.ye_entry = ye;
+        s.ye_cur = ye;
         Z.state[z] = s;
         Z.bd[z] = Z.bd0;
     }
     wave_sync();
 }
 
-// final bookkeeping of a zone (solver.F90:229-237): clip, Ye, outputs
+// final bookkeeping of a zone (solver.F90:229-237): clip, Ye, outputs.  Under backward Euler
+// the Ye of the last step's result equals the Ye of the final abundances (ypsplus leaves every
+// value >= 1e-99, so the clips change nothing); under Bader-Deuflhard the renormalisation and the
+// clip come after the last Ye (solver.F90:234-237), which the zone state carries.
 __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr, bool loop_done,
                                    NUPPN_LDS double *sb) {
     const DevNet &N = a.N;
@@ -195,7 +201,9 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
         wave_sync();
     }
     double ye = s.ye_entry;
-    if (s.solved) {
+    if (a.K.integration_method == 1) {
+        ye = s.ye_cur;
+    } else if (s.solved) {
         double *work = zrow(Z.work, Z.work_stride, z);
         ye = calculate_ye(N, yps, work, sb);
     }
@@ -232,16 +240,22 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
     bool step_ended = false;   // a backward-Euler step just returned (ierr set)
 
     if (s.phase == PH_AFTER_NEWTON && K.integration_method == 1) {
-        // a Bader-Deuflhard step returned (do_one_time_step, solver.F90:164-179)
+        // a Bader-Deuflhard step returned (do_one_time_step, solver.F90:164-179): Ye of the
+        // step's result before any clip (solver.F90:272); kept on success and on the returns
+        // that follow at once (BAD_STEP, SMALL_DT), replaced by ye_old on a failed step
+        const double ye_step = calculate_ye(N, yps, zrow(Z.work, Z.work_stride, z), sb);
         if (s.lib_err || s.step_ierr == E_BAD_STEP) {
+            s.ye_cur = ye_step;
             zone_finish(a, z, s, E_BAD_STEP, false, sb);
             if (lane_id() == 0) Z.state[z] = s;
             return;
         }
         ierr = s.step_ierr;
+        if (ierr == 0 || s.dttry < SMALL_C) s.ye_cur = ye_step;
         step_ended = true;
     } else if (s.phase == PH_AFTER_NEWTON) {
-        if (s.lib_err) {   // the reference stops here (zero in the matrix, singular factor)
+        if (s.lib_err || s.step_ierr == E_BAD_STEP) {
+            // the reference stops here (a zero in the matrix, a singular factor)
             zone_finish(a, z, s, 3, false, sb);
             if (lane_id() == 0) Z.state[z] = s;
             return;
@@ -407,7 +421,8 @@ __device__ inline void zone_newton(const KArgs &a, const Slot &S, LUWork w, int 
     wave_sync();
     reaction_terms(N, S.xn, v, S.vx, S.st1, S.st2);
     wave_sync();
-    int e = assemble_matrix(N, dt, S.st1, S.st2, S.Aval);
+    const int stop = assemble_matrix(N, dt, S.st1, S.st2, S.Aval);
+    int e = stop ? 1 : 0;
     for (int r = lane_id(); r < N.n; r += WAVE) {
         const int sp = N.red2sp[r];
         const double d = dxdt_species(N, r, S.vx);
@@ -437,7 +452,10 @@ __device__ inline void zone_newton(const KArgs &a, const Slot &S, LUWork w, int 
         }
         s.grd = wave_max(g);
     }
-    s.lib_err = e;
+    // a zero where the reference stops (calculate_jacobian / reduce_jacobian) ends the zone
+    // with BAD_STEP; a singular factorisation is the library error NUPPN_ERR_SINGULAR
+    s.lib_err = stop ? 0 : e;
+    s.step_ierr = stop ? E_BAD_STEP : 0;
     s.phase = PH_AFTER_NEWTON;
     if (lane_id() == 0) Z.state[z] = s;
     wave_sync();

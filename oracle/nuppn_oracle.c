@@ -36,7 +36,7 @@ struct orc_ctx {
     int *rowptr, *colind, *map, *row_of;
     int *diag;              /* slot of (i,i) or -1 */
     /* work */
-    double *v, *jac, *jac_cpy, *xn, *dxdt, *rhs, *red_val, *red_rhs;
+    double *v, *jac, *jac_lo, *jac_cpy, *xn, *dxdt, *dxdt_lo, *rhs, *red_val, *red_rhs, *red_xlo;
     int *jrow, *jcol, *red_rowptr, *red_colind, *iexpand;
     unsigned char *mask;
     int nvar1, red_nnz;
@@ -283,6 +283,7 @@ orc_ctx *orc_create(int nsp, int nre, const int *k, const int *rtype, const doub
     makemap(c);
     c->v = malloc(sizeof(double) * nre);
     c->jac = malloc(sizeof(double) * c->nnz);
+    c->jac_lo = malloc(sizeof(double) * c->nnz);
     c->jac_cpy = malloc(sizeof(double) * c->nnz);
     c->jrow = malloc(sizeof(int) * c->nnz);
     c->jcol = malloc(sizeof(int) * c->nnz);
@@ -292,8 +293,10 @@ orc_ctx *orc_create(int nsp, int nre, const int *k, const int *rtype, const doub
     c->iexpand = malloc(sizeof(int) * nsp);
     c->xn = malloc(sizeof(double) * nsp);
     c->dxdt = malloc(sizeof(double) * nsp);
+    c->dxdt_lo = malloc(sizeof(double) * nsp);
     c->rhs = malloc(sizeof(double) * nsp);
     c->red_rhs = malloc(sizeof(double) * nsp);
+    c->red_xlo = malloc(sizeof(double) * nsp);
     c->mask = malloc(nsp);
     c->qcol = calloc((size_t) nsp * NLEVELS, sizeof(double));
     c->yerr = calloc((size_t) nsp * NLEVELS, sizeof(double));
@@ -320,14 +323,23 @@ void orc_set_linear_solver(orc_ctx *c, int which) { c->lsolver = which; }
 
 void orc_set_rates(orc_ctx *c, const double *v) { memcpy(c->v, v, sizeof(double) * c->nre); }
 
+void orc_bd_state(const orc_ctx *c, int *iv, double *dv)
+{
+    iv[0] = c->frst; iv[1] = c->reduce; iv[2] = c->kopt; iv[3] = c->kmax;
+    dv[0] = c->errmax; dv[1] = c->abuerr; dv[2] = c->negerr; dv[3] = c->sumerr;
+    for (int k = 0; k < NLEVELS; ++k) dv[4 + k] = c->err[k];
+}
+
+void orc_bd_set_frst(orc_ctx *c, int frst) { c->frst = frst; }
+
 void orc_destroy(orc_ctx *c)
 {
     if (!c) return;
     free(c->k); free(c->rtype); free(c->coef); free(c->aw); free(c->zw); free(c->consider);
     free(c->rowptr); free(c->colind); free(c->row_of); free(c->map); free(c->diag);
-    free(c->v); free(c->jac); free(c->jac_cpy); free(c->jrow); free(c->jcol);
+    free(c->v); free(c->jac); free(c->jac_lo); free(c->jac_cpy); free(c->jrow); free(c->jcol);
     free(c->red_val); free(c->red_colind); free(c->red_rowptr); free(c->iexpand);
-    free(c->xn); free(c->dxdt); free(c->rhs); free(c->red_rhs); free(c->mask);
+    free(c->xn); free(c->dxdt); free(c->dxdt_lo); free(c->rhs); free(c->red_rhs); free(c->red_xlo); free(c->mask);
     free(c->qcol); free(c->yerr);
     free(c);
 }
@@ -362,13 +374,27 @@ void orc_rates(const orc_ctx *c, double t9in, double rho, double ye, double *v)
     }
 }
 
-/* calculate_dxdt, jac_rhs.F90:217-278 */
+/* s += t as an unevaluated double-double sum (hi, lo): TwoSum (Knuth), the error into lo */
+static void dd_add(double *hi, double *lo, double t)
+{
+    const double s = *hi + t, bb = s - *hi;
+    *lo += (*hi - (s - bb)) + (t - bb);
+    *hi = s;
+}
+
+/* calculate_dxdt, jac_rhs.F90:217-278.  Under ORC_LS_EXACT each species' dX/dt is the value
+ * of sum_l (count A) vx_l rounded once: the products split exactly (FMA) and summed in
+ * double-double, dxdt = RN(S), dxdt_lo = S - RN(S) to double-double accuracy (a yardstick for
+ * the rounding of the reference's sequential FP64 sum). */
 void orc_dxdt(const orc_ctx *c, const double *x, const double *v, double *dxdt)
 {
     const int nsp = c->nsp, nre = c->nre, g = c->igamma;
-    double *xn = c->xn;
+    const int exact = c->lsolver == ORC_LS_EXACT;
+    double *xn = c->xn, *lo = c->dxdt_lo;
+    const double *an = c->aw;
     for (int i = 0; i < nsp; ++i) {
         dxdt[i] = 0.0;
+        lo[i] = 0.0;
         xn[i] = (c->consider[i] && c->aw[i] != 0.0) ? x[i] / c->aw[i] : 0.0;
     }
     xn[g] = 1.0;
@@ -377,18 +403,51 @@ void orc_dxdt(const orc_ctx *c, const double *x, const double *v, double *dxdt)
         int k5 = c->k[4 * nre + l], k6 = c->k[5 * nre + l], k7 = c->k[6 * nre + l];
         int k8 = c->k[7 * nre + l];
         double vx = FACT[k2] * v[l] * powi(xn[k1], k2) * xn[k3];
+        if (exact) {
+            const int sp[4] = {k1, k3, k5, k7};
+            const double cnt[4] = {-k2, -1.0, k6, k8};
+            for (int r = 0; r < 4; ++r) {
+                const double ca = cnt[r] * an[sp[r]], t = ca * vx;
+                dd_add(&dxdt[sp[r]], &lo[sp[r]], t);
+                lo[sp[r]] += fma(ca, vx, -t);
+            }
+            continue;
+        }
         double dx1 = -k2 * vx, dx3 = -vx, dx5 = k6 * vx, dx7 = k8 * vx;
         dxdt[k1] = dxdt[k1] + dx1;
         dxdt[k3] = dxdt[k3] + dx3;
         dxdt[k5] = dxdt[k5] + dx5;
         dxdt[k7] = dxdt[k7] + dx7;
     }
-    for (int i = 0; i < nsp; ++i)
-        if (c->consider[i]) dxdt[i] = dxdt[i] * c->aw[i];
+    for (int i = 0; i < nsp; ++i) {
+        if (!c->consider[i]) continue;
+        if (exact) {
+            const double hi = dxdt[i], r = hi + lo[i];
+            lo[i] = lo[i] - (r - hi);
+            dxdt[i] = r;
+        } else {
+            dxdt[i] = dxdt[i] * c->aw[i];
+        }
+    }
     dxdt[g] = 0.0;
+    lo[g] = 0.0;
 }
 
-/* calculate_jacobian, jac_rhs.F90:282-455 (irdn = 2) */
+/* One Jacobian contribution cnt * stem * a_row / a_col as a real number, added to the
+ * double-double (hi, lo): the ratio cnt a_row / a_col as a double-double (its remainder is
+ * exact with an FMA), the product with stem split exactly.  ORC_LS_EXACT assembles J from
+ * these and rounds each entry once. */
+static void jterm_dd(double *hi, double *lo, double cnt, double stem, double ar, double ac)
+{
+    const double num = cnt * ar;   /* an exact integer */
+    const double ch = num / ac, cl = fma(-ch, ac, num) / ac;
+    const double t = stem * ch;
+    dd_add(hi, lo, t);
+    *lo += fma(stem, ch, -t) + stem * cl;
+}
+
+/* calculate_jacobian, jac_rhs.F90:282-455 (irdn = 2).  Under ORC_LS_EXACT every entry is the
+ * once-rounded sum of its contributions as real values (jterm_dd). */
 int orc_jacobian(const orc_ctx *c, double dzeit, const double *x, const double *v,
                  double *jac, int *row, int *col, unsigned char *mask)
 {
@@ -397,7 +456,9 @@ int orc_jacobian(const orc_ctx *c, double dzeit, const double *x, const double *
     double *xn = c->xn;
     for (int i = 0; i < nsp; ++i) mask[i] = 0;
     const double dfdNmin = c->kn.cyminformal / dzeit;
-    for (int i = 0; i < nnz; ++i) { jac[i] = 1.0e-99; row[i] = -1; col[i] = -1; }
+    const int exact = c->lsolver == ORC_LS_EXACT;
+    double *jlo = c->jac_lo;
+    for (int i = 0; i < nnz; ++i) { jac[i] = 1.0e-99; jlo[i] = 0.0; row[i] = -1; col[i] = -1; }
     for (int i = 0; i < nsp; ++i)
         if (c->consider[i] && an[i] > 0.0) xn[i] = x[i] / an[i];
         else xn[i] = 0.0;   /* untouched in the reference; only gamma (reset below) hits this */
@@ -425,16 +486,33 @@ int orc_jacobian(const orc_ctx *c, double dzeit, const double *x, const double *
         const double dfin1dy2 = -Nin1 * stem, dfin2dy2 = -Nin2 * stem;
         const double dfout1dy2 = Nout1 * stem, dfout2dy2 = Nout2 * stem;
 
-        if (in1 != g) jac[ice1] = jac[ice1] + dfin1dy1 * ain1 / ain1;
-        if (in1 != g && in2 != g) {
-            jac[ice2] = jac[ice2] + dfin2dy1 * ain2 / ain1;
-            jac[ice3] = jac[ice3] + dfin1dy2 * ain1 / ain2;
+        if (exact) {
+            const double s1 = (double) Nin1 * FACT[Nin1] * FACT[Nin2] * powi(y2, Nin2) *
+                              powi(y1, Nin1 - 1) * v[l];
+            const double s2 = (double) Nin2 * FACT[Nin1] * FACT[Nin2] * powi(y1, Nin1) *
+                              powi(y2, Nin2 - 1) * v[l];
+            if (in1 != g) jterm_dd(&jac[ice1], &jlo[ice1], -Nin1, s1, ain1, ain1);
+            if (in1 != g && in2 != g) {
+                jterm_dd(&jac[ice2], &jlo[ice2], -Nin2, s1, ain2, ain1);
+                jterm_dd(&jac[ice3], &jlo[ice3], -Nin1, s2, ain1, ain2);
+            }
+            if (in1 != g && out1 != g) jterm_dd(&jac[ice4], &jlo[ice4], Nout1, s1, aout1, ain1);
+            if (in1 != g && out2 != g) jterm_dd(&jac[ice5], &jlo[ice5], Nout2, s1, aout2, ain1);
+            if (in2 != g) jterm_dd(&jac[ice6], &jlo[ice6], -Nin2, s2, ain2, ain2);
+            if (in2 != g && out1 != g) jterm_dd(&jac[ice7], &jlo[ice7], Nout1, s2, aout1, ain2);
+            if (in2 != g && out2 != g) jterm_dd(&jac[ice8], &jlo[ice8], Nout2, s2, aout2, ain2);
+        } else {
+            if (in1 != g) jac[ice1] = jac[ice1] + dfin1dy1 * ain1 / ain1;
+            if (in1 != g && in2 != g) {
+                jac[ice2] = jac[ice2] + dfin2dy1 * ain2 / ain1;
+                jac[ice3] = jac[ice3] + dfin1dy2 * ain1 / ain2;
+            }
+            if (in1 != g && out1 != g) jac[ice4] = jac[ice4] + dfout1dy1 * aout1 / ain1;
+            if (in1 != g && out2 != g) jac[ice5] = jac[ice5] + dfout2dy1 * aout2 / ain1;
+            if (in2 != g) jac[ice6] = jac[ice6] + dfin2dy2 * ain2 / ain2;
+            if (in2 != g && out1 != g) jac[ice7] = jac[ice7] + dfout1dy2 * aout1 / ain2;
+            if (in2 != g && out2 != g) jac[ice8] = jac[ice8] + dfout2dy2 * aout2 / ain2;
         }
-        if (in1 != g && out1 != g) jac[ice4] = jac[ice4] + dfout1dy1 * aout1 / ain1;
-        if (in1 != g && out2 != g) jac[ice5] = jac[ice5] + dfout2dy1 * aout2 / ain1;
-        if (in2 != g) jac[ice6] = jac[ice6] + dfin2dy2 * ain2 / ain2;
-        if (in2 != g && out1 != g) jac[ice7] = jac[ice7] + dfout1dy2 * aout1 / ain2;
-        if (in2 != g && out2 != g) jac[ice8] = jac[ice8] + dfout2dy2 * aout2 / ain2;
 
         if (in2 != g && !mask[in2]) continue;
         if (out1 != g && !mask[out1]) continue;
@@ -452,6 +530,8 @@ int orc_jacobian(const orc_ctx *c, double dzeit, const double *x, const double *
         }
     }
     mask[g] = 0;
+    if (exact)
+        for (int l = 0; l < nnz; ++l) jac[l] = jac[l] + jlo[l];
     for (int l = 0; l < nnz; ++l)
         if (jac[l] == 0.0 && row[l] != col[l]) return 1;
     return 0;
@@ -676,11 +756,118 @@ static int gepp_factor_csr(int n, int nnz, const int *rowptr, const int *colind,
     return info;
 }
 
+/* ORC_LS_EXACT: the solution of the FP64 system itself, carried as x + xlo (double-double).
+ * x from the factors is refined, x += LU^-1 (b - A (x + xlo)), with the residual accumulated
+ * in quad precision (__float128, 113-bit significand) and rounded once, and each correction
+ * added to x + xlo by TwoSum, until the corrections are below 1e-30 |x_i| (or stop shrinking,
+ * or 200 steps).  The refinement converges to the exact solution whenever cond(A) eps(FP64)
+ * < 1 (Wilkinson; Demmel et al., "Error bounds from extra-precise iterative refinement", ACM
+ * TOMS 32, 2006): the reference algorithm with exact linear algebra, the yardstick for the
+ * rounding of any FP64 solve. */
+static void exact_refine(int n, const int *rowptr, const int *colind, const double *val,
+                         const double *b, double *x, double *xlo,
+                         void (*solve)(const void *, double *), const void *fac)
+{
+    double *d = malloc(sizeof(double) * (n > 0 ? n : 1));
+    double prev = HUGE_VAL;
+    for (int i = 0; i < n; ++i) xlo[i] = 0.0;
+    for (int it = 0; it < 200; ++it) {
+        for (int i = 0; i < n; ++i) {
+            __float128 r = (__float128) b[i];
+            for (int p = rowptr[i]; p < rowptr[i + 1]; ++p)
+                r -= (__float128) val[p] * ((__float128) x[colind[p]] + (__float128) xlo[colind[p]]);
+            d[i] = (double) r;
+        }
+        solve(fac, d);
+        double eta = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double sc = fabs(x[i]) > 0.0 ? fabs(x[i]) : 1e-300;
+            const double e = fabs(d[i]) / sc;
+            if (e > eta) eta = e;
+            double hi = x[i], lo = xlo[i];
+            dd_add(&hi, &lo, d[i]);
+            x[i] = hi + lo;
+            xlo[i] = lo - (x[i] - hi);
+        }
+        if (eta <= 1e-30 || !(eta < prev) || !(eta == eta)) break;
+        prev = eta;
+    }
+    free(d);
+}
+
+static void solve_gepp_cb(const void *lu, double *b) { orc_lu_solve((const orc_lu *) lu, b); }
+
+/* the reduced CSR system factored by the reference c_fortran_dgssv_ (1-based CSC copies) */
+typedef struct {
+    int n, nnz;
+    int *cp, *ri;
+    double *cv;
+    long long handle;
+} ref_handle;
+
+static int ref_factor_csr(const orc_ctx *c, ref_handle *h)
+{
+    const int n = c->nvar1, nnz = c->red_nnz;
+    h->n = n; h->nnz = nnz;
+    h->cp = malloc(sizeof(int) * (n + 1)); h->ri = malloc(sizeof(int) * (nnz + 1));
+    h->cv = malloc(sizeof(double) * (nnz + 1));
+    csr_to_csc(n, nnz, c->red_rowptr, c->red_colind, c->red_val, h->cp, h->ri, h->cv);
+    for (int i = 0; i <= n; ++i) h->cp[i] += 1;
+    for (int i = 0; i < nnz; ++i) h->ri[i] += 1;
+    int info = 0;
+    double dummy = 0.0;
+    orc_ref_dgssv(1, n, nnz, 1, h->cv, h->ri, h->cp, &dummy, n, &h->handle, &info);
+    return info;
+}
+
+static void solve_ref_cb(const void *hv, double *b)
+{
+    ref_handle *h = (ref_handle *) hv;
+    int info = 0;
+    orc_ref_dgssv(2, h->n, h->nnz, 1, h->cv, h->ri, h->cp, b, h->n, &h->handle, &info);
+}
+
+static void ref_free(ref_handle *h)
+{
+    if (h->handle) {
+        int info = 0;
+        double dummy = 0.0;
+        orc_ref_dgssv(3, h->n, h->nnz, 1, h->cv, h->ri, h->cp, &dummy, h->n, &h->handle, &info);
+    }
+    free(h->cp); free(h->ri); free(h->cv);
+    h->cp = NULL; h->ri = NULL; h->cv = NULL; h->handle = 0;
+}
+
 /* solve_Ax_equals_b_csr option 5 (linalg.F90:110-124): rhs overwritten by the solution */
 static int solve_reduced(orc_ctx *c, orc_stats *st)
 {
     int n = c->nvar1, nnz = c->red_nnz, ierr = 0;
     if (st) { st->n_factor++; st->n_solve++; }
+    if (c->lsolver == ORC_LS_EXACT) {
+        double *b = malloc(sizeof(double) * (n > 0 ? n : 1));
+        memcpy(b, c->red_rhs, sizeof(double) * n);
+        if (g_ref) {   /* the reference SuperLU's factors (faster than the oracle GEPP) */
+            ref_handle h = {0};
+            ierr = ref_factor_csr(c, &h);
+            if (ierr == 0) {
+                solve_ref_cb(&h, c->red_rhs);
+                exact_refine(n, c->red_rowptr, c->red_colind, c->red_val, b, c->red_rhs,
+                             c->red_xlo, solve_ref_cb, &h);
+            }
+            ref_free(&h);
+        } else {
+            orc_lu *lu = NULL;
+            ierr = gepp_factor_csr(n, nnz, c->red_rowptr, c->red_colind, c->red_val, &lu);
+            if (ierr == 0) {
+                orc_lu_solve(lu, c->red_rhs);
+                exact_refine(n, c->red_rowptr, c->red_colind, c->red_val, b, c->red_rhs,
+                             c->red_xlo, solve_gepp_cb, lu);
+            }
+            orc_lu_free(lu);
+        }
+        free(b);
+        return ierr;
+    }
     if (c->lsolver == ORC_LS_REF && g_ref) {
         /* slu_solve takes 1-based CSR and shifts it in place; the caller restores it */
         for (int i = 0; i <= n; ++i) c->red_rowptr[i] += 1;
@@ -792,7 +979,19 @@ static int newton_iteration(orc_ctx *c, const double *y_old, const double *y_cur
     if (nn < 0) return BAD_STEP;
     c->red_nnz = nn;
     orc_dxdt(c, y_cur, c->v, c->dxdt);
-    for (int i = 0; i < nsp; ++i) c->rhs[i] = y_old[i] - y_cur[i] + dt * c->dxdt[i];
+    for (int i = 0; i < nsp; ++i) {
+        if (c->lsolver != ORC_LS_EXACT) {
+            c->rhs[i] = y_old[i] - y_cur[i] + dt * c->dxdt[i];
+            continue;
+        }
+        /* y_old - y_cur + dt S (S the double-double dX/dt), rounded once */
+        double hi = y_old[i], lo = 0.0;
+        dd_add(&hi, &lo, -y_cur[i]);
+        const double p = dt * c->dxdt[i];
+        lo += fma(dt, c->dxdt[i], -p) + dt * c->dxdt_lo[i];
+        dd_add(&hi, &lo, p);
+        c->rhs[i] = hi + lo;
+    }
     int m = 0;
     for (int i = 0; i < nsp; ++i) if (c->mask[i]) c->red_rhs[m++] = c->rhs[i];
     int ierr = solve_reduced(c, st);
@@ -800,7 +999,16 @@ static int newton_iteration(orc_ctx *c, const double *y_old, const double *y_cur
     /* reduced_y_new = reduced_y_cur + reduced_rhs; ddecompac back into y_new */
     m = 0;
     for (int i = 0; i < nsp; ++i)
-        if (c->mask[i]) { y_new[i] = y_cur[i] + c->red_rhs[m]; ++m; }
+        if (c->mask[i]) {
+            if (c->lsolver == ORC_LS_EXACT) {   /* y_cur + (x + xlo), rounded once */
+                double hi = y_cur[i], lo = c->red_xlo[m];
+                dd_add(&hi, &lo, c->red_rhs[m]);
+                y_new[i] = hi + lo;
+            } else {
+                y_new[i] = y_cur[i] + c->red_rhs[m];
+            }
+            ++m;
+        }
     double g = 0.0;
     for (int i = 0; i < nsp; ++i) {
         double d = y_new[i] - y_cur[i];
@@ -898,6 +1106,14 @@ static int bd_step(orc_ctx *c, double *y_k, double *delta_k, double h, int stage
     if (use_ref) {
         int info = 0;
         orc_ref_dgssv(2, n, nnz, 1, f->cv, f->ri, f->cp, c->red_rhs, n, &f->handle, &info);
+    } else if (c->lsolver == ORC_LS_EXACT) {
+        double *b = malloc(sizeof(double) * (n > 0 ? n : 1));
+        memcpy(b, c->red_rhs, sizeof(double) * n);
+        orc_lu_solve(f->lu, c->red_rhs);
+        exact_refine(n, c->red_rowptr, c->red_colind, c->red_val, b, c->red_rhs, c->red_xlo,
+                     solve_gepp_cb, f->lu);
+        for (int i = 0; i < n; ++i) c->red_rhs[i] = c->red_rhs[i] + c->red_xlo[i];
+        free(b);
     } else {
         orc_lu_solve(f->lu, c->red_rhs);
     }

@@ -49,7 +49,11 @@ typedef struct {
 typedef struct orc_ctx orc_ctx;
 typedef struct orc_lu orc_lu;
 
-enum { ORC_LS_GEPP = 0, ORC_LS_REF = 1 };
+/* linear solvers: the oracle GEPP; the reference SuperLU (when opened); ORC_LS_EXACT, the
+ * reference algorithm with its rounding taken out where it is not the algorithm's: each Newton
+ * system solved exactly (quad-precision residual refinement) and dX/dt and the Newton rhs
+ * rounded once from double-double sums - a yardstick for the rounding error of FP64 paths */
+enum { ORC_LS_GEPP = 0, ORC_LS_REF = 1, ORC_LS_EXACT = 2 };
 
 void orc_default_knobs(orc_knobs *k);
 
@@ -90,6 +94,11 @@ int orc_reduce_jacobian(const orc_ctx *c, double *jac, int *row, const int *col,
 
 /* rate vector used by orc_backward_euler / orc_bader_deuflhard (module array rates::v) */
 void orc_set_rates(orc_ctx *c, const double *v);
+
+/* Bader-Deuflhard state carried between calls: iv = frst, reduce, kopt, kmax; dv = errmax,
+ * abuerr, negerr, sumerr, err(1:8) (bader_deuflhard.F90:29-38,152; solver.F90:75) */
+void orc_bd_state(const orc_ctx *c, int *iv, double *dv);
+void orc_bd_set_frst(orc_ctx *c, int frst);
 
 /* One Newton iteration / the backward-Euler step / the BD step / the sub-step controller.
  * ierr codes as utils/source/errors.F90:5-16 (0 = success). */

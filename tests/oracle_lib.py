@@ -21,6 +21,7 @@ _bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 LS_GEPP = 0
 LS_REF = 1
+LS_EXACT = 2   # GEPP + quad-precision residual refinement: the FP64 system solved exactly
 
 
 class Knobs(C.Structure):
@@ -64,6 +65,8 @@ def lib():
     L.orc_set_linear_solver.argtypes = [C.c_void_p, C.c_int]
     L.orc_reset_state.argtypes = [C.c_void_p]
     L.orc_set_rates.argtypes = [C.c_void_p, _dp]
+    L.orc_bd_state.argtypes = [C.c_void_p, _ip, _dp]
+    L.orc_bd_set_frst.argtypes = [C.c_void_p, C.c_int]
     L.orc_nnz_full.argtypes = [C.c_void_p]
     L.orc_full_pattern.argtypes = [C.c_void_p, _ip, _ip, _ip]
     L.orc_rates.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp]
@@ -139,6 +142,8 @@ class Oracle:
                               f64(net.aw), f64(net.zw), 0, C.byref(self.knobs))
         if linear_solver == LS_REF and not ref_available():
             raise RuntimeError("reference SuperLU (oracle/_ref) is not built")
+        if linear_solver == LS_EXACT:
+            ref_available()   # its factors when built (faster), else the oracle GEPP's
         L.orc_set_linear_solver(self.h, linear_solver)
         self.nnz = L.orc_nnz_full(self.h)
 
@@ -194,6 +199,16 @@ class Oracle:
 
     def set_rates(self, v):
         lib().orc_set_rates(self.h, f64(v))
+
+    def bd_state(self):
+        iv = np.zeros(4, np.int32)
+        dv = np.zeros(12)
+        lib().orc_bd_state(self.h, iv, dv)
+        return dict(frst=int(iv[0]), reduce=int(iv[1]), kopt=int(iv[2]), kmax=int(iv[3]),
+                    errmax=dv[0], abuerr=dv[1], negerr=dv[2], sumerr=dv[3], err=dv[4:].copy())
+
+    def bd_set_frst(self, frst):
+        lib().orc_bd_set_frst(self.h, int(frst))
 
     def backward_euler(self, y, dt, v=None):
         if v is not None:

@@ -62,7 +62,10 @@ def test_bd_trajectory_c0(pkg, oracle, hip):
                 (st_o.nsubt, st_o.iter, st_o.nsucc, st_o.nfail)
             assert st_g.n_factor == st_o.n_factor and st_g.n_solve == st_o.n_solve
             assert e <= 1e-10
-            # DEUFL renormalises by the plain sum (solver.F90:234)
+            # Ye of the last step's raw result, before the DEUFL renormalisation by the plain
+            # sum (solver.F90:234, 272)
+            print(f"   ye {ye_g:.15f} / {ye_o:.15f}")
+            assert abs(ye_g - ye_o) <= 1e-12
             assert abs(y_g.sum() - 1.0) <= 1e-14
     finally:
         ctx.close()
@@ -90,6 +93,8 @@ def test_bd_stiff_zone_c4_golden(pkg, hip):
         assert (st.nsubt, st.iter, st.nsucc, st.nfail, st.n_factor, st.n_solve) == \
             tuple(int(g[k]) for k in ("nsubt", "iter", "nsucc", "nfail", "n_factor", "n_solve"))
         assert e <= 1e-10
+        print(f"ye {ye:.15f} / {float(g['ye']):.15f}")
+        assert abs(ye - float(g["ye"])) <= 1e-12
         assert abs(y.sum() - 1.0) <= 1e-14
     finally:
         ctx.close()
@@ -132,6 +137,7 @@ def test_bd_batch_matches_oracle_c1(pkg, oracle, hip):
             assert (st["nsubt"][j], st["iter"][j], st["nsucc"][j], st["nfail"][j]) == \
                 (st_o.nsubt, st_o.iter, st_o.nsucc, st_o.nfail)
             assert e <= 1e-10
+            assert abs(ye[j] - ye_o) <= 1e-12, (ye[j], ye_o)
     finally:
         ctx.close()
 
