@@ -1,3 +1,22 @@
+// This file is a modified version of COLAMD, rewritten in C++ with its own data layout from
+// the COLAMD 2.3 sources vendored by the reference (SuperLU_5.0/SRC/colamd.c).  COLAMD's notice:
+//
+//   COLAMD version 2.3.  The authors of the code itself are Stefan I. Larimore and Timothy A.
+//   Davis (davis@cise.ufl.edu), University of Florida.  The algorithm was developed in
+//   collaboration with John Gilbert, Xerox PARC, and Esmond Ng, Oak Ridge National Laboratory.
+//
+//   Copyright and License:
+//   Copyright (c) 1998-2003 by the University of Florida.  All Rights Reserved.
+//   THIS MATERIAL IS PROVIDED AS IS, WITH ABSOLUTELY NO WARRANTY EXPRESSED OR IMPLIED.  ANY USE
+//   IS AT YOUR OWN RISK.
+//   Permission is hereby granted to use, copy, modify, and/or distribute this program,
+//   provided that the Copyright, this License, and the Availability of the original version is
+//   retained on all copies and made accessible to the end-user of any code or package that
+//   includes COLAMD or any modified version of COLAMD.
+//
+//   Availability:
+//   The colamd/symamd library is available at http://www.cise.ufl.edu/research/sparse/colamd/
+//
 // Column approximate minimum degree ordering with the exact step and tie-break semantics of
 // COLAMD 2.3 as vendored by the reference (SuperLU_5.0/SRC/colamd.c, colamd.h:19-21), so that
 // the column order - and therefore SuperLU's pivot sequence - is reproduced bit for bit.

@@ -234,7 +234,39 @@ struct nuppn_ctx {
     long long newton_launches = 0;
     long long last_passes = 0;
     std::mutex mu;
+    ~nuppn_ctx();
 };
+
+// frees every device and pinned allocation (also on the error returns of nuppn_ctx_create,
+// whose unique_ptr deletes a half-built context)
+nuppn_ctx::~nuppn_ctx() {
+    (void) hipDeviceSynchronize();
+    for (void *p : {(void *) d_k, (void *) d_rtype, (void *) d_red2sp, (void *) d_dx_ptr,
+                    (void *) d_dx_ent, (void *) d_jq_ptr, (void *) d_jq_ent, (void *) d_Acp,
+                    (void *) d_Ari, (void *) d_perm_c, (void *) d_iperm_c, (void *) d_isdiag,
+                    (void *) d_coef, (void *) d_aw, (void *) d_zw, (void *) d_slots,
+                    (void *) d_flag, (void *) d_yps_old, (void *) d_y_old, (void *) d_y_cur,
+                    (void *) d_y_new, (void *) d_work, (void *) d_state, (void *) d_bd,
+                    (void *) d_active, (void *) d_yps, (void *) d_t9_0, (void *) d_t9_1,
+                    (void *) d_rho_0, (void *) d_rho_1, (void *) d_dt, (void *) d_ye, (void *) d_v,
+                    (void *) d_ierr, (void *) d_stats, (void *) d_status, (void *) d_b_vpos,
+                    (void *) d_b_diagk, (void *) d_b_ent_ptr, (void *) d_b_ent_k,
+                    (void *) d_b_ent_loc, (void *) d_b_ent_col, (void *) d_b_csr_col,
+                    (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
+                    (void *) d_b_ent_dst, (void *) d_jac_lo})
+        if (p) (void) hipFree(p);
+    for (void *p : d_seg_i)
+        if (p) (void) hipFree(p);
+    for (void *p : d_seg_c)
+        if (p) (void) hipFree(p);
+    for (auto &e : ev)
+        if (e) (void) hipEventDestroy(e);
+    for (auto &e : evn)
+        if (e) (void) hipEventDestroy(e);
+    for (auto &e : evc)
+        if (e) (void) hipEventDestroy(e);
+    if (h_flag) (void) hipHostFree(h_flag);
+}
 
 static DevNet dev_net(const nuppn_ctx *c) {
     DevNet N;
@@ -675,37 +707,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     return c.release();
 }
 
-void nuppn_ctx_destroy(nuppn_ctx *c) {
-    if (!c) return;
-    (void) hipDeviceSynchronize();
-    for (void *p : {(void *) c->d_k, (void *) c->d_rtype, (void *) c->d_red2sp, (void *) c->d_dx_ptr,
-                    (void *) c->d_dx_ent, (void *) c->d_jq_ptr, (void *) c->d_jq_ent,
-                    (void *) c->d_Acp, (void *) c->d_Ari, (void *) c->d_perm_c,
-                    (void *) c->d_iperm_c, (void *) c->d_isdiag, (void *) c->d_coef,
-                    (void *) c->d_aw, (void *) c->d_zw, (void *) c->d_slots, (void *) c->d_flag,
-                    (void *) c->d_yps_old, (void *) c->d_y_old, (void *) c->d_y_cur,
-                    (void *) c->d_y_new, (void *) c->d_work, (void *) c->d_state, (void *) c->d_bd,
-                    (void *) c->d_active, (void *) c->d_yps, (void *) c->d_t9_0, (void *) c->d_t9_1,
-                    (void *) c->d_rho_0, (void *) c->d_rho_1, (void *) c->d_dt, (void *) c->d_ye,
-                    (void *) c->d_v, (void *) c->d_ierr, (void *) c->d_stats,
-                    (void *) c->d_status, (void *) c->d_b_vpos, (void *) c->d_b_diagk,
-                    (void *) c->d_b_ent_ptr, (void *) c->d_b_ent_k, (void *) c->d_b_ent_loc,
-                    (void *) c->d_b_ent_col, (void *) c->d_b_csr_col, (void *) c->d_b_isdiag,
-                    (void *) c->d_b_mass, (void *) c->d_b_ent_ys, (void *) c->d_bslots,
-                    (void *) c->d_b_ent_dst})
-        dev_free(p);
-    for (void *p : c->d_seg_i) dev_free(p);
-    for (void *p : c->d_seg_c) dev_free(p);
-    dev_free(c->d_jac_lo);
-    for (auto &e : c->ev)
-        if (e) (void) hipEventDestroy(e);
-    for (auto &e : c->evn)
-        if (e) (void) hipEventDestroy(e);
-    for (auto &e : c->evc)
-        if (e) (void) hipEventDestroy(e);
-    if (c->h_flag) (void) hipHostFree(c->h_flag);
-    delete c;
-}
+void nuppn_ctx_destroy(nuppn_ctx *c) { delete c; }
 
 int nuppn_set_column_order(nuppn_ctx *c, const int *perm_c) {
     if (!c || !perm_c) return NUPPN_ERR_ARG;
@@ -995,15 +997,13 @@ int nuppn_last_timing(nuppn_ctx *c, double *out, int nout) {
     return NUPPN_OK;
 }
 
-int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
-                          const double *t9_1, const double *rho_0, const double *rho_1, double *ye,
-                          const double *dtwant, int *ierr, nuppn_zone_stats *stats) {
-    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
-        return NUPPN_ERR_ARG;
-    {
-        std::lock_guard<std::mutex> g(c->mu);
-        if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
-    }
+// A host-array batch call: the staging buffers are filled, the batch run and the results read
+// back under the context's lock, so that concurrent calls on one context serialise whole.
+static int host_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, const double *t9_1,
+                      const double *rho_0, const double *rho_1, double *ye, const double *dtwant,
+                      int *ierr, nuppn_zone_stats *stats, const nuppn_batch_opts *bo) {
+    std::lock_guard<std::mutex> g(c->mu);
+    if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
     const size_t nsp = c->net.nsp;
     HIP_OK(hipMemcpy(c->d_yps, yps, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_t9_0, t9_0, sizeof(double) * nz, hipMemcpyHostToDevice));
@@ -1012,9 +1012,9 @@ int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
     HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_ye, ye, sizeof(double) * nz, hipMemcpyHostToDevice));
-    int rc = nuppn_integrate_batch_dev(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
-                                       c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr,
-                                       (nuppn_zone_stats *) c->d_stats, nullptr);
+    const int rc = run_batch(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0, c->d_rho_1,
+                             c->d_ye, c->d_dt, c->d_ierr, (nuppn_zone_stats *) c->d_stats, bo,
+                             nullptr);
     if (rc) return rc;
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(yps, c->d_yps, sizeof(double) * nsp * nz, hipMemcpyDeviceToHost));
@@ -1023,6 +1023,23 @@ int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
     if (stats)
         HIP_OK(hipMemcpy(stats, c->d_stats, sizeof(ZoneStats) * nz, hipMemcpyDeviceToHost));
     return NUPPN_OK;
+}
+
+// the batched path's options, checked (defaults for null)
+static int mixed_opts(const nuppn_batch_opts *opts, nuppn_batch_opts *o) {
+    if (opts) *o = *opts;
+    else nuppn_default_batch_opts(o);
+    if ((o->factor_precision != 32 && o->factor_precision != 64) || o->max_refine < 1)
+        return NUPPN_ERR_ARG;
+    return NUPPN_OK;
+}
+
+int nuppn_integrate_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0,
+                          const double *t9_1, const double *rho_0, const double *rho_1, double *ye,
+                          const double *dtwant, int *ierr, nuppn_zone_stats *stats) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
+        return NUPPN_ERR_ARG;
+    return host_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, nullptr);
 }
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o) {
@@ -1049,10 +1066,7 @@ int nuppn_integrate_batch_mixed_dev(nuppn_ctx *c, int nz, double *yps, const dou
         return NUPPN_ERR_ARG;
     if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     nuppn_batch_opts o;
-    if (opts) o = *opts;
-    else nuppn_default_batch_opts(&o);
-    if ((o.factor_precision != 32 && o.factor_precision != 64) || o.max_refine < 1)
-        return NUPPN_ERR_ARG;
+    if (mixed_opts(opts, &o)) return NUPPN_ERR_ARG;
     std::lock_guard<std::mutex> g(c->mu);
     return run_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, &o,
                      (hipStream_t) stream);
@@ -1065,29 +1079,9 @@ int nuppn_integrate_batch_mixed(nuppn_ctx *c, int nz, double *yps, const double 
     if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !ye || !dtwant || !ierr)
         return NUPPN_ERR_ARG;
     if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
-    {
-        std::lock_guard<std::mutex> g(c->mu);
-        if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
-    }
-    const size_t nsp = c->net.nsp;
-    HIP_OK(hipMemcpy(c->d_yps, yps, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_t9_0, t9_0, sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_t9_1, t9_1, sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_rho_0, rho_0, sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->d_ye, ye, sizeof(double) * nz, hipMemcpyHostToDevice));
-    int rc = nuppn_integrate_batch_mixed_dev(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
-                                             c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr,
-                                             (nuppn_zone_stats *) c->d_stats, opts, nullptr);
-    if (rc) return rc;
-    HIP_OK(hipDeviceSynchronize());
-    HIP_OK(hipMemcpy(yps, c->d_yps, sizeof(double) * nsp * nz, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(ye, c->d_ye, sizeof(double) * nz, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(ierr, c->d_ierr, sizeof(int) * nz, hipMemcpyDeviceToHost));
-    if (stats)
-        HIP_OK(hipMemcpy(stats, c->d_stats, sizeof(ZoneStats) * nz, hipMemcpyDeviceToHost));
-    return NUPPN_OK;
+    nuppn_batch_opts o;
+    if (mixed_opts(opts, &o)) return NUPPN_ERR_ARG;
+    return host_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, &o);
 }
 
 int nuppn_integrate_zone(nuppn_ctx *c, double *yps, double t9_0, double t9_1, double rho_0,
@@ -1155,7 +1149,10 @@ int dropin_factor(int n, const int *colptr, const int *rowind, const double *val
     if (f->lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) lu_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) f->lds);
-    if (ensure_stack((const void *) lu_kernel)) return NUPPN_ERR_HIP;
+    if (ensure_stack((const void *) lu_kernel)) {
+        dropin_free(f);
+        return NUPPN_ERR_HIP;
+    }
     hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), f->lds, 0, n, f->d_Acp, f->d_Ari, f->d_val,
                        f->d_perm_c, f->d_iperm_c, 0, (double *) nullptr, f->st, f->d_perm_r,
                        f->d_Lp, f->d_info, 1);

@@ -147,7 +147,17 @@ int build_symbolic(const HostNetwork &net, Symbolic &s) {
 
 // ------------------------------------------------------------------------------------------
 // Column elimination tree (Liu's algorithm on the row cliques of A'A replaced by stars at each
-// row's first column; disjoint sets with path halving) - SuperLU_5.0/SRC/sp_coletree.c
+// row's first column; disjoint sets with path halving) - SuperLU_5.0/SRC/sp_coletree.c, of which
+// column_etree and dset_find below are a modified version (rewritten in C++).  That file's notice:
+//   Copyright (c) 1994 by Xerox Corporation.  All rights reserved.
+//   THIS MATERIAL IS PROVIDED AS IS, WITH ABSOLUTELY NO WARRANTY EXPRESSED OR IMPLIED.  ANY USE
+//   IS AT YOUR OWN RISK.
+//   Permission is hereby granted to use or copy this program for any purpose, provided the
+//   above notices are retained on all copies.  Permission to modify the code and to distribute
+//   modified code is granted, provided the above notices are retained, and a notice that the
+//   code was modified is included with the above copyright notice.
+//   (SuperLU routine, version 3.1: Univ. of California Berkeley, Xerox Palo Alto Research
+//   Center, and Lawrence Berkeley National Lab.)  This code was modified.
 
 namespace {
 inline int dset_find(std::vector<int> &pp, int i) {
