@@ -207,6 +207,16 @@ void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int coli
 void c_fortran_dgssv_(int *iopt, int *n, int *nnz, int *nrhs, double *values, int *rowind,
                       int *colptr, double *b, int *ldb, long long *f_factors, int *info);
 
+/* slu_solve without device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, index shift, results and error codes as slu_solve;
+ * every device buffer of the call lives in one fixed 48 MiB array per GPU (held until the process
+ * ends), which bounds the system to n up to about 2,000 (dense-capacity factors); larger systems
+ * return *ierr = NUPPN_ERR_UNSUPPORTED without touching the caller's arrays.  Calls are serialised
+ * by a process-wide lock (they share the array); host-side ordering work uses host memory. */
+void slu_solve_noalloc(int n, int nnz, int isrow, int shiftindex, double val[], int colind[],
+                       int rowptr[], double rhs[], int *ierr);
+
 #ifdef __cplusplus
 }
 #endif

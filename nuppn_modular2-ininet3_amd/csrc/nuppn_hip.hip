@@ -1233,6 +1233,149 @@ void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int coli
     dropin_free(f);
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free slu_solve: every device buffer of the call (the CSC pattern and values, the
+// column order, the factors at dense capacity, the right-hand side) is carved out of one
+// file-scope device array.  Its device address differs per GPU and is resolved once per device
+// with hipGetSymbolAddress; host code uses only that address, never the array's name.
+
+constexpr size_t kNoallocScratchBytes = 48ull << 20;   // 48 MiB per GPU, held until exit
+static __device__ __align__(256) char scratch[kNoallocScratchBytes];
+
+namespace {
+
+std::mutex g_noalloc_mu;            // one call at a time per process (the array is shared)
+std::vector<char *> g_noalloc_ptr;  // device address of `scratch`, per device
+
+int noalloc_scratch(char **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return NUPPN_ERR_HIP;
+    if ((size_t) dev >= g_noalloc_ptr.size()) g_noalloc_ptr.resize(dev + 1, nullptr);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "nuppn_hip: hipGetSymbolAddress(scratch) failed: %s\n",
+                    hipGetErrorString(e));
+            return NUPPN_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = (char *) p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return NUPPN_OK;
+}
+
+// byte offsets of the call's buffers in the scratch array (each 256-aligned)
+struct NoallocLayout {
+    size_t cp, ri, val, pc, ipc, pr, lp, info, b, li, lx, ui, ux, up, bytes;
+};
+
+NoallocLayout noalloc_layout(int n, int nnz) {
+    const size_t capL = (size_t) n * (n - 1) / 2 + 1, capU = (size_t) n * (n + 1) / 2 + 1;
+    const size_t len[14] = {4ull * (n + 1), 4ull * nnz, 8ull * nnz, 4ull * n, 4ull * n, 4ull * n,
+                            4ull * (n + 1), 16, 8ull * n, 4 * capL, 8 * capL, 4 * capU,
+                            8 * capU, 4ull * (n + 1)};
+    size_t off[14], o = 0;
+    for (int i = 0; i < 14; ++i) {
+        off[i] = o;
+        o += (len[i] + 255) & ~(size_t) 255;
+    }
+    return {off[0], off[1], off[2], off[3], off[4], off[5], off[6], off[7], off[8], off[9],
+            off[10], off[11], off[12], off[13], o};
+}
+
+}  // namespace
+
+extern "C" {
+
+void slu_solve_noalloc(int n, int nnz, int isrow, int shiftindex, double val[], int colind[],
+                       int rowptr[], double rhs[], int *ierr) {
+    if (!ierr) return;
+    if (n <= 0 || nnz < 0 || !val || !colind || !rowptr || !rhs) {
+        *ierr = -1;
+        return;
+    }
+    const NoallocLayout L = noalloc_layout(n, nnz);
+    if (L.bytes > kNoallocScratchBytes || lds_bytes(n) > 160 * 1024) {
+        fprintf(stderr, "nuppn_hip: slu_solve_noalloc: n = %d, nnz = %d needs %zu bytes of "
+                "device scratch, more than its fixed %zu (n up to about 2,000)\n", n, nnz,
+                L.bytes, kNoallocScratchBytes);
+        *ierr = NUPPN_ERR_UNSUPPORTED;
+        return;
+    }
+    if (shiftindex) {   // in place, as slu_f90.c:11-15 (the Fortran caller restores them)
+        for (int i = 0; i <= n; ++i) rowptr[i] -= 1;
+        for (int i = 0; i < nnz; ++i) colind[i] -= 1;
+    }
+    std::vector<int> cp(n + 1, 0), ri(nnz);
+    std::vector<double> cv(nnz);
+    if (isrow) {   // dCompRow_to_CompCol (dutil.c:154): rows ascending within each column
+        for (int p = 0; p < nnz; ++p) cp[colind[p] + 1]++;
+        for (int i = 0; i < n; ++i) cp[i + 1] += cp[i];
+        std::vector<int> pos(cp.begin(), cp.end() - 1);
+        for (int r = 0; r < n; ++r)
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+                const int q = pos[colind[p]]++;
+                ri[q] = r;
+                cv[q] = val[p];
+            }
+    } else {
+        std::copy(rowptr, rowptr + n + 1, cp.begin());
+        std::copy(colind, colind + nnz, ri.begin());
+        std::copy(val, val + nnz, cv.begin());
+    }
+    std::vector<int> perm_c;
+    if (superlu_column_order(n, cp.data(), ri.data(), perm_c) != 0) {
+        *ierr = -1;
+        return;
+    }
+    std::vector<int> iperm(n);
+    for (int i = 0; i < n; ++i) iperm[perm_c[i]] = i;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    char *sp = nullptr;
+    if (noalloc_scratch(&sp)) {
+        *ierr = NUPPN_ERR_HIP;
+        return;
+    }
+    auto at = [&](size_t off) { return (void *) (sp + off); };
+    const size_t lds = lds_bytes(n);
+    if ((lds > 64 * 1024 &&
+         hipFuncSetAttribute((const void *) lu_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int) lds) != hipSuccess) ||
+        hipMemcpy(at(L.cp), cp.data(), 4ull * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && hipMemcpy(at(L.ri), ri.data(), 4ull * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+        (nnz && hipMemcpy(at(L.val), cv.data(), 8ull * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(at(L.pc), perm_c.data(), 4ull * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(at(L.ipc), iperm.data(), 4ull * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(at(L.b), rhs, 8ull * n, hipMemcpyHostToDevice) != hipSuccess) {
+        *ierr = NUPPN_ERR_HIP;
+        return;
+    }
+    LUStore st;
+    st.Li = (int *) at(L.li);
+    st.Lx = (double *) at(L.lx);
+    st.Ui = (int *) at(L.ui);
+    st.Ux = (double *) at(L.ux);
+    st.Up = (int *) at(L.up);
+    hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), lds, 0, n, (const int *) at(L.cp),
+                       (const int *) at(L.ri), (const double *) at(L.val), (const int *) at(L.pc),
+                       (const int *) at(L.ipc), 1, (double *) at(L.b), st, (int *) at(L.pr),
+                       (int *) at(L.lp), (int *) at(L.info), 1);
+    int info = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(&info, at(L.info), sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        *ierr = NUPPN_ERR_HIP;
+        return;
+    }
+    if (info == 0 && hipMemcpy(rhs, at(L.b), 8ull * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        *ierr = NUPPN_ERR_HIP;
+        return;
+    }
+    *ierr = info;
+}
+
 void c_fortran_dgssv_(int *iopt, int *n, int *nnz, int *nrhs, double *values, int *rowind,
                       int *colptr, double *b, int *ldb, long long *f_factors, int *info) {
     if (!iopt || !info) return;

@@ -110,6 +110,7 @@ def load():
                                                   [C.POINTER(BatchOpts), C.c_void_p])
     L.slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                             C.POINTER(C.c_int)]
+    L.slu_solve_noalloc.argtypes = L.slu_solve.argtypes
     L.c_fortran_dgssv_.argtypes = [C.POINTER(C.c_int)] * 4 + [_dp, _ip, _ip, _dp,
                                                                C.POINTER(C.c_int),
                                                                C.POINTER(C.c_longlong),
@@ -322,6 +323,16 @@ def slu_solve(val, colind, rowptr, rhs, isrow=1, shiftindex=1):
     n = rhs.shape[0]
     ierr = C.c_int(0)
     load().slu_solve(n, val.shape[0], isrow, shiftindex, val, colind, rowptr, rhs, C.byref(ierr))
+    return ierr.value
+
+
+def slu_solve_noalloc(val, colind, rowptr, rhs, isrow=1, shiftindex=1):
+    """slu_solve_noalloc: slu_solve with every device buffer in the library's fixed scratch
+    array (no device allocation); the same arguments and results."""
+    n = rhs.shape[0]
+    ierr = C.c_int(0)
+    load().slu_solve_noalloc(n, val.shape[0], isrow, shiftindex, val, colind, rowptr, rhs,
+                             C.byref(ierr))
     return ierr.value
 
 
