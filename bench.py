@@ -35,10 +35,12 @@ def _args():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--zones", type=int, default=2048)
     p.add_argument("--plain-sample", type=int, default=48,
-                   help="zones (T9 < 0.25) on which the plain FP64 path is timed")
+                   help="zones (stratified in T9 below 0.5) on which the plain FP64 path is timed")
     p.add_argument("--cpu-sample", type=int, default=32, help="zones in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-uneven", action="store_true",
+                   help="skip the one-cycle run of the uneven 8,192-zone workload")
     p.add_argument("--save", default="", help="write the step's abundances and counters (npz)")
     return p.parse_args()
 
@@ -86,11 +88,17 @@ def block_bytes(net, ctx, st):
 
 def cpu_baseline(sample_idx, cores):
     """Oracle integrate_network + reference SuperLU on the sampled zones, one process per core
-    (the reference's one-zone-per-worker model)."""
+    (the reference's one-zone-per-worker model).  Run before anything touches the GPU, so the
+    forked workers hold no device state; the pool is shut down before the GPU part starts."""
+    import multiprocessing
     from concurrent.futures import ProcessPoolExecutor
     t = time.time()
-    with ProcessPoolExecutor(max_workers=cores, initializer=_cpu_init) as ex:
+    ex = ProcessPoolExecutor(max_workers=cores, initializer=_cpu_init,
+                             mp_context=multiprocessing.get_context("fork"))
+    try:
         results = list(ex.map(_cpu_zone, [int(i) for i in sample_idx]))
+    finally:
+        ex.shutdown(wait=True)
     dt = time.time() - t
     ok = sum(1 for r, _ in results if r == 0)
     return {"value": len(sample_idx) / dt, "unit": "zone-timesteps/s", "cores": cores,
@@ -141,6 +149,21 @@ def main():
               f"zones across GPUs (one contiguous block of zones per GPU), which this one-GPU "
               f"build does not run", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline first: no GPU state exists yet in this process or its workers
+    cpu = None
+    cpu_one = None
+    if not a.no_cpu:
+        # the GPU box's CPU share is 16 cores; never more workers than that
+        cores = min(16, os.cpu_count() or 1)
+        idx = np.linspace(0, a.zones - 1, min(a.cpu_sample, a.zones)).round().astype(int)
+        cpu = cpu_baseline(idx, cores)
+        cpu["sample"] += f"; host CPU: {_cpu_model()}, {os.cpu_count()} logical CPUs visible"
+        log(f"cpu baseline: {cpu['value']:.2f} zone-timesteps/s on {cores} cores")
+        # one core: the same zones back to back, from each zone's own time in the run above
+        cpu_one = {"value": len(idx) / cpu.pop("_zone_seconds"), "unit": "zone-timesteps/s",
+                   "sample": "the same zones, summed per-zone times of the run above"}
+        log(f"cpu baseline, one core: {cpu_one['value']:.2f} zone-timesteps/s")
+
     import torch
     from __graft_entry__ import _load_package
     pkg = _load_package()
@@ -217,14 +240,18 @@ def main():
     # --- plain FP64 path (reference pivot order) on a bounded sample of light zones
     plain = None
     if not a.no_plain:
-        light = np.nonzero(w.t9[:, 0] < 0.25)[0]
-        sel = light[np.linspace(0, light.size - 1, min(a.plain_sample, light.size)).round()
-                    .astype(int)]
+        # stratified: zones from the cold end up to T9 0.5 (the plain path's one-wavefront GEPP
+        # takes about 0.1 s per Newton iteration, so a hot zone's 700 iterations take minutes)
+        order = np.argsort(w.t9[:, 0])
+        warm = order[w.t9[order, 0] < 0.5]
+        sel = np.sort(warm[np.linspace(0, warm.size - 1, min(a.plain_sample, warm.size)).round()
+                           .astype(int)])
         elp, yp, iep, stp, _ = run("plain", sel)
         elb, yb, ieb, stb, _ = run("mixed", sel)
         mm = (yp > 1e-20) | (yb > 1e-20)
-        plain = {"zones": int(len(sel)), "selection": "T9 < 0.25 (the plain path's per-iteration "
-                 "latency makes hot zones take minutes)",
+        plain = {"zones": int(len(sel)), "selection": "stratified in T9 over the zones with "
+                 f"T9 < 0.5 (T9 {w.t9[sel, 0].min():.3f} .. {w.t9[sel, 0].max():.3f}; the plain "
+                 "path's per-iteration latency makes the hottest zones take minutes)",
                  "plain_zone_timesteps_per_s": len(sel) / elp,
                  "mixed_zone_timesteps_per_s": len(sel) / elb, "mixed_over_plain": elp / elb,
                  "max_rel_err_mixed_vs_plain": float(np.max(np.abs(yp[mm] - yb[mm]) /
@@ -233,18 +260,48 @@ def main():
                                         np.array_equal(stp["n_newton"], stb["n_newton"]))}
         log(f"plain sample: {elp * 1e3:.1f} ms vs mixed {elb * 1e3:.1f} ms on {len(sel)} zones")
 
-    cpu = None
-    cpu_one = None
-    if not a.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        idx = np.linspace(0, nz - 1, min(a.cpu_sample, nz)).round().astype(int)
-        cpu = cpu_baseline(idx, cores)
-        cpu["sample"] += f"; host CPU: {_cpu_model()}"
-        log(f"cpu baseline: {cpu['value']:.2f} zone-timesteps/s on {cores} cores")
-        # one core: the same zones back to back, from each zone's own time in the run above
-        cpu_one = {"value": len(idx) / cpu.pop("_zone_seconds"), "unit": "zone-timesteps/s",
-                   "sample": "the same zones, summed per-zone times of the run above"}
-        log(f"cpu baseline, one core: {cpu_one['value']:.2f} zone-timesteps/s")
+    # --- the uneven 8,192-zone workload (same network; T9 0.02..4, dt 1..1e8 s), one cycle
+    uneven = None
+    if not a.no_uneven:
+        w2 = pkg.network.config(2)
+        n2 = w2.nz
+        y2 = tens(w2.x0)
+        a2 = [tens(x) for x in (w2.t9[:, 0], w2.t9[:, 1], w2.rho[:, 0], w2.rho[:, 1], w2.dt)]
+        e2 = torch.full((n2,), 0.5, dtype=torch.float64, device=dev)
+        i2 = torch.zeros(n2, dtype=torch.int32, device=dev)
+        s2 = torch.zeros((n2, nstat), dtype=torch.int64, device=dev)
+        res = {}
+        for path in ("mixed", "fp64"):
+            y2.copy_(tens(w2.x0))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.integrate_batch_mixed_dev(n2, y2.data_ptr(), *(t.data_ptr() for t in a2[:4]),
+                                          e2.data_ptr(), a2[4].data_ptr(), i2.data_ptr(),
+                                          s2.data_ptr(), opts=hip.batch_opts(
+                                              factor_precision=32 if path == "mixed" else 64))
+            torch.cuda.synchronize()
+            el2 = time.perf_counter() - t0
+            st2 = np.frombuffer(s2.cpu().numpy().tobytes(), dtype=hip.STATS_DTYPE)
+            res[path] = (el2, y2.cpu().numpy(), i2.cpu().numpy(), st2)
+            log(f"uneven 8,192-zone cycle ({path}): {el2 * 1e3:.1f} ms")
+        el2, yy, ii, st2 = res["mixed"]
+        y64u, st64u = res["fp64"][1], res["fp64"][3]
+        mu = (yy > 1e-20) | (y64u > 1e-20)
+        relu = np.where(mu, np.abs(yy - y64u) / np.maximum(np.maximum(yy, y64u), 1e-300), 0.0)
+        sameu = (st2["nsubt"] == st64u["nsubt"]) & (st2["n_newton"] == st64u["n_newton"])
+        uneven = {"zones": n2, "zone_timesteps_per_s_mixed": n2 / el2,
+                  "zone_timesteps_per_s_fp64_factors": n2 / res["fp64"][0],
+                  "mixed_equals_fp64_bitwise": bool(np.array_equal(yy, y64u)),
+                  "zones_not_bitwise_equal": int((yy != y64u).any(axis=1).sum()),
+                  "max_rel_err_mixed_vs_fp64_factors": float(relu.max()),
+                  "zones_rel_err_above_1e-8": int((relu.max(axis=1) > 1e-8).sum()),
+                  "zones_same_decisions_as_fp64": int(sameu.sum()),
+                  "ierr_nonzero": int((ii != 0).sum()),
+                  "max_abs_sumX_minus_1": float(np.abs(yy.sum(axis=1) - 1.0).max()),
+                  "max_sub_steps": int(st2["nsubt"].max()),
+                  "max_newton_per_zone": int(st2["n_newton"].max()),
+                  "zones_on_fp64": int((st2["n_factor_fp64"] > 0).sum())}
+        del y2, a2, e2, i2, s2
 
     if a.save:
         np.savez(a.save, y_mixed=y_mixed, y_fp64=y64, nsubt_mixed=st_mixed["nsubt"],
@@ -296,6 +353,7 @@ def main():
             "newton_ms": tm["newton_ms"], "control_ms": tm["control_ms"],
             "passes": tm["passes"], "algorithmic_bytes_per_step": bytes_step,
             "plain_sample": plain,
+            "uneven_8192_zones_one_cycle": uneven,
             "cpu_one_core": None if cpu_one is None else {
                 "value": cpu_one["value"], "unit": cpu_one["unit"], "sample": cpu_one["sample"]},
         },
