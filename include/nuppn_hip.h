@@ -115,6 +115,16 @@ int nuppn_reduced_pattern(nuppn_ctx *ctx, int *colptr, int *rowind, int *red2sp)
 int nuppn_rates_eval(nuppn_ctx *ctx, int nz, const double *t9, const double *rho,
                      const double *ye, double *v);
 
+/* The REACLIB T9 cache (physics knob use_cache, physics/source/physics_knobs.F90:34): on the
+ * first switch-on, reaclib_preprocessor (reaclib.F90:609-646) tabulates every rate at rho = 1 on
+ * 5,001 points T9 = 0.01 .. 10 on the device; while it is on, every rate evaluation of the
+ * context (nuppn_rates_eval and the integration entry points) interpolates the table linearly
+ * (reaclib.F90:446-470) instead of evaluating the formula.  use_cache 0 switches back. */
+int nuppn_set_rate_cache(nuppn_ctx *ctx, int use_cache);
+/* the table, [5001 * nre] (row i: T9 = 0.01 + i (10 - 0.01) / 5000); NUPPN_ERR_UNSUPPORTED if
+ * it was never built */
+int nuppn_rate_cache_table(nuppn_ctx *ctx, double *table);
+
 /* dX/dt (calculate_dxdt, jac_rhs.F90:217-278) for one state; x, dxdt: [nsp], v: [nre]. */
 int nuppn_dxdt(nuppn_ctx *ctx, const double *x, const double *v, double *dxdt);
 

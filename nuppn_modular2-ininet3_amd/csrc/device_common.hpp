@@ -168,7 +168,41 @@ struct DevNet {
     const unsigned char *A_isdiag;  // [nnzA]
     const int *perm_c;      // [n] SuperLU convention
     const int *iperm_c;     // [n]
+    // the REACLIB T9 cache (null when the rates are evaluated directly): [kNT9][nre] rates at
+    // rho = 1 on the grid rc_t9 [kNT9] of spacing rc_dt9
+    const double *rcache;
+    const double *rc_t9;
+    double rc_dt9;
 };
+
+// num_t9_preproc (physics/source/reaclib.F90:155): the cache's grid, T9 = 0.01 .. 10
+constexpr int kNT9 = 5001;
+
+// a rate from the T9 cache (reaclib.F90:443-470): the grid interval of max(t9, 0.01)
+// (floor((t9 - 0.01) / dt9), clamped to the table), linear interpolation, x rho for the
+// two-body rates, then the floors of reaclib.F90:581 and evaluate_rates.F90:46,172
+struct CachePos {
+    int i1;
+    double f;
+};
+
+__device__ __forceinline__ CachePos cache_pos(const DevNet &N, double t9in) {
+    const double t9 = fmax(t9in, 0.01);
+    int i1 = (int) floor((t9 - 0.01) / N.rc_dt9);
+    i1 = max(0, min(i1, kNT9 - 2));
+    return {i1, (t9 - N.rc_t9[i1]) / N.rc_dt9};
+}
+
+__device__ __forceinline__ double cached_rate(const DevNet &N, int l, int rtype, CachePos p,
+                                              double rho) {
+    const double *c1 = N.rcache + (size_t) p.i1 * N.nre;
+    const double a = c1[l], b = c1[N.nre + l];
+    double rv = a + p.f * (b - a);
+    if (rtype == 0) rv = rv * rho;
+    rv = fmax(rv, 1.0e-99);
+    rv = fmax(rv * 1.0, 1.0e-99);
+    return fmax(rv, 1.0e-99);
+}
 
 // Pointers reach the device code as generic pointers (struct fields, arguments of functions
 // that are not inlined), so the compiler emits flat memory instructions for them.  A flat load

@@ -104,10 +104,16 @@ __device__ __forceinline__ double ypsplus_value(double t) {
 }
 
 
-// rates of one state into v (one wavefront)
+// rates of one state into v (one wavefront): the REACLIB formula, or the T9 cache when it is on
 __device__ inline void zone_rates(const DevNet &N, double t9, double rho, double *v) {
-    const T9Pow p = t9_powers(t9);
-    for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho);
+    if (N.rcache) {
+        const CachePos c = cache_pos(N, t9);
+        for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = cached_rate(N, l, N.rtype[l], c, rho);
+    } else {
+        const T9Pow p = t9_powers(t9);
+        for (int l = lane_id(); l < N.nre; l += WAVE)
+            v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho);
+    }
     wave_sync();
 }
 

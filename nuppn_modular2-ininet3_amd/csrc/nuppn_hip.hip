@@ -44,21 +44,54 @@ using namespace nuppn;
 // ------------------------------------------------------------------------------------------
 // kernels
 
-__global__ void __launch_bounds__(256) rates_kernel(const double *__restrict__ coef,
-                                                    const int *__restrict__ rtype, int nre, int nz,
+// reactions per thread of the rate kernels: the zone's T9 powers (a pow, a log, two divisions)
+// or cache position are computed once per thread and serve RATE_PER_THREAD rates
+constexpr int RATE_PER_THREAD = 8;
+
+__global__ void __launch_bounds__(256) rates_kernel(DevNet N, int nz,
                                                     const double *__restrict__ t9_0,
                                                     const double *__restrict__ t9_1,
                                                     const double *__restrict__ rho_0,
                                                     const double *__restrict__ rho_1,
                                                     double *__restrict__ v) {
     const int z = blockIdx.y;
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= nz || l >= nre) return;
+    if (z >= nz) return;
+    const int nre = N.nre;
     // evaluate_all_rates at the cycle mid-point (ppn.F90:287-298, mppnp.F90:719)
     const double t9 = 0.5 * (t9_0[z] + t9_1[z]);
     const double rho = 0.5 * (rho_0[z] + rho_1[z]);
-    const T9Pow p = t9_powers(t9);
-    v[(size_t) z * nre + l] = reaclib_rate(coef, nre, l, rtype[l], p, rho);
+    double *vz = v + (size_t) z * nre;
+    const int l0 = blockIdx.x * blockDim.x * RATE_PER_THREAD + threadIdx.x;
+    if (N.rcache) {
+        const CachePos c = cache_pos(N, t9);
+#pragma unroll
+        for (int r = 0; r < RATE_PER_THREAD; ++r) {
+            const int l = l0 + r * blockDim.x;
+            if (l < nre) vz[l] = cached_rate(N, l, N.rtype[l], c, rho);
+        }
+    } else {
+        const T9Pow p = t9_powers(t9);
+#pragma unroll
+        for (int r = 0; r < RATE_PER_THREAD; ++r) {
+            const int l = l0 + r * blockDim.x;
+            if (l < nre) vz[l] = reaclib_rate(N.coef, nre, l, N.rtype[l], p, rho);
+        }
+    }
+}
+
+// reaclib_preprocessor (physics/source/reaclib.F90:609-646): the rates at rho = 1 on the T9
+// grid, each floored at 1e-99 (the formula's own floors)
+__global__ void __launch_bounds__(256) rate_cache_kernel(DevNet N, const double *t9grid,
+                                                         double *cache) {
+    const int i = blockIdx.y;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kNT9 || l >= N.nre) return;
+    const T9Pow p = t9_powers(t9grid[i]);
+    cache[(size_t) i * N.nre + l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, 1.0);
+}
+
+static dim3 rates_grid(int nre, int nz) {
+    return dim3((nre + 256 * RATE_PER_THREAD - 1) / (256 * RATE_PER_THREAD), nz);
 }
 
 
@@ -223,6 +256,9 @@ struct nuppn_ctx {
     int *d_seg_i[9] = {};         // jac, dxdt, resid: e_idx, lane_t0, lane_s0
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     double *d_jac_lo = nullptr;   // jac: e_coef_lo
+    double *d_rcache = nullptr, *d_rc_t9 = nullptr;   // REACLIB T9 cache, grid
+    double rc_dt9 = 0.0;
+    int use_cache = 0;
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
     int bmax_slots = 0, bslots = 0;
@@ -253,7 +289,7 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_diagk, (void *) d_b_ent_ptr, (void *) d_b_ent_k,
                     (void *) d_b_ent_loc, (void *) d_b_ent_col, (void *) d_b_csr_col,
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
-                    (void *) d_b_ent_dst, (void *) d_jac_lo})
+                    (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9})
         if (p) (void) hipFree(p);
     for (void *p : d_seg_i)
         if (p) (void) hipFree(p);
@@ -290,6 +326,9 @@ static DevNet dev_net(const nuppn_ctx *c) {
     N.A_isdiag = c->d_isdiag;
     N.perm_c = c->d_perm_c;
     N.iperm_c = c->d_iperm_c;
+    N.rcache = c->use_cache ? c->d_rcache : nullptr;
+    N.rc_t9 = c->d_rc_t9;
+    N.rc_dt9 = c->rc_dt9;
     return N;
 }
 
@@ -728,6 +767,34 @@ int nuppn_get_column_order(nuppn_ctx *c, int *perm_c) {
     return NUPPN_OK;
 }
 
+int nuppn_set_rate_cache(nuppn_ctx *c, int use_cache) {
+    if (!c) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (use_cache && !c->d_rcache) {
+        const int nre = c->net.nre;
+        c->rc_dt9 = (10.0 - 0.01) / (kNT9 - 1);
+        std::vector<double> grid(kNT9);
+        for (int i = 0; i < kNT9; ++i) grid[i] = 0.01 + i * c->rc_dt9;
+        if (dev_upload(&c->d_rc_t9, grid) ||
+            dev_upload(&c->d_rcache, (const double *) nullptr, (size_t) kNT9 * nre))
+            return NUPPN_ERR_HIP;
+        hipLaunchKernelGGL(rate_cache_kernel, dim3((nre + 255) / 256, kNT9), dim3(256), 0, 0,
+                           dev_net(c), c->d_rc_t9, c->d_rcache);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipDeviceSynchronize());
+    }
+    c->use_cache = use_cache ? 1 : 0;
+    return NUPPN_OK;
+}
+
+int nuppn_rate_cache_table(nuppn_ctx *c, double *table) {
+    if (!c || !table) return NUPPN_ERR_ARG;
+    if (!c->d_rcache) return NUPPN_ERR_UNSUPPORTED;
+    HIP_OK(hipMemcpy(table, c->d_rcache, sizeof(double) * kNT9 * (size_t) c->net.nre,
+                     hipMemcpyDeviceToHost));
+    return NUPPN_OK;
+}
+
 int nuppn_reduced_size(nuppn_ctx *c, int *n, int *nnz) {
     if (!c || !n || !nnz) return NUPPN_ERR_ARG;
     *n = c->sym.n;
@@ -752,8 +819,7 @@ int nuppn_rates_eval(nuppn_ctx *c, int nz, const double *t9, const double *rho, 
     HIP_OK(hipMemcpy(c->d_t9_0, t9, sizeof(double) * nz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_rho_0, rho, sizeof(double) * nz, hipMemcpyHostToDevice));
     const int nre = c->net.nre;
-    dim3 grid((nre + 255) / 256, nz);
-    hipLaunchKernelGGL(rates_kernel, grid, dim3(256), 0, 0, c->d_coef, c->d_rtype, nre, nz,
+    hipLaunchKernelGGL(rates_kernel, rates_grid(nre, nz), dim3(256), 0, 0, dev_net(c), nz,
                        c->d_t9_0, c->d_t9_0, c->d_rho_0, c->d_rho_0, c->d_v);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpy(v, c->d_v, sizeof(double) * nre * (size_t) nz, hipMemcpyDeviceToHost));
@@ -854,8 +920,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     if (ensure_rate_buffer(c, nz) || ensure_zone_state(c, nz)) return NUPPN_ERR_HIP;
     const int nre = c->net.nre;
     HIP_OK(hipEventRecord(c->ev[0], s));
-    dim3 grid((nre + 255) / 256, nz);
-    hipLaunchKernelGGL(rates_kernel, grid, dim3(256), 0, s, c->d_coef, c->d_rtype, nre, nz, t9_0,
+    hipLaunchKernelGGL(rates_kernel, rates_grid(nre, nz), dim3(256), 0, s, dev_net(c), nz, t9_0,
                        t9_1, rho_0, rho_1, c->d_v);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(c->ev[1], s));

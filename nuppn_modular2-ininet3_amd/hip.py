@@ -111,6 +111,8 @@ def load():
     L.slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                             C.POINTER(C.c_int)]
     L.slu_solve_noalloc.argtypes = L.slu_solve.argtypes
+    L.nuppn_set_rate_cache.argtypes = [C.c_void_p, C.c_int]
+    L.nuppn_rate_cache_table.argtypes = [C.c_void_p, _dp]
     L.c_fortran_dgssv_.argtypes = [C.POINTER(C.c_int)] * 4 + [_dp, _ip, _ip, _dp,
                                                                C.POINTER(C.c_int),
                                                                C.POINTER(C.c_longlong),
@@ -223,6 +225,19 @@ class Context:
         if rc != OK:
             raise RuntimeError(f"nuppn_dxdt failed ({rc})")
         return out
+
+    def set_rate_cache(self, use_cache=True):
+        """nuppn_set_rate_cache: the REACLIB T9 cache on (built on first use) or off"""
+        rc = load().nuppn_set_rate_cache(self.h, int(use_cache))
+        if rc != OK:
+            raise RuntimeError(f"nuppn_set_rate_cache failed ({rc})")
+
+    def rate_cache_table(self):
+        out = np.zeros(5001 * self.net.nre)
+        rc = load().nuppn_rate_cache_table(self.h, out)
+        if rc != OK:
+            raise RuntimeError(f"nuppn_rate_cache_table failed ({rc})")
+        return out.reshape(5001, self.net.nre)
 
     def jacobian_reduced(self, dt, x, v):
         out = np.zeros(self.nnzA)

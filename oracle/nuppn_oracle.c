@@ -21,6 +21,7 @@ enum { SLV_CNVRG = 1, MAX_SUBSTEPS, BAD_STEP, NEG_ABUNDS, HUGE_ABUNDS, SUM_X, GO
 #define SUM_TOL 1.0e-6          /* solver_knobs.F90:26 */
 #define SMALL_C 1.0e-30         /* constants.F90:81 */
 #define NLEVELS 8               /* bader_deuflhard.F90:27 */
+#define ORC_NT9 5001            /* num_t9_preproc, reaclib.F90:155 */
 
 struct orc_ctx {
     int nsp, nre, igamma;
@@ -49,6 +50,10 @@ struct orc_ctx {
     double *qcol, *yerr;    /* [NLEVELS*nsp] column-major like the reference (nsp, nlevels) */
     double err[NLEVELS], xcol[NLEVELS];
     double errmax, abuerr, negerr, sumerr, xest;
+    /* the REACLIB T9 cache (reaclib.F90:155-156, 609-646) */
+    int use_cache;
+    double *rcache;         /* [ORC_NT9][nre] */
+    double rc_t9[ORC_NT9], rc_dt9;
 };
 
 /* ------------------------------------------------------------------------------------------ */
@@ -340,16 +345,61 @@ void orc_destroy(orc_ctx *c)
     free(c->v); free(c->jac); free(c->jac_lo); free(c->jac_cpy); free(c->jrow); free(c->jcol);
     free(c->red_val); free(c->red_colind); free(c->red_rowptr); free(c->iexpand);
     free(c->xn); free(c->dxdt); free(c->dxdt_lo); free(c->rhs); free(c->red_rhs); free(c->red_xlo); free(c->mask);
-    free(c->qcol); free(c->yerr);
+    free(c->qcol); free(c->yerr); free(c->rcache);
     free(c);
 }
 
 /* ------------------------------------------------------------------------------------------ */
 /* rates: reaclib_calculate_rates (reaclib.F90:431-582), direct-evaluation branch, one REACLIB
- * set per reaction, Gfac = 1; then evaluate_all_rates' clips (evaluate_rates.F90:46,172).     */
+ * set per reaction, Gfac = 1; then evaluate_all_rates' clips (evaluate_rates.F90:46,172).
+ * With the T9 cache on (physics knob use_cache, reaclib.F90:446-470) the rates are interpolated
+ * from the table of orc_set_rate_cache instead.                                              */
+static void rates_direct(const orc_ctx *c, double t9in, double rho, double *v);
+
 void orc_rates(const orc_ctx *c, double t9in, double rho, double ye, double *v)
 {
     (void) ye; /* rho*ye scaling applies to '  ec' rates only; the network has none */
+    if (c->use_cache && c->rcache) {
+        /* reaclib.F90:443-470: i1 = floor((t9 - 0.01) / dt9) + 1 clamped to [1, n - 1]
+         * (0-based here), linear interpolation, x rho for the two-body rates, the floor; then
+         * evaluate_all_rates' clips */
+        const int nre = c->nre;
+        const double t9 = fmax(t9in, 0.01);
+        int i1 = (int) floor((t9 - 0.01) / c->rc_dt9);
+        i1 = i1 < 0 ? 0 : (i1 > ORC_NT9 - 2 ? ORC_NT9 - 2 : i1);
+        const double dt9 = (t9 - c->rc_t9[i1]) / c->rc_dt9;
+        const double *c1 = c->rcache + (size_t) i1 * nre, *c2 = c1 + nre;
+        for (int l = 0; l < nre; ++l) {
+            double rv = c1[l] + dt9 * (c2[l] - c1[l]);
+            if (c->rtype[l] == 0) rv = rv * rho;
+            rv = fmax(rv, 1.0e-99);
+            rv = fmax(rv * 1.0, 1.0e-99);
+            v[l] = fmax(rv, 1.0e-99);
+        }
+        return;
+    }
+    rates_direct(c, t9in, rho, v);
+}
+
+/* reaclib_preprocessor (reaclib.F90:609-646): the rates at rho = 1 on the 5,001-point grid
+ * t9_i = 0.01 + i dt9, dt9 = (10 - 0.01) / 5000, each floored at 1e-99 */
+void orc_set_rate_cache(orc_ctx *c, int use_cache)
+{
+    c->use_cache = use_cache;
+    if (!use_cache || c->rcache) return;
+    const int nre = c->nre;
+    c->rcache = malloc(sizeof(double) * (size_t) nre * ORC_NT9);
+    c->rc_dt9 = (10.0 - 0.01) / (ORC_NT9 - 1);
+    for (int i = 0; i < ORC_NT9; ++i) {
+        c->rc_t9[i] = 0.01 + i * c->rc_dt9;
+        rates_direct(c, c->rc_t9[i], 1.0, c->rcache + (size_t) i * nre);
+    }
+}
+
+const double *orc_rate_cache(const orc_ctx *c) { return c->rcache; }
+
+static void rates_direct(const orc_ctx *c, double t9in, double rho, double *v)
+{
     const double t9 = fmax(t9in, 0.01);          /* t9_lowlimit_reaclib, reaclib.F90:443 */
     const double t9inv = 1.0 / t9;
     const double t9pow13 = pow(t9, 1.0 / 3.0);

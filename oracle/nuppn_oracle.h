@@ -76,6 +76,12 @@ void orc_full_pattern(const orc_ctx *c, int *rowptr, int *colind, int *map);
  * evaluate_rates.F90:46,172 (rfac = 1) */
 void orc_rates(const orc_ctx *c, double t9, double rho, double ye, double *v);
 
+/* The REACLIB T9 cache (physics knob use_cache): reaclib_preprocessor (reaclib.F90:609-646)
+ * tabulates the rates at rho = 1 on 5,001 points T9 = 0.01 .. 10, and orc_rates then interpolates
+ * them linearly (reaclib.F90:446-470).  orc_rate_cache: the table [5001][nre] or NULL. */
+void orc_set_rate_cache(orc_ctx *c, int use_cache);
+const double *orc_rate_cache(const orc_ctx *c);
+
 /* calculate_dxdt, solver/source/jac_rhs.F90:217-278 */
 void orc_dxdt(const orc_ctx *c, const double *x, const double *v, double *dxdt);
 

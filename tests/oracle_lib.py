@@ -66,6 +66,9 @@ def lib():
     L.orc_reset_state.argtypes = [C.c_void_p]
     L.orc_set_rates.argtypes = [C.c_void_p, _dp]
     L.orc_bd_state.argtypes = [C.c_void_p, _ip, _dp]
+    L.orc_set_rate_cache.argtypes = [C.c_void_p, C.c_int]
+    L.orc_rate_cache.restype = C.POINTER(C.c_double)
+    L.orc_rate_cache.argtypes = [C.c_void_p]
     L.orc_bd_set_frst.argtypes = [C.c_void_p, C.c_int]
     L.orc_nnz_full.argtypes = [C.c_void_p]
     L.orc_full_pattern.argtypes = [C.c_void_p, _ip, _ip, _ip]
@@ -206,6 +209,16 @@ class Oracle:
         lib().orc_bd_state(self.h, iv, dv)
         return dict(frst=int(iv[0]), reduce=int(iv[1]), kopt=int(iv[2]), kmax=int(iv[3]),
                     errmax=dv[0], abuerr=dv[1], negerr=dv[2], sumerr=dv[3], err=dv[4:].copy())
+
+    def set_rate_cache(self, use_cache=True):
+        lib().orc_set_rate_cache(self.h, int(use_cache))
+
+    def rate_cache(self):
+        """the T9 table [5001, nre] (a copy), or None"""
+        p = lib().orc_rate_cache(self.h)
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(5001, self.net.nre)).copy()
 
     def bd_set_frst(self, frst):
         lib().orc_bd_set_frst(self.h, int(frst))
