@@ -102,12 +102,12 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
 
 // Factors of the assembled matrix (A in CSR order) in precision T into the factor area fac.
 // Returns 0, or 1 when a pivot or the border Schur complement is not finite or zero.
-template <class T, int NB>
+template <class T, int NB, class TM = Team<1>>
 __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *Aval, char *fac,
                                  NUPPN_LDS char *lds, ZoneStats &st) {
     const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
     const BlockLds<T> L = make_block_lds<T, NB>(lds, b.B.nip);
-    const int fail = block_factor<T, NB>(b.B, Aval, F, L);
+    const int fail = block_factor<T, NB, TM>(b.B, Aval, F, L);
     st.n_factor++;
     if (sizeof(T) == 8) st.n_factor_fp64++;
     return fail;
@@ -121,7 +121,7 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
 // relative to its own size.  Returns 0 (x, xlo hold the solution), 1 (FP32 factors cannot
 // deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64 factors: the
 // reference's singular-matrix outcome).
-template <class T, int NB>
+template <class T, int NB, class TM = Team<1>>
 __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
                             const double *rhs, const double *y, double *x, double *xlo,
                             double *res, double *d, char *fac, NUPPN_LDS char *lds,
@@ -134,19 +134,20 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
     NUPPN_GLB double *xg = gptr(x), *xl = gptr(xlo), *resg = gptr(res);
     const NUPPN_GLB double *dg = gptr((const double *) d), *yg = gptr(y), *rhsg = gptr(rhs);
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
-    for (int i = lane_id(); i < n; i += WAVE) {
+    const int t0 = TM::tid();
+    for (int i = t0; i < n; i += TM::NT) {
         xg[i] = 0.0;
         xl[i] = 0.0;
         resg[i] = rhsg[i];
     }
-    wave_sync();
+    TM::sync();
     double eta_prev = 0.0;
     for (int k = 0; k < b.O.max_refine; ++k) {
-        block_solve<T, NB>(B, F, L, res, d);
+        block_solve<T, NB, TM>(B, F, L, res, d);
         st.n_solve++;
         double eta = 0.0;
         bool fin = true;
-        for (int i = lane_id(); i < n; i += WAVE) {
+        for (int i = t0; i < n; i += TM::NT) {
             const double di = dg[i], xh = xg[i];
             // (xh + xlo) + d: TwoSum, the rounding error into the low part
             const double sh = xh + di;
@@ -159,15 +160,14 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
             eta = fmax(eta, fabs(di) / scale);
             fin = fin && isfinite(hi);
         }
-        eta = wave_max(eta);
-        wave_sync();
-        if (ballot(!fin)) return fp64 ? -4 : 1;
+        eta = team_max<TM>(eta, L.red);
+        if (team_any<TM>(!fin, L.red)) return fp64 ? -4 : 1;
         if (eta <= b.O.refine_tol) return 0;   // converged corrections
         // the next residual, extra-precise (d, the correction just applied, serves as its
         // scratch); with accept_tol > 0, x is also accepted once it is backward stable: it
         // solves a system within accept_tol, entry by entry, of A and rhs
-        const double omega = block_residual(B, Aval, rhs, x, xlo, res, d,
-                                            (double *) (fac + b.off[B_LO]));
+        const double omega = block_residual<TM>(B, Aval, rhs, x, xlo, res, d,
+                                                (double *) (fac + b.off[B_LO]), L.red);
         st.n_refine++;
         if (b.O.accept_tol > 0.0 && omega <= b.O.accept_tol) return 0;
         // otherwise the corrections must keep contracting (the first correction's eta is
@@ -183,13 +183,13 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
 
 // Refined solve with fresh factors (the backward-Euler Newton iteration: one solve per
 // factorisation).  Same return codes as block_refine.
-template <class T, int NB>
+template <class T, int NB, class TM>
 __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
                                    const double *rhs, const double *y, double *x, double *xlo,
                                    double *res, double *d, char *fac, NUPPN_LDS char *lds,
                                    ZoneStats &st) {
-    if (block_factor_zone<T, NB>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
-    return block_refine<T, NB>(N, b, Aval, rhs, y, x, xlo, res, d, fac, lds, st);
+    if (block_factor_zone<T, NB, TM>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
+    return block_refine<T, NB, TM>(N, b, Aval, rhs, y, x, xlo, res, d, fac, lds, st);
 }
 
 // dX/dt of the reduced species from the per-reaction fluxes vx, each species' sum carried in
@@ -199,12 +199,13 @@ __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs
 // an FP64 sum loses the rounding of its gross fluxes (forward and reverse rates of nuclei near
 // equilibrium exceed the net by many orders): the mass drift of a hot zone over a cycle falls
 // from ~1e-11 to the last bits.
+template <class TM = Team<1>>
 __device__ __forceinline__ void block_dxdt_dd(const DevBlock &B, const double *vxp, double *outp,
                                               double *lop) {
     const DevSeg &D = B.dxdt;
     const NUPPN_GLB double *coef = gptr(D.e_coef), *vxg = gptr(vxp);
     NUPPN_GLB double *o = gptr(outp), *lo = gptr(lop);
-    seg_stream<SumDD2>(
+    seg_stream<SumDD2, TM>(
         D,
         [&](int e, int idx) {
             const double c = coef[e], f = vxg[idx];
@@ -221,12 +222,13 @@ __device__ __forceinline__ void block_dxdt_dd(const DevBlock &B, const double *v
 // contributions cnt stem a_row / a_col as real values - the coefficient a double-double from the
 // host, its product with the stem split exactly, the sum in double-double (Dot2) - plus the
 // 1e-99 seed of calculate_jacobian (jac_rhs.F90:309), rounded once.  jlo: [nnzA] scratch.
+template <class TM = Team<1>>
 __device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const double *stemp,
                                                   double *jacp, double *jlop) {
     const DevSeg &J = B.jac;
     const NUPPN_GLB double *coef = gptr(J.e_coef), *coefl = gptr(J.e_coef_lo), *st = gptr(stemp);
     NUPPN_GLB double *jac = gptr(jacp), *jlo = gptr(jlop);
-    seg_stream<SumDD2>(
+    seg_stream<SumDD2, TM>(
         J,
         [&](int e, int idx) {
             const double s = st[idx], ch = coef[e];
@@ -237,38 +239,41 @@ __device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const doubl
             jac[k] = t.hi;
             jlo[k] = t.lo;
         });
-    for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+    for (int k = TM::tid(); k < B.nnzA; k += TM::NT) {
         const double hi = jac[k], seed = 1.0e-99;
         const double s = hi + seed, bb = s - hi;
         jac[k] = s + (((hi - (s - bb)) + (seed - bb)) + jlo[k]);
     }
-    wave_sync();
+    TM::sync();
 }
 
 // the batched path's assembly: fluxes and stems per reaction, then I - dt J (CSR order, with
 // reduce_jacobian's drops) and the Newton rhs y_old - y_cur + dt dX/dt as streamed sums over
 // precomputed coefficient lists (J, dX/dt and the rhs each rounded once from double-double
 // sums).  lo: [n], jlo: [nnzA] scratch.  Returns 1 where the reference stops (reduced_value).
+template <class TM>
 __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
-                                      double *rhs, double *lop, double *jlo) {
-    molar(N, y_cur, xn);
-    wave_sync();
-    reaction_terms(N, xn, v, vx, stem, stem + N.nre);
-    wave_sync();
+                                      double *rhs, double *lop, double *jlo,
+                                      NUPPN_LDS double *red) {
+    const int t0 = TM::tid();
+    molar(N, y_cur, xn, t0, TM::NT);
+    TM::sync();
+    reaction_terms(N, xn, v, vx, stem, stem + N.nre, t0, TM::NT);
+    TM::sync();
     // the streamed sums store raw totals; the scalings follow in coalesced passes
     const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
     NUPPN_GLB double *Ag = gptr(Aval);
-    block_jacobian_dd(B, stem, Aval, jlo);
+    block_jacobian_dd<TM>(B, stem, Aval, jlo);
     int stop = 0;
-    for (int k = lane_id(); k < B.nnzA; k += WAVE) Ag[k] = reduced_value(Ag[k], dt, isdiag[k], stop);
-    stop = ballot(stop) ? 1 : 0;
+    for (int k = t0; k < B.nnzA; k += TM::NT) Ag[k] = reduced_value(Ag[k], dt, isdiag[k], stop);
+    stop = team_any<TM>(stop != 0, red) ? 1 : 0;
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur), *lo = gptr((const double *) lop);
     NUPPN_GLB double *rg = gptr(rhs);
-    block_dxdt_dd(B, vx, rhs, lop);
-    for (int r = lane_id(); r < N.n; r += WAVE) {
+    block_dxdt_dd<TM>(B, vx, rhs, lop);
+    for (int r = t0; r < N.n; r += TM::NT) {
         // y_old - y_cur + dt (hi + lo): each step split exactly (TwoSum, FMA), rounded once
         const int sp = red2sp[r];
         const double a = yo[sp], b = -yc[sp], fh = rg[r];
@@ -280,7 +285,7 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B
         const double e3 = (s1 - (s2 - b2)) + (p - b2);
         rg[r] = s2 + (e1 + (e2 + e3));
     }
-    wave_sync();
+    TM::sync();
     return stop;
 }
 
@@ -288,7 +293,7 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B
 // the solve with FP32 factors while the zone's factor precision (state pad0) is FP32, and with
 // FP64 factors when that fails (the zone then stays on FP64 for the rest of the call) or when
 // the zone is on FP64 already.  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON.
-template <int NB>
+template <int NB, class TM>
 __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
                                   NUPPN_LDS char *lds, int z) {
     const DevNet &N = a.N;
@@ -310,42 +315,44 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     double *res = (double *) (slot + b.off[B_RES]);
     double *d = (double *) (slot + b.off[B_D]);
     const double dt = s.dttry;
-    const int stop = block_assemble(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
-                                    (double *) (slot + b.off[B_LO]),
-                                    (double *) (slot + b.off[B_JAC]));
+    NUPPN_LDS double *red = make_block_lds<double, NB>(lds, b.B.nip).red;
+    const int t0 = TM::tid();
+    const int stop = block_assemble<TM>(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
+                                        (double *) (slot + b.off[B_LO]),
+                                        (double *) (slot + b.off[B_JAC]), red);
     int rc = stop ? 2 : 1;
     if (!stop && s.pad0 == 0) {
-        rc = block_refined_solve<float, NB>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot, lds,
-                                            s.st);
+        rc = block_refined_solve<float, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
+                                                lds, s.st);
         if (rc == 1) s.pad0 = 1;
     }
     if (rc == 1)
-        rc = block_refined_solve<double, NB>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot, lds,
-                                             s.st);
+        rc = block_refined_solve<double, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
+                                                 lds, s.st);
     if (rc == 0) {
         // y_new = y_cur + (x + xlo), rounded once
-        for (int r = lane_id(); r < N.n; r += WAVE) {
+        for (int r = t0; r < N.n; r += TM::NT) {
             const int sp = N.red2sp[r];
             const double yc = y_cur[sp], xh = x[r];
             const double sh = yc + xh;
             const double bb = sh - yc;
             y_new[sp] = sh + (((yc - (sh - bb)) + (xh - bb)) + xlo[r]);
         }
-        wave_sync();
+        TM::sync();
         double g = 0.0;
-        for (int i = lane_id(); i < N.nsp; i += WAVE) {
+        for (int i = t0; i < N.nsp; i += TM::NT) {
             const double yc = y_cur[i];
             const double dd = y_new[i] - yc;
             const double ga = (yc > K.grdthreshold) ? fabs(dd / yc) : 0.0;
             g = fmax(g, ga);
         }
-        s.grd = wave_max(g);
+        s.grd = team_max<TM>(g, red);
     }
     s.lib_err = stop ? 0 : rc;
     s.step_ierr = stop ? E_BAD_STEP : 0;
     s.phase = PH_AFTER_NEWTON;
-    if (lane_id() == 0) Z.state[z] = s;
-    wave_sync();
+    if (t0 == 0) Z.state[z] = s;
+    TM::sync();
 }
 
 // The batched path's solver for the Bader-Deuflhard step: J once per step by the chunked sums
@@ -466,7 +473,24 @@ __global__ void __launch_bounds__(64, W) zone_newton_block_kernel(KArgs a, BKArg
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
     for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
-        zone_newton_block<NB>(a, b, slot, lds, a.Z.active[i]);
+        zone_newton_block<NB, Team<1>>(a, b, slot, lds, a.Z.active[i]);
+}
+
+// wavefronts per zone of the kernel for passes with few zones
+constexpr int kTeamWaves = kSegRounds;
+
+// The same iteration with kTeamWaves wavefronts per zone (one workgroup, one zone per CU): the
+// assembly streams, the residuals, the dense-block set-up, the border columns and every
+// element-wise pass are shared; the block Thomas loop and the solves' sweeps stay on the first
+// wavefront.  For passes whose list has no more zones than the device has CUs, where the
+// single-wavefront iteration's latency sets the pass time.
+template <int NB>
+__global__ void __launch_bounds__(64 * kTeamWaves, 1) zone_newton_team_kernel(KArgs a, BKArgs b) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
+    char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
+    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+        zone_newton_block<NB, Team<kTeamWaves>>(a, b, slot, lds, a.Z.active[i]);
 }
 
 // every zone in PH_NEWTON under integration_method 1: one Bader-Deuflhard step each (a kernel

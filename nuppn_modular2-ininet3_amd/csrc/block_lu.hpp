@@ -22,6 +22,7 @@
 // A solve is then two sweeps of dense NB x NB mat-vecs plus the border correction.
 #pragma once
 #include "device_common.hpp"
+#include "symbolic.hpp"
 
 namespace nuppn {
 
@@ -32,8 +33,8 @@ struct DevSeg {
     const int *e_idx;       // operand index, bit 31 on a segment's last term
     const double *e_coef;
     const double *e_coef_lo;   // J: low parts of the coefficients (null for the other streams)
-    const int *lane_t0;     // [65]
-    const int *lane_s0;     // [64]
+    const int *lane_t0;     // [64 kSegRounds + 1]
+    const int *lane_s0;     // [64 kSegRounds]
 };
 
 struct DevBlock {
@@ -68,15 +69,19 @@ struct BlockCfg {
     static constexpr int LD = NB + 4;     // LDS leading dimension (16-byte aligned, conflict-free)
 };
 
-// LDS of one wavefront: the two block-product operands X1, X2 of the factorisation loop (X2
-// also holds the Gauss-Jordan pivot columns and rows between the products), which the solve
-// vector vec overlays (the loop and the solves never run at the same time), and the border
-// scratch.  About 18.6 KB with FP64 factors (NB = 32, 1,100 species), so eight wavefronts share
-// a CU's 160 KB.
+// LDS of one zone: the two block-product operands X1, X2 of the factorisation loop (X2 also
+// holds the Gauss-Jordan pivot columns and rows between the products), which the solve vector
+// vec overlays (the loop and the solves never run at the same time), the border scratch, the
+// team reduction slots and, for a zone of NW > 1 wavefronts, one more solve vector per extra
+// wavefront (wvec: the border columns are solved in parallel).  About 18.9 KB with FP64 factors
+// and one wavefront (NB = 32, 1,100 species), so eight single-wavefront zones share a CU's
+// 160 KB.
 template <class T>
 struct BlockLds {
     NUPPN_LDS T *X1, *X2, *vec;
     NUPPN_LDS double *bscr;   // [16] border solve scratch
+    NUPPN_LDS double *red;    // [16] team reductions
+    NUPPN_LDS T *wvec;        // [(NW - 1) (nip + 8)]
 };
 
 template <class T, int NB>
@@ -87,9 +92,10 @@ __host__ __device__ constexpr size_t block_lds_span(int nip) {
 }
 
 template <class T, int NB>
-__host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd) {
+__host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd, int nw = 1) {
     (void) nbd;   // nbd <= 8: the border entries fit the 8 slots after nip
-    return sizeof(T) * block_lds_span<T, NB>(nip) + 16 + 16 * sizeof(double);
+    return sizeof(T) * block_lds_span<T, NB>(nip) + 16 + 32 * sizeof(double) +
+           (size_t) (nw - 1) * sizeof(T) * (nip + 8);
 }
 
 template <class T, int NB>
@@ -104,7 +110,60 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
     NUPPN_LDS char *q = (NUPPN_LDS char *) p;
     q = (NUPPN_LDS char *) (((uintptr_t) q + 15) & ~(uintptr_t) 15);
     L.bscr = (NUPPN_LDS double *) q;
+    L.red = L.bscr + 16;
+    L.wvec = (NUPPN_LDS T *) (L.red + 16);
     return L;
+}
+
+// --- reductions over the threads of a zone (Team): every thread gets the result.  With one
+// wavefront they are cross-lane operations; with several, each wavefront's value goes through
+// the LDS slots red and the wavefronts' values are combined in wavefront order.
+template <class TM>
+__device__ __forceinline__ double team_max(double v, NUPPN_LDS double *red) {
+    v = wave_max(v);
+    if constexpr (TM::NWAVE == 1) {
+        return v;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = v;
+        TM::sync();
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < TM::NWAVE; ++w) m = fmax(m, red[w]);
+        TM::sync();
+        return m;
+    }
+}
+
+template <class TM>
+__device__ __forceinline__ double team_sum(double v, NUPPN_LDS double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    if constexpr (TM::NWAVE == 1) {
+        return v;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = v;
+        TM::sync();
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < TM::NWAVE; ++w) m = m + red[w];
+        TM::sync();
+        return m;
+    }
+}
+
+template <class TM>
+__device__ __forceinline__ bool team_any(bool p, NUPPN_LDS double *red) {
+    const bool w = ballot(p) != 0;
+    if constexpr (TM::NWAVE == 1) {
+        return w;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = w ? 1.0 : 0.0;
+        TM::sync();
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < TM::NWAVE; ++k) any = any || red[k] != 0.0;
+        TM::sync();
+        return any;
+    }
 }
 
 template <class T>
@@ -464,57 +523,16 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Factor the reduced matrix (FP64 CSC values Aval) into F.  Returns 0, or 1 when a pivot or
-// the border Schur complement is not finite / zero (the caller falls back or reports).
+// The block Thomas loop of block_factor on one wavefront (the dense blocks are in place): S_i^-1
+// by Gauss-Jordan, W_i = S_i^-1 C_i and S_{i+1} = D_{i+1} - B_i W_i.  Only wavefront-local
+// ordering points, so in a multi-wavefront zone the other wavefronts wait at their next barrier.
+// Returns 1 when an inverse is not finite.
 template <class T, int NB>
-__device__ __forceinline__ int block_factor(const DevBlock &B, const double *Aval, const BlockFactors<T> &F,
-                            const BlockLds<T> &L) {
+__device__ __forceinline__ int block_thomas_factor(const DevBlock &B, const BlockFactors<T> &F,
+                                                   const BlockLds<T> &L) {
     using C = BlockCfg<T, NB>;
     constexpr size_t BS = (size_t) NB * NB;
-    const int lane = lane_id();
     const int nblk = B.nblk;
-    bool bad = false;
-    for (int i = lane; i < B.n; i += WAVE) {
-        const double d = gptr(Aval)[gptr(B.diag_a)[i]];
-        if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
-        gptr(F.cs)[i] = 1.0 / d;
-    }
-    if (ballot(bad)) return 1;
-    wave_sync();
-    // scalings by padded position (interior blocks, then border): padding rows/columns stay 1
-    for (int p = B.ni + lane; p < B.nip; p += WAVE) {
-        gptr(F.colsc)[p] = 1.0;
-        gptr(F.rowinv)[p] = 1.0;
-    }
-    for (int i = lane; i < B.n; i += WAVE) {
-        const int p = gptr(B.vpos)[i];
-        const double ms = gptr(B.mass)[i];
-        gptr(F.colsc)[p] = ms * gptr(F.cs)[i];
-        gptr(F.rowinv)[p] = 1.0 / ms;
-    }
-    wave_sync();
-    // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
-    // where W_i goes, B_i in place (the loop below then only streams them, prefetched)
-    {
-        const size_t area = (size_t) nblk * BS;
-        NUPPN_GLB T *S0 = gptr(F.Sinv), *W0 = gptr(F.W), *B0 = gptr(F.Bm);
-        for (size_t e = lane; e < area; e += WAVE) {
-            S0[e] = T(0);
-            W0[e] = T(0);
-            B0[e] = T(0);
-        }
-        wave_sync();
-        const int nent = gptr(B.ent_ptr)[3 * nblk - 2];
-        for (int e = lane; e < nent; e += WAVE) {
-            const int dst = gptr(B.ent_dst)[e];
-            const int ar = (int) (dst / area);
-            NUPPN_GLB T *base = ar == 0 ? S0 : (ar == 1 ? W0 : B0);
-            base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
-        }
-        const int pr = lane, last = nblk - 1;   // identity on the padded diagonal positions
-        if (pr < NB && last * NB + pr >= B.ni) S0[(size_t) last * BS + pr * NB + pr] = T(1);
-        wave_sync();
-    }
     Regs<T, NB> a, cb, dn, bb;
     MTiles<T> dnt;   // NB = 32: D_{i+1} in the matrix-core tile layout
     glb_load_cm<T, NB>(F.Sinv, a);
@@ -570,7 +588,66 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
             gemm_lds_sub<T, NB>(L.X1, L.X2, a);                          // D - B_i W_i
         }
     }
-    wave_sync();
+    wave_fence();
+    return 0;
+}
+
+// Factor the reduced matrix (FP64 CSC values Aval) into F with the zone's threads (TM).  Returns
+// 0, or 1 when a pivot or the border Schur complement is not finite / zero (the caller falls back
+// or reports).  The Thomas loop runs on the zone's first wavefront; with several wavefronts the
+// dense-block set-up, the border columns' solves (one column per wavefront) and the Schur
+// complement's sums are shared.
+template <class T, int NB, class TM>
+__device__ __forceinline__ int block_factor(const DevBlock &B, const double *Aval, const BlockFactors<T> &F,
+                            const BlockLds<T> &L) {
+    constexpr size_t BS = (size_t) NB * NB;
+    constexpr int NT = TM::NT;
+    const int t0 = TM::tid();
+    const int nblk = B.nblk;
+    bool bad = false;
+    for (int i = t0; i < B.n; i += NT) {
+        const double d = gptr(Aval)[gptr(B.diag_a)[i]];
+        if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
+        gptr(F.cs)[i] = 1.0 / d;
+    }
+    if (team_any<TM>(bad, L.red)) return 1;
+    // scalings by padded position (interior blocks, then border): padding rows/columns stay 1
+    for (int p = B.ni + t0; p < B.nip; p += NT) {
+        gptr(F.colsc)[p] = 1.0;
+        gptr(F.rowinv)[p] = 1.0;
+    }
+    for (int i = t0; i < B.n; i += NT) {
+        const int p = gptr(B.vpos)[i];
+        const double ms = gptr(B.mass)[i];
+        gptr(F.colsc)[p] = ms * gptr(F.cs)[i];
+        gptr(F.rowinv)[p] = 1.0 / ms;
+    }
+    TM::sync();
+    // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
+    // where W_i goes, B_i in place (the loop then only streams them, prefetched)
+    {
+        const size_t area = (size_t) nblk * BS;
+        NUPPN_GLB T *S0 = gptr(F.Sinv), *W0 = gptr(F.W), *B0 = gptr(F.Bm);
+        for (size_t e = t0; e < area; e += NT) {
+            S0[e] = T(0);
+            W0[e] = T(0);
+            B0[e] = T(0);
+        }
+        TM::sync();
+        const int nent = gptr(B.ent_ptr)[3 * nblk - 2];
+        for (int e = t0; e < nent; e += NT) {
+            const int dst = gptr(B.ent_dst)[e];
+            const int ar = (int) (dst / area);
+            NUPPN_GLB T *base = ar == 0 ? S0 : (ar == 1 ? W0 : B0);
+            base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
+        }
+        const int pr = t0, last = nblk - 1;   // identity on the padded diagonal positions
+        if (pr < NB && last * NB + pr >= B.ni) S0[(size_t) last * BS + pr * NB + pr] = T(1);
+        TM::sync();
+    }
+    int fail = 0;
+    if (TM::wave() == 0) fail = block_thomas_factor<T, NB>(B, F, L);
+    if (team_any<TM>(fail != 0, L.red)) return 1;
     // border: E, X_F = T^-1 F, G' = G - E X_F and its LU with partial pivoting
     const int nbd = B.nbd, nip = B.nip;
     if (nbd > 0) {
@@ -578,50 +655,65 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         NUPPN_GLB T *Eg = gptr(F.E), *XFg = gptr(F.XF);
         NUPPN_GLB double *Gg = gptr(F.G);
         const NUPPN_GLB int *eptr = gptr(B.ent_ptr), *eloc = gptr(B.ent_loc);
-        for (int i = lane; i < nbd * nip; i += WAVE) {
+        for (int i = t0; i < nbd * nip; i += NT) {
             Eg[i] = T(0);
             XFg[i] = T(0);
         }
-        for (int i = lane; i < nbd * nbd; i += WAVE) Gg[i] = 0.0;
-        wave_sync();
-        for (int e = eptr[LE] + lane; e < eptr[LE + 1]; e += WAVE)
+        for (int i = t0; i < nbd * nbd; i += NT) Gg[i] = 0.0;
+        TM::sync();
+        for (int e = eptr[LE] + t0; e < eptr[LE + 1]; e += NT)
             Eg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = eptr[LF] + lane; e < eptr[LF + 1]; e += WAVE)
+        for (int e = eptr[LF] + t0; e < eptr[LF + 1]; e += NT)
             XFg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = eptr[LG] + lane; e < eptr[LG + 1]; e += WAVE)
+        for (int e = eptr[LG] + t0; e < eptr[LG + 1]; e += NT)
             Gg[eloc[e]] = block_entry(B, Aval, F.cs, e);
-        wave_sync();
-        // X_F = T^-1 F, two columns per sweep where two vectors fit the LDS span
+        TM::sync();
+        const int lane = lane_id();
         const int vs = nip + 8;
-        const bool two = 2 * (size_t) vs <= block_lds_span<T, NB>(nip);
-        for (int b = 0; b < nbd;) {
-            const int nv = (two && b + 1 < nbd) ? 2 : 1;
-            for (int q = 0; q < nv; ++q) {
-                NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
-                for (int j = lane; j < nip; j += WAVE) L.vec[q * vs + j] = xf[j];
+        if constexpr (TM::NWAVE == 1) {
+            // X_F = T^-1 F, two columns per sweep where two vectors fit the LDS span
+            const bool two = 2 * (size_t) vs <= block_lds_span<T, NB>(nip);
+            for (int b = 0; b < nbd;) {
+                const int nv = (two && b + 1 < nbd) ? 2 : 1;
+                for (int q = 0; q < nv; ++q) {
+                    NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
+                    for (int j = lane; j < nip; j += WAVE) L.vec[q * vs + j] = xf[j];
+                }
+                lds_fence();
+                if (nv == 2) thomas_solve<T, NB, 2>(B, F, L.vec, vs);
+                else thomas_solve<T, NB>(B, F, L.vec);
+                for (int q = 0; q < nv; ++q) {
+                    NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
+                    for (int j = lane; j < nip; j += WAVE) xf[j] = L.vec[q * vs + j];
+                }
+                lds_fence();
+                b += nv;
             }
-            lds_fence();
-            if (nv == 2) thomas_solve<T, NB, 2>(B, F, L.vec, vs);
-            else thomas_solve<T, NB>(B, F, L.vec);
-            for (int q = 0; q < nv; ++q) {
-                NUPPN_GLB T *xf = XFg + (size_t) (b + q) * nip;
-                for (int j = lane; j < nip; j += WAVE) xf[j] = L.vec[q * vs + j];
+        } else {
+            // X_F = T^-1 F, one column per wavefront, each in its own LDS vector
+            const int w = TM::wave();
+            NUPPN_LDS T *v = w == 0 ? L.vec : L.wvec + (size_t) (w - 1) * vs;
+            for (int b = w; b < nbd; b += TM::NWAVE) {
+                NUPPN_GLB T *xf = XFg + (size_t) b * nip;
+                for (int j = lane; j < nip; j += WAVE) v[j] = xf[j];
+                lds_fence();
+                thomas_solve<T, NB>(B, F, v);
+                for (int j = lane; j < nip; j += WAVE) xf[j] = v[j];
+                lds_fence();
             }
-            lds_fence();
-            b += nv;
         }
-        wave_sync();
+        TM::sync();
         for (int b2 = 0; b2 < nbd; ++b2)
             for (int b1 = 0; b1 < nbd; ++b1) {
-                double s = 0.0;
-                for (int j = lane; j < nip; j += WAVE)
-                    s += (double) Eg[(size_t) b1 * nip + j] * (double) XFg[(size_t) b2 * nip + j];
-                s = wave_sum(s);
-                if (lane == 0) F.G[b1 + b2 * nbd] -= s;
+                double sacc = 0.0;
+                for (int j = t0; j < nip; j += NT)
+                    sacc += (double) Eg[(size_t) b1 * nip + j] * (double) XFg[(size_t) b2 * nip + j];
+                sacc = team_sum<TM>(sacc, L.red);
+                if (t0 == 0) Gg[b1 + b2 * nbd] -= sacc;
             }
-        wave_sync();
+        TM::sync();
         int gbad = 0;
-        if (lane == 0) {
+        if (t0 == 0) {
             double *G = F.G;
             for (int k = 0; k < nbd; ++k) {
                 int p = k;
@@ -644,46 +736,48 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
                 }
             }
         }
-        gbad = readlane_i(gbad, 0);
-        wave_sync();
-        if (gbad) return 1;
+        if (team_any<TM>(gbad != 0, L.red)) return 1;
     }
     return 0;
 }
 
-// x = A^-1 b with the factors F (b, x: FP64 [n], reduced order, mass-fraction units)
-template <class T, int NB>
+// x = A^-1 b with the factors F (b, x: FP64 [n], reduced order, mass-fraction units).  The
+// Thomas sweeps run on the zone's first wavefront; the scalings and the border correction are
+// shared by the zone's threads.
+template <class T, int NB, class TM>
 __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactors<T> &F, const BlockLds<T> &L,
                             const double *bp, double *xp) {
+    constexpr int NT = TM::NT;
     const NUPPN_GLB double *b = gptr(bp), *rowinv = gptr(F.rowinv), *colsc = gptr(F.colsc);
     const NUPPN_GLB int *vpos = gptr(B.vpos);
     NUPPN_GLB double *x = gptr(xp);
     const NUPPN_GLB T *Eg = gptr(F.E), *XFg = gptr(F.XF);
-    const int lane = lane_id(), nip = B.nip, nbd = B.nbd;
+    const int t0 = TM::tid(), nip = B.nip, nbd = B.nbd;
     NUPPN_LDS T *v = L.vec;
     // an exact power of two brings the largest entry to [1, 2): FP32 then resolves entries down
     // to 2^-126 of the largest one rather than 2^-126 absolute (refinement residuals shrink far
     // below the FP32 range while the abundances they correct stay meaningful)
     double m = 0.0;
-    for (int c = lane; c < B.n; c += WAVE) m = fmax(m, fabs(b[c] * rowinv[vpos[c]]));
-    m = wave_max(m);
+    for (int c = t0; c < B.n; c += NT) m = fmax(m, fabs(b[c] * rowinv[vpos[c]]));
+    m = team_max<TM>(m, L.red);
     const int ex = (m > 0.0 && isfinite(m)) ? ilogb(m) : 0;
     const double sc = ldexp(1.0, -ex), unsc = ldexp(1.0, ex);
-    for (int j = B.ni + lane; j < nip; j += WAVE) v[j] = T(0);
-    for (int c = lane; c < B.n; c += WAVE) v[vpos[c]] = (T) (b[c] * rowinv[vpos[c]] * sc);
-    lds_fence();
-    thomas_solve<T, NB>(B, F, v);
+    for (int j = B.ni + t0; j < nip; j += NT) v[j] = T(0);
+    for (int c = t0; c < B.n; c += NT) v[vpos[c]] = (T) (b[c] * rowinv[vpos[c]] * sc);
+    TM::sync();
+    if (TM::wave() == 0) thomas_solve<T, NB>(B, F, v);
+    TM::sync();
     if (nbd > 0) {
         NUPPN_LDS double *tb = L.bscr;
         for (int bb = 0; bb < nbd; ++bb) {
             double s = 0.0;
-            for (int j = lane; j < nip; j += WAVE) s += (double) Eg[(size_t) bb * nip + j] * (double) v[j];
-            s = wave_sum(s);
-            if (lane == 0) tb[bb] = (double) v[nip + bb] - s;
+            for (int j = t0; j < nip; j += NT) s += (double) Eg[(size_t) bb * nip + j] * (double) v[j];
+            s = team_sum<TM>(s, L.red);
+            if (t0 == 0) tb[bb] = (double) v[nip + bb] - s;
         }
-        lds_fence();
-        // G' u = t with the row interchanges, unit-lower and upper triangular solves (lane 0)
-        if (lane == 0) {
+        TM::sync();
+        // G' u = t with the row interchanges, unit-lower and upper triangular solves (thread 0)
+        if (t0 == 0) {
             for (int k = 0; k < nbd; ++k) {
                 const int p = F.gpiv[k];
                 if (p != k) { const double t = tb[k]; tb[k] = tb[p]; tb[p] = t; }
@@ -696,16 +790,16 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
             }
             for (int k = 0; k < nbd; ++k) v[nip + k] = (T) tb[k];
         }
-        lds_fence();
-        for (int j = lane; j < nip; j += WAVE) {
+        TM::sync();
+        for (int j = t0; j < nip; j += NT) {
             T s = v[j];
             for (int bb = 0; bb < nbd; ++bb) s = fma_t(-XFg[(size_t) bb * nip + j], v[nip + bb], s);
             v[j] = s;
         }
-        lds_fence();
+        TM::sync();
     }
-    for (int c = lane; c < B.n; c += WAVE) x[c] = (double) v[vpos[c]] * colsc[vpos[c]] * unsc;
-    wave_sync();
+    for (int c = t0; c < B.n; c += NT) x[c] = (double) v[vpos[c]] * colsc[vpos[c]] * unsc;
+    TM::sync();
 }
 
 // one or two running sums of a segmented sum
@@ -723,20 +817,21 @@ struct Sum2 {
     __device__ Sum2 lane(int k) const { return {readlane_d(s, k), readlane_d(a, k)}; }
 };
 
-// Segmented sums of val(e, operand) over the terms of S by one wavefront.  Lane l sums its range
-// of terms in order, sixteen independent loads at a time, so that a lane keeps many gathers in
-// flight instead of one short segment after another.  A segment that ends inside a lane's range
-// and did not start before it is finished there; the pieces of the others (the first segment of
-// every lane, and the one still open at its end) are added in lane order afterwards.  Every
-// segment is summed in term order, its pieces in lane order, so the result does not depend on
-// timing.  emit(s, total) is called once per segment, from any lane.
+// Segmented sums of val(e, operand) over the terms of round r of S by one wavefront.  Lane l
+// sums its range of terms in order, sixteen independent loads at a time, so that a lane keeps
+// many gathers in flight instead of one short segment after another.  A segment that ends inside
+// a lane's range and did not start before it is finished there; the pieces of the others (the
+// first segment of every lane, and the one still open at its end) are added in lane order
+// afterwards.  Every segment is summed in term order, its pieces in lane order, so the result
+// does not depend on timing.  emit(s, total) is called once per segment, from any lane.  A round
+// starts and ends at segment boundaries, so the rounds are independent.
 template <class A, class V, class E>
-__device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
+__device__ __forceinline__ void seg_round(const DevSeg &S, int r, V val, E emit) {
     constexpr int BATCH = 16;
     const int l = lane_id();
     const NUPPN_GLB int *eidx = gptr(S.e_idx);
-    const int t0 = gptr(S.lane_t0)[l], t1 = gptr(S.lane_t0)[l + 1];
-    const int s0 = gptr(S.lane_s0)[l];
+    const int t0 = gptr(S.lane_t0)[64 * r + l], t1 = gptr(S.lane_t0)[64 * r + l + 1];
+    const int s0 = gptr(S.lane_s0)[64 * r + l];
     int s = s0;
     A acc{}, first{};
     bool has_first = false, open = false;
@@ -796,7 +891,14 @@ __device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
         }
     }
     if (cur >= 0 && l == 0) emit(cur, tot);
-    wave_sync();
+}
+
+// All rounds of S by the zone's threads: one wavefront takes them one after another, several
+// wavefronts one round each (the same pieces either way, so the same bits).
+template <class A, class TM, class V, class E>
+__device__ __forceinline__ void seg_stream(const DevSeg &S, V val, E emit) {
+    for (int r = TM::wave(); r < kSegRounds; r += TM::NWAVE) seg_round<A>(S, r, val, emit);
+    TM::sync();
 }
 
 // A running sum carried in twice the working precision (Ogita, Rump and Oishi, "Accurate sum
@@ -841,16 +943,18 @@ struct SumDD2 {
 // level: Demmel, Hida, Kahan, Li, Mukherjee and Riedy, "Error bounds from extra-precise
 // iterative refinement", ACM TOMS 32 (2006)).  Returns the componentwise backward error
 // max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager).  lo, absum: [n] scratch.
+template <class TM>
 __device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp,
                                                  const double *rhsp, const double *xp,
                                                  const double *xlop, double *rp,
-                                                 double *absump, double *lop) {
+                                                 double *absump, double *lop,
+                                                 NUPPN_LDS double *red) {
     const NUPPN_GLB double *Acsr = gptr(Acsrp), *rhs = gptr(rhsp), *x = gptr(xp);
     const NUPPN_GLB double *xlo = gptr(xlop);
     const NUPPN_GLB int *csr_col = gptr(B.csr_col);
     NUPPN_GLB double *r = gptr(rp), *absum = gptr(absump), *lo = gptr(lop);
     // the residual's terms are the CSR entries in order: term e is entry e
-    seg_stream<SumDD>(
+    seg_stream<SumDD, TM>(
         B.resid,
         [&](int e, int k) {
             (void) k;
@@ -865,7 +969,7 @@ __device__ __forceinline__ double block_residual(const DevBlock &B, const double
             absum[i] = t.a;
         });
     double om = 0.0;
-    for (int i = lane_id(); i < B.n; i += WAVE) {
+    for (int i = TM::tid(); i < B.n; i += TM::NT) {
         // rhs - (hi + lo): the difference rhs - hi split exactly, then the low parts
         const double b = rhs[i], hi = r[i];
         const double s = b - hi;
@@ -875,8 +979,8 @@ __device__ __forceinline__ double block_residual(const DevBlock &B, const double
         r[i] = ri;
         om = fmax(om, fabs(ri) / (absum[i] + fabs(b) + kAbundanceFloor));
     }
-    om = wave_max(om);
-    wave_sync();
+    om = team_max<TM>(om, red);
+    TM::sync();
     return om;
 }
 

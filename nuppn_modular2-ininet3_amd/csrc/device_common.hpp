@@ -45,6 +45,27 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); _
 // prevented.
 __device__ __forceinline__ void lds_fence() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
+// Ordering point for one wavefront's global and LDS accesses without a workgroup barrier: the
+// other wavefronts of a multi-wavefront workgroup may be anywhere else meanwhile (an s_barrier
+// here would pair with theirs).  In a single-wavefront workgroup it is the same as wave_sync.
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The threads that work on one zone: one wavefront (NW = 1: the plain path and the batched
+// path's many-zone kernels) or the NW wavefronts of one workgroup (the batched path's kernel for
+// passes with few zones).  sync() is then a workgroup barrier; the reductions go through red, a
+// few LDS slots of the workgroup.
+template <int NW>
+struct Team {
+    static constexpr int NWAVE = NW;
+    static constexpr int NT = WAVE * NW;
+    __device__ static int tid() { return NW == 1 ? lane_id() : (int) threadIdx.x; }
+    __device__ static int wave() { return NW == 1 ? 0 : (int) threadIdx.x / WAVE; }
+    __device__ static void sync() { wave_sync(); }
+};
+
 __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
     return v;
@@ -220,23 +241,25 @@ __device__ __forceinline__ const NUPPN_GLB T *gptr(const T *p) {
     return (const NUPPN_GLB T *) p;
 }
 // xn = X/A for considered isotopes (A > 0), xn(gamma) = 1 (jac_rhs.F90:235-240, 312-317)
-__device__ inline void molar(const DevNet &N, const double *xp, double *xnp) {
+// (thread t0 of the nt that share the zone)
+__device__ inline void molar(const DevNet &N, const double *xp, double *xnp, int t0 = lane_id(),
+                             int nt = WAVE) {
     const NUPPN_GLB double *x = gptr(xp), *aw = gptr(N.aw);
     NUPPN_GLB double *xn = gptr(xnp);
-    for (int i = lane_id(); i < N.nsp; i += WAVE)
-        xn[i] = (i != N.igamma && aw[i] > 0.0) ? x[i] / aw[i] : 0.0;
-    if (lane_id() == 0) xn[N.igamma] = 1.0;
+    for (int i = t0; i < N.nsp; i += nt)
+        xn[i] = i == N.igamma ? 1.0 : (aw[i] > 0.0 ? x[i] / aw[i] : 0.0);
 }
 
 // per-reaction quantities: vx (calculate_dxdt flux, jac_rhs.F90:245) and the two Jacobian stems
 // (jac_rhs.F90:355, 359)
 __device__ inline void reaction_terms(const DevNet &N, const double *xnp, const double *vp,
-                                      double *vxp, double *st1p, double *st2p) {
+                                      double *vxp, double *st1p, double *st2p,
+                                      int t0 = lane_id(), int nt = WAVE) {
     const int nre = N.nre;
     const NUPPN_GLB int *k = gptr(N.k);
     const NUPPN_GLB double *xn = gptr(xnp), *v = gptr(vp);
     NUPPN_GLB double *vx = gptr(vxp), *st1 = gptr(st1p), *st2 = gptr(st2p);
-    for (int l = lane_id(); l < nre; l += WAVE) {
+    for (int l = t0; l < nre; l += nt) {
         const int in1 = k[l], n1 = k[nre + l], in2 = k[2 * nre + l], n2 = k[3 * nre + l];
         const double y1 = xn[in1], y2 = xn[in2], vl = v[l];
         vx[l] = kFact[n1] * vl * powi(y1, n1) * y2;

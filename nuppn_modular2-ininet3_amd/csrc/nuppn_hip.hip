@@ -137,6 +137,8 @@ template __global__ void zone_newton_block_kernel<32, 1>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<32, 2>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<64, 1>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<64, 2>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<32>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<64>(KArgs a, BKArgs b);
 template __global__ void zone_bd_block_kernel<32>(KArgs a, BKArgs b);
 template __global__ void zone_bd_block_kernel<64>(KArgs a, BKArgs b);
 }  // namespace nuppn
@@ -261,6 +263,9 @@ struct nuppn_ctx {
     int use_cache = 0;
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
+    size_t blds_team = 0;            // LDS of the multi-wavefront kernel's workgroup
+    int team_ok = 0;                 // the multi-wavefront kernel fits the device
+    int ncu = 0;                     // compute units of the device
     int bmax_slots = 0, bslots = 0;
     int bgrid[3] = {0, 0, 0};   // resident wavefronts of the worker held to 1 / 2 per SIMD
     char *d_bslots = nullptr;
@@ -400,6 +405,11 @@ static const void *block_kernel_ptr(int nb, int method, int waves) {
                       : (const void *) zone_newton_block_kernel<32, 1>;
 }
 
+static const void *team_kernel_ptr(int nb) {
+    return nb == 64 ? (const void *) zone_newton_team_kernel<64>
+                    : (const void *) zone_newton_team_kernel<32>;
+}
+
 // The runtime gives every thread a private (scratch) segment of hipLimitStackSize bytes (1,024
 // by default).  A kernel whose register spills and private arrays need more would run past it,
 // so the limit is raised to the largest private size of the kernels a context launches.
@@ -474,6 +484,24 @@ static int setup_batched(nuppn_ctx *c) {
     c->bmax_slots = std::max(1, std::min(ncu * occ, by_mem));
     c->bgrid[1] = std::max(1, std::min(ncu * occ1, c->bmax_slots));
     c->bgrid[2] = std::max(1, std::min(ncu * occ2, c->bmax_slots));
+    c->ncu = std::max(1, ncu);
+    // the multi-wavefront kernel for passes with few zones: one workgroup of kTeamWaves
+    // wavefronts per zone, one more solve vector in LDS per extra wavefront
+    c->blds_team = rnd(nb == 64 ? block_lds_bytes<double, 64>(b.nip, b.nbd, kTeamWaves)
+                                : block_lds_bytes<double, 32>(b.nip, b.nbd, kTeamWaves));
+    c->team_ok = 0;
+    if (method == 0 && c->blds_team <= 160 * 1024) {
+        const void *tk = team_kernel_ptr(nb);
+        int per_cu = 0;
+        if (ensure_stack(tk)) return NUPPN_ERR_HIP;
+        if ((c->blds_team <= 64 * 1024 ||
+             hipFuncSetAttribute(tk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) c->blds_team) == hipSuccess) &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk, 64 * kTeamWaves,
+                                                         c->blds_team) == hipSuccess &&
+            per_cu >= 1)
+            c->team_ok = 1;
+    }
     c->bnb = nb;
     return 0;
 }
@@ -1011,13 +1039,20 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
                                       : (const void *) zone_newton_kernel,
                                    dim3(std::min(grid_n, active)), dim3(64), args, c->lds, s));
         } else {
-            // the register-lean two-per-SIMD variant only when the single-wavefront one cannot
-            // give every zone of the list its own wavefront
-            const int waves = (!bd && active > c->bgrid[1]) ? 2 : 1;
-            const int grid = std::min({grid_n, c->bgrid[waves], active});
             void *args[] = {&a, &b};
-            HIP_OK(hipLaunchKernel(block_kernel_ptr(c->bnb, bd ? 1 : 0, waves), dim3(grid),
-                                   dim3(64), args, c->blds64, s));
+            // few zones: kTeamWaves wavefronts per zone shorten each one's iteration
+            const int team_max = bo->team_zones < 0 ? c->ncu : bo->team_zones;
+            if (!bd && c->team_ok && active <= team_max) {
+                HIP_OK(hipLaunchKernel(team_kernel_ptr(c->bnb), dim3(std::min(grid_n, active)),
+                                       dim3(64 * kTeamWaves), args, c->blds_team, s));
+            } else {
+                // the register-lean two-per-SIMD variant only when the single-wavefront one
+                // cannot give every zone of the list its own wavefront
+                const int waves = (!bd && active > c->bgrid[1]) ? 2 : 1;
+                const int grid = std::min({grid_n, c->bgrid[waves], active});
+                HIP_OK(hipLaunchKernel(block_kernel_ptr(c->bnb, bd ? 1 : 0, waves), dim3(grid),
+                                       dim3(64), args, c->blds64, s));
+            }
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipEventRecord(c->evn[1], s));
@@ -1112,6 +1147,7 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->max_refine = 12;
     o->refine_tol = 1.0e-17;
     o->accept_tol = 0.0;
+    o->team_zones = -1;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {

@@ -407,10 +407,22 @@ void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, 
         out.e_idx.back() |= (int) (1u << 31);
     }
     const long long nterm = (long long) out.e_idx.size();
-    out.lane_t0.assign(65, 0);
-    out.lane_s0.assign(64, 0);
-    for (int l = 0; l <= 64; ++l) out.lane_t0[l] = (int) ((l * nterm) / 64);
-    for (int l = 0; l < 64; ++l)
+    // round boundaries: the first segment start at or after an even share of the terms
+    std::vector<long long> rb(kSegRounds + 1, nterm);
+    rb[0] = 0;
+    for (int r = 1; r < kSegRounds; ++r) {
+        long long t = std::max(rb[r - 1], (r * nterm) / kSegRounds);
+        while (t < nterm && t > 0 && seg_of[t] == seg_of[t - 1]) ++t;
+        rb[r] = t;
+    }
+    const int NL = 64 * kSegRounds;
+    out.lane_t0.assign(NL + 1, (int) nterm);
+    out.lane_s0.assign(NL, out.nseg);
+    for (int r = 0; r < kSegRounds; ++r) {
+        const long long len = rb[r + 1] - rb[r];
+        for (int l = 0; l < 64; ++l) out.lane_t0[64 * r + l] = (int) (rb[r] + (l * len) / 64);
+    }
+    for (int l = 0; l < NL; ++l)
         out.lane_s0[l] = out.lane_t0[l] < nterm ? seg_of[out.lane_t0[l]] : out.nseg;
 }
 

@@ -32,22 +32,23 @@ struct Symbolic {
 // builds everything but the column order; returns 0 or a negative error
 int build_symbolic(const HostNetwork &net, Symbolic &s);
 
-// Segmented sums in chunks: segment s owns elements [e of its chunks]; chunk c covers elements
-// ch_beg[c] .. ch_beg[c+1]-1 (at most a fixed number) and segment s chunks seg_ch[s] ..
-// seg_ch[s+1]-1.  A wavefront sums the chunks in parallel, then the chunk partials per segment,
-// so a segment with thousands of terms (the light projectiles' rows) does not serialise one lane.
-// Segmented sums as one wavefront streams them: the terms of all segments back to back, in
-// segment order and, inside a segment, in summation order; the last term of every segment carries
-// bit 31 in its operand index, and an empty segment gets one zero term.  Lane l sums terms
-// lane_t0[l] .. lane_t0[l+1]-1 (an even split of the terms); lane_s0[l] is the segment holding
-// term lane_t0[l].
+// Segmented sums as the device streams them: the terms of all segments back to back, in segment
+// order and, inside a segment, in summation order; the last term of every segment carries bit 31
+// in its operand index, and an empty segment gets one zero term.  The terms are cut into
+// kSegRounds rounds at segment boundaries (near-even shares), and every round into 64 lane
+// ranges: lane l of round r sums terms lane_t0[64 r + l] .. lane_t0[64 r + l + 1] - 1, and
+// lane_s0[64 r + l] is the segment holding its first term.  A round is the work of one wavefront
+// pass; a zone's single wavefront takes the rounds one after another, the kSegRounds wavefronts of
+// a multi-wavefront zone one each.  Both sum every segment in the same pieces and order, so the
+// two give the same bits.
+constexpr int kSegRounds = 4;
 struct SegStream {
     int nseg = 0;
     std::vector<int> e_idx;       // operand index, | (1 << 31) on a segment's last term
     std::vector<double> e_coef;   // coefficient
     std::vector<double> e_coef_lo;  // J only: the coefficient's low part (hi + lo its exact value)
-    std::vector<int> lane_t0;     // [65]
-    std::vector<int> lane_s0;     // [64]
+    std::vector<int> lane_t0;     // [64 kSegRounds + 1]
+    std::vector<int> lane_s0;     // [64 kSegRounds]
 };
 
 // Static block structure of the reduced matrix for the pivot-free batched factorisation.

@@ -48,7 +48,8 @@ class ZoneStats(C.Structure):
 class BatchOpts(C.Structure):
     """nuppn_batch_opts: factor precision (32 / 64) and the FP64 refinement tolerances."""
     _fields_ = [("factor_precision", C.c_int), ("max_refine", C.c_int),
-                ("refine_tol", C.c_double), ("accept_tol", C.c_double)]
+                ("refine_tol", C.c_double), ("accept_tol", C.c_double),
+                ("team_zones", C.c_int)]
 
 
 class Network(C.Structure):

@@ -99,6 +99,37 @@ def test_mixed_equals_fp64_factors(c1_full, hip):
     assert np.any(st32["n_factor_fp64"] == 0) and np.any(st32["n_factor_fp64"] > 0)
 
 
+def test_team_kernel_equals_single_wavefront(c1_full, hip):
+    """The four-wavefront kernel (passes whose work list is short) and the one-wavefront kernels
+    give the same bits: the cycle of these cold and hot zones once with every pass on one
+    wavefront per zone (team_zones 0), once with every pass on four (a bound above the list), the
+    same abundances, Ye, codes, sub-steps, Newton iterations and factorisations.  (The segmented
+    sums cut their terms into the same pieces for both; the factors and solves may differ in the
+    last bits, which the refinement to below half an ulp absorbs - now and then with one
+    refinement step more or less, so the solve counts agree only to a few per thousand.)"""
+    import time
+    w, ctx = c1_full
+    sel = np.array(MIXED_ZONES)
+    args = _args(w, sel)
+    out = {}
+    for tz in (0, 1 << 20):
+        t = time.perf_counter()
+        out[tz] = ctx.integrate_batch_mixed(w.x0[sel], *args, opts=hip.batch_opts(team_zones=tz))
+        print(f"team_zones {tz}: {time.perf_counter() - t:.2f} s", flush=True)
+    ie1, y1, ye1, st1 = out[0]
+    ie4, y4, ye4, st4 = out[1 << 20]
+    np.testing.assert_array_equal(ie1, 0)
+    np.testing.assert_array_equal(ie4, ie1)
+    print("solves", st1["n_solve"], st4["n_solve"], "refinement residuals", st1["n_refine"],
+          st4["n_refine"])
+    for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers", "n_factor", "n_factor_fp64"):
+        np.testing.assert_array_equal(st4[f], st1[f], err_msg=f)
+    for f in ("n_solve", "n_refine"):
+        assert np.all(np.abs(st4[f] - st1[f]) <= 0.005 * st1[f] + 1), f
+    np.testing.assert_array_equal(y4, y1)
+    np.testing.assert_array_equal(ye4, ye1)
+
+
 def _exact_oracle(oracle, net):
     return oracle.Oracle(net, oracle.default_knobs(cyminformal=-1.0), oracle.LS_EXACT)
 
