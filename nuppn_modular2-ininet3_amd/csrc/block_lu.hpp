@@ -72,16 +72,16 @@ struct BlockCfg {
 // LDS of one zone: the two block-product operands X1, X2 of the factorisation loop (X2 also
 // holds the Gauss-Jordan pivot columns and rows between the products), which the solve vector
 // vec overlays (the loop and the solves never run at the same time), the border scratch, the
-// team reduction slots and, for a zone of NW > 1 wavefronts, one more solve vector per extra
-// wavefront (wvec: the border columns are solved in parallel).  About 18.9 KB with FP64 factors
-// and one wavefront (NB = 32, 1,100 species), so eight single-wavefront zones share a CU's
-// 160 KB.
+// team reduction slots and, for a zone of several wavefronts, a second operand area X1b, X2b
+// (the bottom-up sweep of the twisted factorisation; later the border columns of the second
+// wavefront pair, from vecb).  About 18.9 KB with FP64 factors and one wavefront (NB = 32,
+// 1,100 species), so eight single-wavefront zones share a CU's 160 KB.
 template <class T>
 struct BlockLds {
     NUPPN_LDS T *X1, *X2, *vec;
     NUPPN_LDS double *bscr;   // [16] border solve scratch
     NUPPN_LDS double *red;    // [16] team reductions
-    NUPPN_LDS T *wvec;        // [(NW - 1) (nip + 8)]
+    NUPPN_LDS T *X1b, *X2b, *vecb;
 };
 
 template <class T, int NB>
@@ -94,8 +94,7 @@ __host__ __device__ constexpr size_t block_lds_span(int nip) {
 template <class T, int NB>
 __host__ __device__ constexpr size_t block_lds_bytes(int nip, int nbd, int nw = 1) {
     (void) nbd;   // nbd <= 8: the border entries fit the 8 slots after nip
-    return sizeof(T) * block_lds_span<T, NB>(nip) + 16 + 32 * sizeof(double) +
-           (size_t) (nw - 1) * sizeof(T) * (nip + 8);
+    return sizeof(T) * block_lds_span<T, NB>(nip) * (nw > 1 ? 2 : 1) + 16 + 32 * sizeof(double);
 }
 
 template <class T, int NB>
@@ -111,7 +110,9 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
     q = (NUPPN_LDS char *) (((uintptr_t) q + 15) & ~(uintptr_t) 15);
     L.bscr = (NUPPN_LDS double *) q;
     L.red = L.bscr + 16;
-    L.wvec = (NUPPN_LDS T *) (L.red + 16);
+    L.X1b = (NUPPN_LDS T *) (L.red + 16);
+    L.X2b = L.X1b + NB * C::LD;
+    L.vecb = L.X1b;
     return L;
 }
 
@@ -508,6 +509,110 @@ __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFacto
     }
 }
 
+// the middle block of the twisted factorisation: top-down above it, bottom-up below it
+__device__ __forceinline__ int twist_block(int nblk) { return nblk / 2; }
+
+// v_q <- T^-1 v_q (q < nv <= NV) with the twisted factors of block_factor (several
+// wavefronts): a pair of wavefronts, role 0 the rows above the middle block m (forward
+// v_i = S_i^-1 (v_i - B_{i-1} v_{i-1}), back v_i -= W_i v_{i+1}), role 1 the rows below it
+// (forward v_i = R_i^-1 (v_i - C_i v_{i+1}), back v_i -= V_i v_{i-1}), the middle block
+// v_m = Z_m^-1 (v_m - B_{m-1} v_{m-1} - C_m v_{m+1}) by role 0 in between.  Every wavefront of
+// the zone calls it (the three barriers pair across the workgroup); active false: a wavefront
+// that only takes part in the barriers.
+template <class T, int NB, int NV, class TM>
+__device__ __forceinline__ void twisted_solve(const DevBlock &B, const BlockFactors<T> &F,
+                                              NUPPN_LDS T *v, int vs, int nv, int role,
+                                              bool active) {
+    constexpr int CPL = NB / BlockCfg<T, NB>::H;
+    constexpr size_t BS = (size_t) NB * NB;
+    const int rr = lane_id() % NB, h = lane_id() / NB;
+    const int N = B.nblk, m = twist_block(N);
+    // v_i <- v_i - M x (x = block j of each vector) or v_i <- M v_i
+    auto sub = [&](const T (&M)[CPL], int i, int j) {
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            acc[q] = q < nv ? matvec_regs<T, NB>(M, v + q * vs + j * NB) : T(0);
+        lds_fence();
+        if (h == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                if (q < nv) v[q * vs + i * NB + rr] = v[q * vs + i * NB + rr] - acc[q];
+        lds_fence();
+    };
+    auto mul = [&](const T (&M)[CPL], int i) {
+        T acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            acc[q] = q < nv ? matvec_regs<T, NB>(M, v + q * vs + i * NB) : T(0);
+        lds_fence();
+        if (h == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                if (q < nv) v[q * vs + i * NB + rr] = acc[q];
+        lds_fence();
+    };
+    auto copy = [&](T (&d)[CPL], const T (&src)[CPL]) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) d[j] = src[j];
+    };
+    T mS[CPL], mO[CPL], nS[CPL], nO[CPL];
+    if (active && role == 0 && m > 0) {
+        load_mv<T, NB>(F.Sinv, mS);
+        for (int i = 0; i < m; ++i) {
+            if (i + 1 < m) load_mv<T, NB>(F.Sinv + (size_t) (i + 1) * BS, nS);
+            load_mv<T, NB>(F.Bm + (size_t) i * BS, nO);                       // B_i, next step
+            if (i > 0) sub(mO, i, i - 1);
+            mul(mS, i);
+            copy(mS, nS);
+            copy(mO, nO);
+        }
+    } else if (active && role == 1 && N - 1 > m) {
+        load_mv<T, NB>(F.Sinv + (size_t) (N - 1) * BS, mS);
+        for (int i = N - 1; i > m; --i) {
+            if (i - 1 > m) load_mv<T, NB>(F.Sinv + (size_t) (i - 1) * BS, nS);
+            load_mv<T, NB>(F.W + (size_t) (i - 1) * BS, nO);                  // C_{i-1}, next
+            if (i < N - 1) sub(mO, i, i + 1);
+            mul(mS, i);
+            copy(mS, nS);
+            copy(mO, nO);
+        }
+    }
+    wave_fence();
+    TM::sync();
+    if (active && role == 0) {
+        if (m > 0) {
+            load_mv<T, NB>(F.Bm + (size_t) (m - 1) * BS, mO);
+            sub(mO, m, m - 1);
+        }
+        if (m < N - 1) {
+            load_mv<T, NB>(F.W + (size_t) m * BS, mO);
+            sub(mO, m, m + 1);
+        }
+        load_mv<T, NB>(F.Sinv + (size_t) m * BS, mS);
+        mul(mS, m);
+        wave_fence();
+    }
+    TM::sync();
+    if (active && role == 0 && m > 0) {
+        load_mv<T, NB>(F.W + (size_t) (m - 1) * BS, mO);
+        for (int i = m - 1; i >= 0; --i) {
+            if (i > 0) load_mv<T, NB>(F.W + (size_t) (i - 1) * BS, nO);
+            sub(mO, i, i + 1);
+            copy(mO, nO);
+        }
+    } else if (active && role == 1 && N - 1 > m) {
+        load_mv<T, NB>(F.Bm + (size_t) m * BS, mO);                            // V_{m+1}
+        for (int i = m + 1; i < N; ++i) {
+            if (i + 1 < N) load_mv<T, NB>(F.Bm + (size_t) i * BS, nO);         // V_{i+1}
+            sub(mO, i, i - 1);
+            copy(mO, nO);
+        }
+    }
+    wave_fence();
+    TM::sync();
+}
+
 // mass fractions far below any reported abundance (1e-20) and below the reference's clipping
 // of abundances to 1e-99 (y < 1e-27, utils.F90:726-740): the refinement tests measure every
 // species against this floor at least
@@ -523,73 +628,107 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// The block Thomas loop of block_factor on one wavefront (the dense blocks are in place): S_i^-1
-// by Gauss-Jordan, W_i = S_i^-1 C_i and S_{i+1} = D_{i+1} - B_i W_i.  Only wavefront-local
-// ordering points, so in a multi-wavefront zone the other wavefronts wait at their next barrier.
-// Returns 1 when an inverse is not finite.
-template <class T, int NB>
-__device__ __forceinline__ int block_thomas_factor(const DevBlock &B, const BlockFactors<T> &F,
-                                                   const BlockLds<T> &L) {
+// nsteps elimination steps of a block sweep on one wavefront, the dense blocks in place (D_i at
+// Sinv[i], C_i at W[i], B_i at Bm[i]).  Step on block i, with a = the block's updated diagonal:
+//   top-down (UP false):  a <- S_i^-1 -> Sinv[i];  W_i = S_i^-1 C_i -> W[i];
+//                         a <- D_{i+1} - B_i W_i, next block i + 1;
+//   bottom-up (UP true):  a <- R_i^-1 -> Sinv[i];  V_i = R_i^-1 B_{i-1} -> Bm[i-1];
+//                         a <- D_{i-1} - C_{i-1} V_i, next block i - 1
+// (on the last step with zero_last, a <- -C_{i-1} V_i: the middle block's share).  a holds the
+// next block's updated diagonal on return.  The operands of a step's products load before its
+// inversion, so their latency overlaps it.  Only wavefront-local ordering points, so in a
+// multi-wavefront zone the other wavefronts may meanwhile run their own sweep or wait at their
+// next barrier.  Returns 1 when an inverse is not finite.
+template <class T, int NB, bool UP>
+__device__ __forceinline__ int block_sweep(const BlockFactors<T> &F, NUPPN_LDS T *X1,
+                                           NUPPN_LDS T *X2, int i0, int nsteps, bool zero_last,
+                                           Regs<T, NB> &a) {
     using C = BlockCfg<T, NB>;
     constexpr size_t BS = (size_t) NB * NB;
-    const int nblk = B.nblk;
-    Regs<T, NB> a, cb, dn, bb;
-    MTiles<T> dnt;   // NB = 32: D_{i+1} in the matrix-core tile layout
-    glb_load_cm<T, NB>(F.Sinv, a);
-    for (int i = 0; i < nblk; ++i) {
-        const bool more = i + 1 < nblk;
-        if (more) {   // operands of this step's products, independent of the inversion
-            glb_load_cm<T, NB>(F.W + i * BS, cb);
-            glb_load_cm<T, NB>(F.Bm + i * BS, bb);
-            if constexpr (NB == 32) mtile_load_glb<T>(F.Sinv + (i + 1) * BS, dnt);
-            else glb_load_cm<T, NB>(F.Sinv + (i + 1) * BS, dn);
+    Regs<T, NB> cb, dn, bb;
+    MTiles<T> dnt;   // NB = 32: the next diagonal block in the matrix-core tile layout
+    for (int k = 0; k < nsteps; ++k) {
+        const int i = UP ? i0 - k : i0 + k;
+        const int nx = UP ? i - 1 : i + 1;                    // the next block
+        T *m1 = UP ? F.Bm + (size_t) (i - 1) * BS : F.W + (size_t) i * BS;    // C_i / B_{i-1}
+        T *m2 = UP ? F.W + (size_t) (i - 1) * BS : F.Bm + (size_t) i * BS;    // B_i / C_{i-1}
+        const bool zero = zero_last && k == nsteps - 1;
+        glb_load_cm<T, NB>(m1, cb);
+        glb_load_cm<T, NB>(m2, bb);
+        if constexpr (NB == 32) {
+            if (zero) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) dnt.t[q] = typename MVec<T>::type{0, 0, 0, 0};
+            } else {
+                mtile_load_glb<T>(F.Sinv + (size_t) nx * BS, dnt);
+            }
+        } else {
+            if (zero) {
+#pragma unroll
+                for (int t = 0; t < C::R; ++t) dn.v[t] = T(0);
+            } else {
+                glb_load_cm<T, NB>(F.Sinv + (size_t) nx * BS, dn);
+            }
         }
-        gj_inverse<T, NB>(a, L.X2);   // X2 is free between the products
-        glb_store_cm<T, NB>(F.Sinv + i * BS, a);
+        gj_inverse<T, NB>(a, X2);   // X2 is free between the products
+        glb_store_cm<T, NB>(F.Sinv + (size_t) i * BS, a);
         bool fin = true;
 #pragma unroll
         for (int t = 0; t < C::R; ++t) fin = fin && isfinite((double) a.v[t]);
         if (ballot(!fin)) return 1;
-        if (!more) break;
         lds_fence();
-        lds_store_cm<T, NB>(L.X1, a);
-        lds_store_rm<T, NB>(L.X2, cb);
+        lds_store_cm<T, NB>(X1, a);
+        lds_store_rm<T, NB>(X2, cb);
         lds_fence();
         if constexpr (NB == 32) {
-            MTiles<T> w;   // W_i = S_i^-1 C_i
+            MTiles<T> w;   // the first product
 #pragma unroll
             for (int q = 0; q < 4; ++q) w.t[q] = typename MVec<T>::type{0, 0, 0, 0};
-            mfma_gemm32<T, C::LD>(L.X1, L.X2, w, false);
-            // W_i to global column-major through LDS (the tile layout scatters a lane's rows),
-            // and row-major into X2 for the second product
+            mfma_gemm32<T, C::LD>(X1, X2, w, false);
+            // to global column-major through LDS (the tile layout scatters a lane's rows), and
+            // row-major into X2 for the second product
             lds_fence();
-            mtile_store_lds_cm<T, C::LD>(L.X1, w);
-            mtile_store_lds_rm<T, C::LD>(L.X2, w);
+            mtile_store_lds_cm<T, C::LD>(X1, w);
+            mtile_store_lds_rm<T, C::LD>(X2, w);
             lds_fence();
-            lds_load_cm<T, NB>(L.X1, a);
-            glb_store_cm<T, NB>(F.W + i * BS, a);
+            lds_load_cm<T, NB>(X1, a);
+            glb_store_cm<T, NB>(m1, a);
             lds_fence();
-            lds_store_cm<T, NB>(L.X1, bb);
+            lds_store_cm<T, NB>(X1, bb);
             lds_fence();
-            mfma_gemm32<T, C::LD>(L.X1, L.X2, dnt, true);                  // D - B_i W_i
+            mfma_gemm32<T, C::LD>(X1, X2, dnt, true);                  // D - (second) (first)
             lds_fence();
-            mtile_store_lds_cm<T, C::LD>(L.X1, dnt);
+            mtile_store_lds_cm<T, C::LD>(X1, dnt);
             lds_fence();
-            lds_load_cm<T, NB>(L.X1, a);
+            lds_load_cm<T, NB>(X1, a);
         } else {
-            gemm_lds<T, NB>(L.X1, L.X2, a);                              // W_i = S_i^-1 C_i
-            glb_store_cm<T, NB>(F.W + i * BS, a);
+            gemm_lds<T, NB>(X1, X2, a);                              // the first product
+            glb_store_cm<T, NB>(m1, a);
             lds_fence();
-            lds_store_rm<T, NB>(L.X2, a);
-            lds_store_cm<T, NB>(L.X1, bb);
+            lds_store_rm<T, NB>(X2, a);
+            lds_store_cm<T, NB>(X1, bb);
             lds_fence();
 #pragma unroll
             for (int t = 0; t < C::R; ++t) a.v[t] = dn.v[t];
-            gemm_lds_sub<T, NB>(L.X1, L.X2, a);                          // D - B_i W_i
+            gemm_lds_sub<T, NB>(X1, X2, a);                          // D - (second) (first)
         }
     }
     wave_fence();
     return 0;
+}
+
+// The last (middle) block: a <- a^-1 -> Sinv[i]; 1 when not finite
+template <class T, int NB>
+__device__ __forceinline__ int block_last_inverse(const BlockFactors<T> &F, NUPPN_LDS T *X2,
+                                                  int i, Regs<T, NB> &a) {
+    using C = BlockCfg<T, NB>;
+    gj_inverse<T, NB>(a, X2);
+    glb_store_cm<T, NB>(F.Sinv + (size_t) i * NB * NB, a);
+    bool fin = true;
+#pragma unroll
+    for (int t = 0; t < C::R; ++t) fin = fin && isfinite((double) a.v[t]);
+    wave_fence();
+    return ballot(!fin) ? 1 : 0;
 }
 
 // Factor the reduced matrix (FP64 CSC values Aval) into F with the zone's threads (TM).  Returns
@@ -646,7 +785,39 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         TM::sync();
     }
     int fail = 0;
-    if (TM::wave() == 0) fail = block_thomas_factor<T, NB>(B, F, L);
+    if constexpr (TM::NWAVE == 1) {
+        // block Thomas: S_0 = D_0, W_i = S_i^-1 C_i, S_{i+1} = D_{i+1} - B_i W_i
+        Regs<T, NB> a;
+        glb_load_cm<T, NB>(F.Sinv, a);
+        fail = block_sweep<T, NB, false>(F, L.X1, L.X2, 0, nblk - 1, false, a);
+        if (!fail) fail = block_last_inverse<T, NB>(F, L.X2, nblk - 1, a);
+    } else {
+        // twisted (two-ended) block factorisation: wavefront 0 eliminates top-down to the middle
+        // block m, wavefront 1 bottom-up to it (R_{N-1} = D_{N-1}, V_i = R_i^-1 B_{i-1},
+        // R_{i-1} = D_{i-1} - C_{i-1} V_i), and the middle block is
+        // Z_m = D_m - B_{m-1} W_{m-1} - C_m V_{m+1}: half the dependent chain of block Thomas
+        const int m = twist_block(nblk);
+        Regs<T, NB> a;
+        if (TM::wave() == 0) {
+            glb_load_cm<T, NB>(F.Sinv, a);
+            fail = block_sweep<T, NB, false>(F, L.X1, L.X2, 0, m, false, a);
+        } else if (TM::wave() == 1 && nblk - 1 > m) {
+            glb_load_cm<T, NB>(F.Sinv + (size_t) (nblk - 1) * BS, a);
+            fail = block_sweep<T, NB, true>(F, L.X1b, L.X2b, nblk - 1, nblk - 1 - m, true, a);
+            lds_store_cm<T, NB>(L.X1b, a);   // -C_m V_{m+1}
+            wave_fence();
+        }
+        TM::sync();
+        if (TM::wave() == 0 && !fail) {
+            if (nblk - 1 > m) {
+                Regs<T, NB> p;
+                lds_load_cm<T, NB>(L.X1b, p);
+#pragma unroll
+                for (int t = 0; t < BlockCfg<T, NB>::R; ++t) a.v[t] = a.v[t] + p.v[t];
+            }
+            fail = block_last_inverse<T, NB>(F, L.X2, m, a);
+        }
+    }
     if (team_any<TM>(fail != 0, L.red)) return 1;
     // border: E, X_F = T^-1 F, G' = G - E X_F and its LU with partial pivoting
     const int nbd = B.nbd, nip = B.nip;
@@ -690,16 +861,29 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
                 b += nv;
             }
         } else {
-            // X_F = T^-1 F, one column per wavefront, each in its own LDS vector
-            const int w = TM::wave();
-            NUPPN_LDS T *v = w == 0 ? L.vec : L.wvec + (size_t) (w - 1) * vs;
-            for (int b = w; b < nbd; b += TM::NWAVE) {
-                NUPPN_GLB T *xf = XFg + (size_t) b * nip;
-                for (int j = lane; j < nip; j += WAVE) v[j] = xf[j];
-                lds_fence();
-                thomas_solve<T, NB>(B, F, v);
-                for (int j = lane; j < nip; j += WAVE) xf[j] = v[j];
-                lds_fence();
+            // X_F = T^-1 F with the twisted factors: wavefronts 0, 1 take two columns (one if two
+            // vectors do not fit the LDS span), wavefronts 2, 3 the next column, round by round
+            const bool two = 2 * (size_t) vs <= block_lds_span<T, NB>(nip);
+            const int w = TM::wave(), pair = w >> 1, role = w & 1;
+            const int nv0 = two ? 2 : 1, per_round = nv0 + (TM::NWAVE >= 4 ? 1 : 0);
+            for (int b = 0; b < nbd; b += per_round) {
+                const int c0 = pair == 0 ? b : b + nv0;
+                const int nv = pair == 0 ? min(nv0, nbd - b) : (c0 < nbd ? 1 : 0);
+                const bool act = pair < 2 && nv > 0;
+                NUPPN_LDS T *v = pair == 0 ? L.vec : L.vecb;
+                if (act && role == 0)
+                    for (int q = 0; q < nv; ++q) {
+                        NUPPN_GLB T *xf = XFg + (size_t) (c0 + q) * nip;
+                        for (int j = lane; j < nip; j += WAVE) v[q * vs + j] = xf[j];
+                    }
+                TM::sync();
+                twisted_solve<T, NB, 2, TM>(B, F, v, vs, act ? nv : 1, role, act);
+                if (act && role == 0)
+                    for (int q = 0; q < nv; ++q) {
+                        NUPPN_GLB T *xf = XFg + (size_t) (c0 + q) * nip;
+                        for (int j = lane; j < nip; j += WAVE) xf[j] = v[q * vs + j];
+                    }
+                TM::sync();
             }
         }
         TM::sync();
@@ -765,7 +949,8 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
     for (int j = B.ni + t0; j < nip; j += NT) v[j] = T(0);
     for (int c = t0; c < B.n; c += NT) v[vpos[c]] = (T) (b[c] * rowinv[vpos[c]] * sc);
     TM::sync();
-    if (TM::wave() == 0) thomas_solve<T, NB>(B, F, v);
+    if constexpr (TM::NWAVE == 1) thomas_solve<T, NB>(B, F, v);
+    else twisted_solve<T, NB, 1, TM>(B, F, v, 0, 1, TM::wave(), TM::wave() < 2);
     TM::sync();
     if (nbd > 0) {
         NUPPN_LDS double *tb = L.bscr;

@@ -103,10 +103,12 @@ def test_team_kernel_equals_single_wavefront(c1_full, hip):
     """The four-wavefront kernel (passes whose work list is short) and the one-wavefront kernels
     give the same bits: the cycle of these cold and hot zones once with every pass on one
     wavefront per zone (team_zones 0), once with every pass on four (a bound above the list), the
-    same abundances, Ye, codes, sub-steps, Newton iterations and factorisations.  (The segmented
-    sums cut their terms into the same pieces for both; the factors and solves may differ in the
-    last bits, which the refinement to below half an ulp absorbs - now and then with one
-    refinement step more or less, so the solve counts agree only to a few per thousand.)"""
+    same abundances, Ye, codes, sub-steps and Newton iterations.  The segmented sums cut their
+    terms into the same pieces for both; the factorisations differ (block Thomas on one
+    wavefront, the twisted two-ended form on four), and so do the factors' rounding errors and
+    the refinement steps they need - with FP32 factors the twisted form contracts better and keeps
+    some zones off FP64 factors longer - but the refinement to below half an ulp of the FP64
+    system's solution gives every Newton iterate the same bits."""
     import time
     w, ctx = c1_full
     sel = np.array(MIXED_ZONES)
@@ -120,12 +122,10 @@ def test_team_kernel_equals_single_wavefront(c1_full, hip):
     ie4, y4, ye4, st4 = out[1 << 20]
     np.testing.assert_array_equal(ie1, 0)
     np.testing.assert_array_equal(ie4, ie1)
-    print("solves", st1["n_solve"], st4["n_solve"], "refinement residuals", st1["n_refine"],
-          st4["n_refine"])
-    for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers", "n_factor", "n_factor_fp64"):
+    for f in ("n_factor", "n_factor_fp64", "n_solve", "n_refine"):
+        print(f, st1[f], st4[f])
+    for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers"):
         np.testing.assert_array_equal(st4[f], st1[f], err_msg=f)
-    for f in ("n_solve", "n_refine"):
-        assert np.all(np.abs(st4[f] - st1[f]) <= 0.005 * st1[f] + 1), f
     np.testing.assert_array_equal(y4, y1)
     np.testing.assert_array_equal(ye4, ye1)
 
