@@ -174,15 +174,19 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
  * switches to FP64 factors for the rest of the call and the iteration is redone; with FP64
  * factors the last iterate is kept.
  * team_zones: a pass whose work list holds at most this many zones runs every zone on a
- * workgroup of four wavefronts (shared assembly, residuals and border solves; a shorter
- * iteration), a longer list on one wavefront per zone.  -1 (default): the device's compute-unit
- * count; 0: never.  Both kernels give the same results bit for bit. */
+ * workgroup of several wavefronts (shared assembly and residuals, a twisted two-ended
+ * factorisation: a shorter iteration), a longer list on one wavefront per zone.  -1 (default):
+ * twice the device's compute-unit count; 0: never.
+ * team_waves: the wavefronts of such a workgroup, 2 or 4; 0 (default): 4 when the list has no
+ * more zones than the device has compute units, else 2.
+ * Every choice gives the same results bit for bit. */
 typedef struct {
     int factor_precision;   /* 32 (default) or 64 */
     int max_refine;         /* refinement steps per solve (12) */
     double refine_tol;      /* 1e-17: below half an ulp of every new abundance */
     double accept_tol;      /* 0 (off) */
     int team_zones;         /* -1 (default), 0 or a zone count */
+    int team_waves;         /* 0 (default), 2 or 4 */
 } nuppn_batch_opts;
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o);

@@ -476,21 +476,19 @@ __global__ void __launch_bounds__(64, W) zone_newton_block_kernel(KArgs a, BKArg
         zone_newton_block<NB, Team<1>>(a, b, slot, lds, a.Z.active[i]);
 }
 
-// wavefronts per zone of the kernel for passes with few zones
-constexpr int kTeamWaves = kSegRounds;
-
-// The same iteration with kTeamWaves wavefronts per zone (one workgroup, one zone per CU): the
-// assembly streams, the residuals, the dense-block set-up, the border columns and every
-// element-wise pass are shared; the block Thomas loop and the solves' sweeps stay on the first
-// wavefront.  For passes whose list has no more zones than the device has CUs, where the
-// single-wavefront iteration's latency sets the pass time.
-template <int NB>
-__global__ void __launch_bounds__(64 * kTeamWaves, 1) zone_newton_team_kernel(KArgs a, BKArgs b) {
+// The same iteration with NW = 2 or 4 wavefronts per zone (one workgroup per zone; with the
+// full register file one wavefront per SIMD, so 4 / NW zones per CU): the assembly streams, the
+// residuals, the dense-block set-up, the border columns and every element-wise pass are shared,
+// and the block factorisation and solves run twisted, top-down on wavefront 0 and bottom-up on
+// wavefront 1.  For passes whose list is short, where one zone's iteration latency sets the pass
+// time.
+template <int NB, int NW>
+__global__ void __launch_bounds__(64 * NW, 1) zone_newton_team_kernel(KArgs a, BKArgs b) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
     for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
-        zone_newton_block<NB, Team<kTeamWaves>>(a, b, slot, lds, a.Z.active[i]);
+        zone_newton_block<NB, Team<NW>>(a, b, slot, lds, a.Z.active[i]);
 }
 
 // every zone in PH_NEWTON under integration_method 1: one Bader-Deuflhard step each (a kernel

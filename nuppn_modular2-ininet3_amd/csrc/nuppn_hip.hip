@@ -137,8 +137,10 @@ template __global__ void zone_newton_block_kernel<32, 1>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<32, 2>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<64, 1>(KArgs a, BKArgs b);
 template __global__ void zone_newton_block_kernel<64, 2>(KArgs a, BKArgs b);
-template __global__ void zone_newton_team_kernel<32>(KArgs a, BKArgs b);
-template __global__ void zone_newton_team_kernel<64>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<32, 2>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<32, 4>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<64, 2>(KArgs a, BKArgs b);
+template __global__ void zone_newton_team_kernel<64, 4>(KArgs a, BKArgs b);
 template __global__ void zone_bd_block_kernel<32>(KArgs a, BKArgs b);
 template __global__ void zone_bd_block_kernel<64>(KArgs a, BKArgs b);
 }  // namespace nuppn
@@ -263,8 +265,8 @@ struct nuppn_ctx {
     int use_cache = 0;
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
-    size_t blds_team = 0;            // LDS of the multi-wavefront kernel's workgroup
-    int team_ok = 0;                 // the multi-wavefront kernel fits the device
+    size_t blds_team = 0;            // LDS of a multi-wavefront kernel's workgroup
+    int team_ok[5] = {};             // [2], [4]: that multi-wavefront kernel fits the device
     int ncu = 0;                     // compute units of the device
     int bmax_slots = 0, bslots = 0;
     int bgrid[3] = {0, 0, 0};   // resident wavefronts of the worker held to 1 / 2 per SIMD
@@ -405,9 +407,12 @@ static const void *block_kernel_ptr(int nb, int method, int waves) {
                       : (const void *) zone_newton_block_kernel<32, 1>;
 }
 
-static const void *team_kernel_ptr(int nb) {
-    return nb == 64 ? (const void *) zone_newton_team_kernel<64>
-                    : (const void *) zone_newton_team_kernel<32>;
+static const void *team_kernel_ptr(int nb, int nw) {
+    if (nb == 64)
+        return nw == 4 ? (const void *) zone_newton_team_kernel<64, 4>
+                       : (const void *) zone_newton_team_kernel<64, 2>;
+    return nw == 4 ? (const void *) zone_newton_team_kernel<32, 4>
+                   : (const void *) zone_newton_team_kernel<32, 2>;
 }
 
 // The runtime gives every thread a private (scratch) segment of hipLimitStackSize bytes (1,024
@@ -485,22 +490,23 @@ static int setup_batched(nuppn_ctx *c) {
     c->bgrid[1] = std::max(1, std::min(ncu * occ1, c->bmax_slots));
     c->bgrid[2] = std::max(1, std::min(ncu * occ2, c->bmax_slots));
     c->ncu = std::max(1, ncu);
-    // the multi-wavefront kernel for passes with few zones: one workgroup of kTeamWaves
-    // wavefronts per zone, one more solve vector in LDS per extra wavefront
-    c->blds_team = rnd(nb == 64 ? block_lds_bytes<double, 64>(b.nip, b.nbd, kTeamWaves)
-                                : block_lds_bytes<double, 32>(b.nip, b.nbd, kTeamWaves));
-    c->team_ok = 0;
-    if (method == 0 && c->blds_team <= 160 * 1024) {
-        const void *tk = team_kernel_ptr(nb);
+    // the multi-wavefront kernels for passes with few zones: one workgroup of 2 or 4 wavefronts
+    // per zone, with a second block-operand area in LDS for the bottom-up sweep
+    c->blds_team = rnd(nb == 64 ? block_lds_bytes<double, 64>(b.nip, b.nbd, 2)
+                                : block_lds_bytes<double, 32>(b.nip, b.nbd, 2));
+    for (int nw = 2; nw <= 4; nw += 2) {
+        c->team_ok[nw] = 0;
+        if (method != 0 || c->blds_team > 160 * 1024) continue;
+        const void *tk = team_kernel_ptr(nb, nw);
         int per_cu = 0;
         if (ensure_stack(tk)) return NUPPN_ERR_HIP;
         if ((c->blds_team <= 64 * 1024 ||
              hipFuncSetAttribute(tk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int) c->blds_team) == hipSuccess) &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk, 64 * kTeamWaves,
-                                                         c->blds_team) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk, 64 * nw, c->blds_team) ==
+                hipSuccess &&
             per_cu >= 1)
-            c->team_ok = 1;
+            c->team_ok[nw] = 1;
     }
     c->bnb = nb;
     return 0;
@@ -1040,11 +1046,14 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
                                    dim3(std::min(grid_n, active)), dim3(64), args, c->lds, s));
         } else {
             void *args[] = {&a, &b};
-            // few zones: kTeamWaves wavefronts per zone shorten each one's iteration
-            const int team_max = bo->team_zones < 0 ? c->ncu : bo->team_zones;
-            if (!bd && c->team_ok && active <= team_max) {
-                HIP_OK(hipLaunchKernel(team_kernel_ptr(c->bnb), dim3(std::min(grid_n, active)),
-                                       dim3(64 * kTeamWaves), args, c->blds_team, s));
+            // few zones: several wavefronts per zone shorten each one's iteration (four when the
+            // list fits one zone per CU, else two)
+            const int team_max = bo->team_zones < 0 ? 2 * c->ncu : bo->team_zones;
+            int nw = bo->team_waves ? bo->team_waves : (active <= c->ncu ? 4 : 2);
+            if (!c->team_ok[nw]) nw = c->team_ok[2] ? 2 : 0;
+            if (!bd && nw && active <= team_max) {
+                HIP_OK(hipLaunchKernel(team_kernel_ptr(c->bnb, nw), dim3(std::min(grid_n, active)),
+                                       dim3(64 * nw), args, c->blds_team, s));
             } else {
                 // the register-lean two-per-SIMD variant only when the single-wavefront one
                 // cannot give every zone of the list its own wavefront
@@ -1129,7 +1138,8 @@ static int host_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, con
 static int mixed_opts(const nuppn_batch_opts *opts, nuppn_batch_opts *o) {
     if (opts) *o = *opts;
     else nuppn_default_batch_opts(o);
-    if ((o->factor_precision != 32 && o->factor_precision != 64) || o->max_refine < 1)
+    if ((o->factor_precision != 32 && o->factor_precision != 64) || o->max_refine < 1 ||
+        (o->team_waves != 0 && o->team_waves != 2 && o->team_waves != 4))
         return NUPPN_ERR_ARG;
     return NUPPN_OK;
 }
@@ -1148,6 +1158,7 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->refine_tol = 1.0e-17;
     o->accept_tol = 0.0;
     o->team_zones = -1;
+    o->team_waves = 0;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {

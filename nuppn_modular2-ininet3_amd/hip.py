@@ -49,7 +49,7 @@ class BatchOpts(C.Structure):
     """nuppn_batch_opts: factor precision (32 / 64) and the FP64 refinement tolerances."""
     _fields_ = [("factor_precision", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("accept_tol", C.c_double),
-                ("team_zones", C.c_int)]
+                ("team_zones", C.c_int), ("team_waves", C.c_int)]
 
 
 class Network(C.Structure):

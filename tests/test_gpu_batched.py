@@ -100,34 +100,36 @@ def test_mixed_equals_fp64_factors(c1_full, hip):
 
 
 def test_team_kernel_equals_single_wavefront(c1_full, hip):
-    """The four-wavefront kernel (passes whose work list is short) and the one-wavefront kernels
-    give the same bits: the cycle of these cold and hot zones once with every pass on one
-    wavefront per zone (team_zones 0), once with every pass on four (a bound above the list), the
-    same abundances, Ye, codes, sub-steps and Newton iterations.  The segmented sums cut their
-    terms into the same pieces for both; the factorisations differ (block Thomas on one
-    wavefront, the twisted two-ended form on four), and so do the factors' rounding errors and
-    the refinement steps they need - with FP32 factors the twisted form contracts better and keeps
-    some zones off FP64 factors longer - but the refinement to below half an ulp of the FP64
-    system's solution gives every Newton iterate the same bits."""
+    """The multi-wavefront kernels (passes whose work list is short) and the one-wavefront
+    kernels give the same bits: the cycle of these cold and hot zones once with every pass on one
+    wavefront per zone (team_zones 0), then with every pass on four and on two (a bound above
+    the list), the same abundances, Ye, codes, sub-steps and Newton iterations.  The segmented
+    sums cut their terms into the same pieces for all; the factorisations differ (block Thomas on
+    one wavefront, the twisted two-ended form on two or four), and so do the factors' rounding
+    errors and the refinement steps they need - with FP32 factors the twisted form contracts
+    better and keeps some zones off FP64 factors longer - but the refinement to below half an ulp
+    of the FP64 system's solution gives every Newton iterate the same bits."""
     import time
     w, ctx = c1_full
     sel = np.array(MIXED_ZONES)
     args = _args(w, sel)
     out = {}
-    for tz in (0, 1 << 20):
+    for tz, tw in ((0, 0), (1 << 20, 4), (1 << 20, 2)):
         t = time.perf_counter()
-        out[tz] = ctx.integrate_batch_mixed(w.x0[sel], *args, opts=hip.batch_opts(team_zones=tz))
-        print(f"team_zones {tz}: {time.perf_counter() - t:.2f} s", flush=True)
+        out[tw] = ctx.integrate_batch_mixed(w.x0[sel], *args,
+                                            opts=hip.batch_opts(team_zones=tz, team_waves=tw))
+        print(f"team_zones {tz} team_waves {tw}: {time.perf_counter() - t:.2f} s", flush=True)
     ie1, y1, ye1, st1 = out[0]
-    ie4, y4, ye4, st4 = out[1 << 20]
     np.testing.assert_array_equal(ie1, 0)
-    np.testing.assert_array_equal(ie4, ie1)
-    for f in ("n_factor", "n_factor_fp64", "n_solve", "n_refine"):
-        print(f, st1[f], st4[f])
-    for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers"):
-        np.testing.assert_array_equal(st4[f], st1[f], err_msg=f)
-    np.testing.assert_array_equal(y4, y1)
-    np.testing.assert_array_equal(ye4, ye1)
+    for tw in (4, 2):
+        ie4, y4, ye4, st4 = out[tw]
+        np.testing.assert_array_equal(ie4, ie1)
+        for f in ("n_factor", "n_factor_fp64", "n_solve", "n_refine"):
+            print(tw, f, st1[f], st4[f])
+        for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers"):
+            np.testing.assert_array_equal(st4[f], st1[f], err_msg=f)
+        np.testing.assert_array_equal(y4, y1)
+        np.testing.assert_array_equal(ye4, ye1)
 
 
 def _exact_oracle(oracle, net):
