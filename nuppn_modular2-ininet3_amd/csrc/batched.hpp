@@ -36,9 +36,10 @@ struct BatchOpts {
 
 // per-slot scratch of the block Newton kernels (one slot per resident wavefront)
 // (B_JAC .. B_SORT: the Bader-Deuflhard step's J, tableau and work vectors)
-// (B_LO: the low parts of the extra-precise residual; B_XLO: those of the solution)
+// (B_LO: the low parts of the extra-precise residual; B_XLO: those of the solution; B_APHYS:
+// the assembled matrix in the residual stream's physical order)
 enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_JAC, B_YK, B_YEXT, B_YSCAL,
-                B_DELTA, B_QCOL, B_YERR, B_SORT, B_LO, B_XLO, B_COUNT };
+                B_DELTA, B_QCOL, B_YERR, B_SORT, B_LO, B_XLO, B_APHYS, B_COUNT };
 
 // the factors, in the same slot after the arrays above
 enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
@@ -50,14 +51,15 @@ struct BLayout {
 };
 
 inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
-                            int nbd) {
+                            int nbd, int nres_phys) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
     size_t pow2 = 1;
     while (pow2 < (size_t) nsp) pow2 <<= 1;
     const size_t len[B_COUNT] = {8ull * nsp, 8ull * nre, 16ull * nre, 8ull * nnzA, 8ull * n,
                                  8ull * n, 8ull * n, 8ull * n, 8ull * nnzA,
                                  8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
-                                 64ull * nsp, 8ull * pow2, 8ull * n, 8ull * n};
+                                 64ull * nsp, 8ull * pow2, 8ull * n, 8ull * n,
+                                 8ull * nres_phys};
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                   4ull * nbd + 4};
@@ -113,8 +115,8 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
     return fail;
 }
 
-// Refined solve of A x = rhs with the precision-T factors (fac: the zone's slot).  y: current
-// abundances (species order).  The solution is kept as x + xlo, an unevaluated sum carried in
+// Refined solve of A x = rhs with the precision-T factors (fac: the zone's slot; A is read there
+// in the residual stream's order, B_APHYS).  y: current abundances (species order).  The solution is kept as x + xlo, an unevaluated sum carried in
 // twice the FP64 precision, so that y + x is accurate relative to itself even for a species
 // that the step burns down by many orders of magnitude (y + x << |x|).  The corrections stop
 // when every |d_i| <= refine_tol (|y_i + x_i| + kRefineFloor): each new abundance has converged
@@ -122,7 +124,7 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
 // deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64 factors: the
 // reference's singular-matrix outcome).
 template <class T, int NB, class TM = Team<1>>
-__device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, const double *Aval,
+__device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b,
                             const double *rhs, const double *y, double *x, double *xlo,
                             double *res, double *d, char *fac, NUPPN_LDS char *lds,
                             ZoneStats &st) {
@@ -166,7 +168,8 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b, co
         // the next residual, extra-precise (d, the correction just applied, serves as its
         // scratch); with accept_tol > 0, x is also accepted once it is backward stable: it
         // solves a system within accept_tol, entry by entry, of A and rhs
-        const double omega = block_residual<TM>(B, Aval, rhs, x, xlo, res, d,
+        const double omega = block_residual<TM>(B, (const double *) (fac + b.off[B_APHYS]),
+                                                rhs, x, xlo, res, d,
                                                 (double *) (fac + b.off[B_LO]), L.red);
         st.n_refine++;
         if (b.O.accept_tol > 0.0 && omega <= b.O.accept_tol) return 0;
@@ -189,7 +192,7 @@ __device__ __forceinline__ int block_refined_solve(const DevNet &N, const BKArgs
                                    double *res, double *d, char *fac, NUPPN_LDS char *lds,
                                    ZoneStats &st) {
     if (block_factor_zone<T, NB, TM>(b, Aval, fac, lds, st)) return sizeof(T) == 8 ? -4 : 1;
-    return block_refine<T, NB, TM>(N, b, Aval, rhs, y, x, xlo, res, d, fac, lds, st);
+    return block_refine<T, NB, TM>(N, b, rhs, y, x, xlo, res, d, fac, lds, st);
 }
 
 // dX/dt of the reduced species from the per-reaction fluxes vx, each species' sum carried in
@@ -255,7 +258,7 @@ template <class TM>
 __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
-                                      double *rhs, double *lop, double *jlo,
+                                      double *rhs, double *lop, double *jlo, double *aphys,
                                       NUPPN_LDS double *red) {
     const int t0 = TM::tid();
     molar(N, y_cur, xn, t0, TM::NT);
@@ -267,7 +270,13 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B
     NUPPN_GLB double *Ag = gptr(Aval);
     block_jacobian_dd<TM>(B, stem, Aval, jlo);
     int stop = 0;
-    for (int k = t0; k < B.nnzA; k += TM::NT) Ag[k] = reduced_value(Ag[k], dt, isdiag[k], stop);
+    const NUPPN_GLB int *rpos = gptr(B.resid_pos);
+    NUPPN_GLB double *Ap = gptr(aphys);
+    for (int k = t0; k < B.nnzA; k += TM::NT) {
+        const double val = reduced_value(Ag[k], dt, isdiag[k], stop);
+        Ag[k] = val;
+        Ap[rpos[k]] = val;
+    }
     stop = team_any<TM>(stop != 0, red) ? 1 : 0;
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur), *lo = gptr((const double *) lop);
@@ -319,7 +328,8 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     const int t0 = TM::tid();
     const int stop = block_assemble<TM>(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
                                         (double *) (slot + b.off[B_LO]),
-                                        (double *) (slot + b.off[B_JAC]), red);
+                                        (double *) (slot + b.off[B_JAC]),
+                                        (double *) (slot + b.off[B_APHYS]), red);
     int rc = stop ? 2 : 1;
     if (!stop && s.pad0 == 0) {
         rc = block_refined_solve<float, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
@@ -397,9 +407,13 @@ struct BlockBDSolver {
         const DevBlock &B = b.B;
         const double *jac = arr(B_JAC);
         double *Aval = arr(B_AVAL);
+        double *Ap = arr(B_APHYS);
         int stop = 0;
-        for (int k = lane_id(); k < B.nnzA; k += WAVE)
-            Aval[k] = reduced_value(jac[k], h, B.csr_isdiag[k], stop);
+        for (int k = lane_id(); k < B.nnzA; k += WAVE) {
+            const double val = reduced_value(jac[k], h, B.csr_isdiag[k], stop);
+            Aval[k] = val;
+            Ap[B.resid_pos[k]] = val;
+        }
         wave_sync();
         if (ballot(stop)) return E_BAD_STEP;
         if (s.pad0 == 0 && block_factor_zone<float, NB>(b, Aval, slot, lds, s.st)) s.pad0 = 1;
@@ -422,7 +436,7 @@ struct BlockBDSolver {
         const double *Aval = arr(B_AVAL);
         int rc = 1;
         if (s.pad0 == 0) {
-            rc = block_refine<float, NB>(N, b, Aval, rhs, ycur, x, arr(B_XLO), arr(B_RES),
+            rc = block_refine<float, NB>(N, b, rhs, ycur, x, arr(B_XLO), arr(B_RES),
                                          arr(B_D), slot, lds, s.st);
             if (rc == 1) {
                 s.pad0 = 1;
@@ -432,7 +446,7 @@ struct BlockBDSolver {
         // a non-finite FP64 solution is returned as it is: the step then ends in GOT_NAN, as
         // the reference's does when its solve overflows
         if (rc == 1)
-            (void) block_refine<double, NB>(N, b, Aval, rhs, ycur, x, arr(B_XLO), arr(B_RES),
+            (void) block_refine<double, NB>(N, b, rhs, ycur, x, arr(B_XLO), arr(B_RES),
                                             arr(B_D), slot, lds, s.st);
         else if (rc != 0)
             return rc;

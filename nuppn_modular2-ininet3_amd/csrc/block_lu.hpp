@@ -26,14 +26,14 @@
 
 namespace nuppn {
 
-// chunked segmented sum (see SegChunks in symbolic.hpp)
-// a SegStream on the device (the host builder's comment has the layout)
+// a SegStream on the device (the host builder's comment has the layout), physical order
 struct DevSeg {
     int nseg;
     const int *e_idx;       // operand index, bit 31 on a segment's last term
     const double *e_coef;
     const double *e_coef_lo;   // J: low parts of the coefficients (null for the other streams)
-    const int *lane_t0;     // [64 kSegRounds + 1]
+    const int *round_base;  // [kSegRounds + 1]
+    const int *lane_len;    // [64 kSegRounds]
     const int *lane_s0;     // [64 kSegRounds]
 };
 
@@ -47,6 +47,7 @@ struct DevBlock {
     const double *ent_ys;
     const int *csr_col;    // [nnzA] column of each A entry (A kept in CSR order)
     const unsigned char *csr_isdiag;
+    const int *resid_pos;  // [nnzA] physical position of each A entry in the residual stream
     DevSeg jac, dxdt, resid;
 };
 
@@ -1002,37 +1003,40 @@ struct Sum2 {
     __device__ Sum2 lane(int k) const { return {readlane_d(s, k), readlane_d(a, k)}; }
 };
 
-// Segmented sums of val(e, operand) over the terms of round r of S by one wavefront.  Lane l
-// sums its range of terms in order, sixteen independent loads at a time, so that a lane keeps
-// many gathers in flight instead of one short segment after another.  A segment that ends inside
-// a lane's range and did not start before it is finished there; the pieces of the others (the
-// first segment of every lane, and the one still open at its end) are added in lane order
-// afterwards.  Every segment is summed in term order, its pieces in lane order, so the result
-// does not depend on timing.  emit(s, total) is called once per segment, from any lane.  A round
-// starts and ends at segment boundaries, so the rounds are independent.
+// Segmented sums of val(P, operand) (P: a term's physical position) over the terms of round r
+// of S by one wavefront.  Lane l sums its range of terms in order, sixteen independent loads at
+// a time, so that a lane keeps many gathers in flight instead of one short segment after
+// another; the lanes' j-th terms are adjacent in memory, so each of those loads is one
+// contiguous access.  A segment that ends inside a lane's range and did not start before it is
+// finished there; the pieces of the others (the first segment of every lane, and the one still
+// open at its end) are added in lane order afterwards.  Every segment is summed in term order,
+// its pieces in lane order, so the result does not depend on timing.  emit(s, total) is called
+// once per segment, from any lane.  A round starts and ends at segment boundaries, so the
+// rounds are independent.
 template <class A, class V, class E>
 __device__ __forceinline__ void seg_round(const DevSeg &S, int r, V val, E emit) {
     constexpr int BATCH = 16;
     const int l = lane_id();
     const NUPPN_GLB int *eidx = gptr(S.e_idx);
-    const int t0 = gptr(S.lane_t0)[64 * r + l], t1 = gptr(S.lane_t0)[64 * r + l + 1];
+    const int len = gptr(S.lane_len)[64 * r + l];
+    const int base = gptr(S.round_base)[r] + l;
     const int s0 = gptr(S.lane_s0)[64 * r + l];
     int s = s0;
     A acc{}, first{};
     bool has_first = false, open = false;
-    for (int t = t0; t < t1; t += BATCH) {
+    for (int j0 = 0; j0 < len; j0 += BATCH) {
         A v[BATCH];
         bool end[BATCH];
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
-            const int e = min(t + j, t1 - 1);   // past the range: a repeat, not added below
-            const int raw = eidx[e];
+            const int P = base + WAVE * min(j0 + j, len - 1);   // past the range: a repeat
+            const int raw = eidx[P];
             end[j] = raw < 0;
-            v[j] = val(e, raw & 0x7fffffff);
+            v[j] = val(P, raw & 0x7fffffff);
         }
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
-            if (t + j < t1) {
+            if (j0 + j < len) {
                 acc.add(v[j]);
                 open = !end[j];
                 if (end[j]) {
@@ -1120,31 +1124,29 @@ struct SumDD2 {
     __device__ SumDD2 lane(int k) const { return {readlane_d(hi, k), readlane_d(lo, k)}; }
 };
 
-// r = rhs - A (x + xlo) (A in CSR order; x + xlo the solution carried in twice the FP64
-// precision) with the products and sums carried in twice the FP64
-// precision, rounded once: the residual of the FP64 system to FP64 accuracy even where the
+// r = rhs - A (x + xlo) (A's entries in the residual stream's physical order, Aphys; x + xlo
+// the solution carried in twice the FP64 precision) with the products and sums carried in
+// twice the FP64 precision, rounded once: the residual of the FP64 system to FP64 accuracy even where the
 // terms cancel by many orders of magnitude, so iterative refinement converges to the solution
 // of the FP64 system itself (its forward error, not only its backward error, reaches the FP64
 // level: Demmel, Hida, Kahan, Li, Mukherjee and Riedy, "Error bounds from extra-precise
 // iterative refinement", ACM TOMS 32 (2006)).  Returns the componentwise backward error
 // max_i |r_i| / (|A| |x| + |rhs| + kAbundanceFloor)_i (Oettli-Prager).  lo, absum: [n] scratch.
 template <class TM>
-__device__ __forceinline__ double block_residual(const DevBlock &B, const double *Acsrp,
+__device__ __forceinline__ double block_residual(const DevBlock &B, const double *Aphysp,
                                                  const double *rhsp, const double *xp,
                                                  const double *xlop, double *rp,
                                                  double *absump, double *lop,
                                                  NUPPN_LDS double *red) {
-    const NUPPN_GLB double *Acsr = gptr(Acsrp), *rhs = gptr(rhsp), *x = gptr(xp);
+    const NUPPN_GLB double *Aphys = gptr(Aphysp), *rhs = gptr(rhsp), *x = gptr(xp);
     const NUPPN_GLB double *xlo = gptr(xlop);
-    const NUPPN_GLB int *csr_col = gptr(B.csr_col);
     NUPPN_GLB double *r = gptr(rp), *absum = gptr(absump), *lo = gptr(lop);
-    // the residual's terms are the CSR entries in order: term e is entry e
+    // the residual's term at physical position P: A's entry stored there (Aphys) times x of the
+    // entry's column (the operand)
     seg_stream<SumDD, TM>(
         B.resid,
-        [&](int e, int k) {
-            (void) k;
-            const int c = csr_col[e];
-            const double av = Acsr[e], xv = x[c];
+        [&](int P, int c) {
+            const double av = Aphys[P], xv = x[c];
             const double t = av * xv;
             return SumDD{t, fma(av, xlo[c], fma(av, xv, -t)), fabs(t)};
         },

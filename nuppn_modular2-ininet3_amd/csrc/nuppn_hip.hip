@@ -257,7 +257,8 @@ struct nuppn_ctx {
     int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr, *d_b_ent_dst = nullptr;
     unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
-    int *d_seg_i[9] = {};         // jac, dxdt, resid: e_idx, lane_t0, lane_s0
+    int *d_seg_i[12] = {};        // jac, dxdt, resid: e_idx, lane_len, lane_s0, round_base
+    int *d_b_resid_pos = nullptr; // A entry -> physical position in the residual stream
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     double *d_jac_lo = nullptr;   // jac: e_coef_lo
     double *d_rcache = nullptr, *d_rc_t9 = nullptr;   // REACLIB T9 cache, grid
@@ -296,7 +297,8 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_diagk, (void *) d_b_ent_ptr, (void *) d_b_ent_k,
                     (void *) d_b_ent_loc, (void *) d_b_ent_col, (void *) d_b_csr_col,
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
-                    (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9})
+                    (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9,
+                    (void *) d_b_resid_pos})
         if (p) (void) hipFree(p);
     for (void *p : d_seg_i)
         if (p) (void) hipFree(p);
@@ -374,13 +376,15 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     B.ent_ys = c->d_b_ent_ys;
     B.csr_col = c->d_b_csr_col;
     B.csr_isdiag = c->d_b_isdiag;
+    B.resid_pos = c->d_b_resid_pos;
     const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     DevSeg *ds[3] = {&B.jac, &B.dxdt, &B.resid};
     for (int i = 0; i < 3; ++i) {
         ds[i]->nseg = sc[i]->nseg;
-        ds[i]->e_idx = c->d_seg_i[3 * i];
-        ds[i]->lane_t0 = c->d_seg_i[3 * i + 1];
-        ds[i]->lane_s0 = c->d_seg_i[3 * i + 2];
+        ds[i]->e_idx = c->d_seg_i[4 * i];
+        ds[i]->lane_len = c->d_seg_i[4 * i + 1];
+        ds[i]->lane_s0 = c->d_seg_i[4 * i + 2];
+        ds[i]->round_base = c->d_seg_i[4 * i + 3];
         ds[i]->e_coef = c->d_seg_c[i];
         ds[i]->e_coef_lo = nullptr;
     }
@@ -457,15 +461,17 @@ static int setup_batched(nuppn_ctx *c) {
         return NUPPN_ERR_HIP;
     const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     for (int i = 0; i < 3; ++i) {
-        if (dev_upload(&c->d_seg_i[3 * i], sc[i]->e_idx) ||
-            dev_upload(&c->d_seg_i[3 * i + 1], sc[i]->lane_t0) ||
-            dev_upload(&c->d_seg_i[3 * i + 2], sc[i]->lane_s0) ||
+        if (dev_upload(&c->d_seg_i[4 * i], sc[i]->e_idx) ||
+            dev_upload(&c->d_seg_i[4 * i + 1], sc[i]->lane_len) ||
+            dev_upload(&c->d_seg_i[4 * i + 2], sc[i]->lane_s0) ||
+            dev_upload(&c->d_seg_i[4 * i + 3], sc[i]->round_base) ||
             dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
             return NUPPN_ERR_HIP;
     }
-    if (dev_upload(&c->d_jac_lo, b.jac.e_coef_lo)) return NUPPN_ERR_HIP;
+    if (dev_upload(&c->d_jac_lo, b.jac.e_coef_lo) || dev_upload(&c->d_b_resid_pos, b.resid_pos))
+        return NUPPN_ERR_HIP;
     c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
-                              b.nbd);
+                              b.nbd, b.resid.round_base[kSegRounds]);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
     c->blds32 = rnd(nb == 64 ? block_lds_bytes<float, 64>(b.nip, b.nbd)
                              : block_lds_bytes<float, 32>(b.nip, b.nbd));

@@ -386,27 +386,31 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
 
 namespace {
 
-// segments given as per-segment element lists -> the streamed layout (SegStream)
-void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, SegStream &out) {
+// segments given as per-segment element lists (and, for J, the coefficients' low parts) -> the
+// streamed layout (SegStream)
+void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, SegStream &out,
+                  const std::vector<std::vector<double>> *seg_lo = nullptr) {
     out.nseg = (int) segs.size();
-    out.e_idx.clear();
-    out.e_coef.clear();
-    std::vector<int> seg_of;   // segment of every term
+    std::vector<int> idx;       // logical order
+    std::vector<double> cf, lo;
+    std::vector<int> seg_of;    // segment of every term
     for (int sg = 0; sg < out.nseg; ++sg) {
         const auto &seg = segs[sg];
         if (seg.empty()) {
-            out.e_idx.push_back(0);
-            out.e_coef.push_back(0.0);
+            idx.push_back(0);
+            cf.push_back(0.0);
+            lo.push_back(0.0);
             seg_of.push_back(sg);
         }
-        for (const auto &t : seg) {
-            out.e_idx.push_back(t.first);
-            out.e_coef.push_back(t.second);
+        for (size_t t = 0; t < seg.size(); ++t) {
+            idx.push_back(seg[t].first);
+            cf.push_back(seg[t].second);
+            lo.push_back(seg_lo ? (*seg_lo)[sg][t] : 0.0);
             seg_of.push_back(sg);
         }
-        out.e_idx.back() |= (int) (1u << 31);
+        idx.back() |= (int) (1u << 31);
     }
-    const long long nterm = (long long) out.e_idx.size();
+    const long long nterm = (long long) idx.size();
     // round boundaries: the first segment start at or after an even share of the terms
     std::vector<long long> rb(kSegRounds + 1, nterm);
     rb[0] = 0;
@@ -416,14 +420,37 @@ void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, 
         rb[r] = t;
     }
     const int NL = 64 * kSegRounds;
-    out.lane_t0.assign(NL + 1, (int) nterm);
+    out.lane_len.assign(NL, 0);
     out.lane_s0.assign(NL, out.nseg);
+    out.round_base.assign(kSegRounds + 1, 0);
+    out.phys.assign(nterm, 0);
+    std::vector<long long> t0(NL + 1);
     for (int r = 0; r < kSegRounds; ++r) {
         const long long len = rb[r + 1] - rb[r];
-        for (int l = 0; l < 64; ++l) out.lane_t0[64 * r + l] = (int) (rb[r] + (l * len) / 64);
+        int maxlen = 0;
+        for (int l = 0; l < 64; ++l) {
+            const long long a = rb[r] + (l * len) / 64, e = rb[r] + ((l + 1) * len) / 64;
+            t0[64 * r + l] = a;
+            out.lane_len[64 * r + l] = (int) (e - a);
+            out.lane_s0[64 * r + l] = a < nterm ? seg_of[a] : out.nseg;
+            maxlen = std::max(maxlen, (int) (e - a));
+        }
+        out.round_base[r + 1] = out.round_base[r] + 64 * maxlen;
     }
-    for (int l = 0; l < NL; ++l)
-        out.lane_s0[l] = out.lane_t0[l] < nterm ? seg_of[out.lane_t0[l]] : out.nseg;
+    const int nphys = out.round_base[kSegRounds];
+    out.e_idx.assign(nphys, 0);
+    out.e_coef.assign(nphys, 0.0);
+    out.e_coef_lo.assign(seg_lo ? nphys : 0, 0.0);
+    for (int r = 0; r < kSegRounds; ++r)
+        for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < out.lane_len[64 * r + l]; ++j) {
+                const long long t = t0[64 * r + l] + j;
+                const int P = out.round_base[r] + 64 * j + l;
+                out.phys[t] = P;
+                out.e_idx[P] = idx[t];
+                out.e_coef[P] = cf[t];
+                if (seg_lo) out.e_coef_lo[P] = lo[t];
+            }
 }
 
 }  // namespace
@@ -472,13 +499,7 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
             seg_lo[k].push_back(std::fma(-hi, ac, num) / ac);
         }
     }
-    build_stream(segs, b.jac);
-    b.jac.e_coef_lo.clear();   // in build_stream's term order (an empty segment: one zero term)
-    for (int k = 0; k < nnz; ++k) {
-        if (seg_lo[k].empty()) b.jac.e_coef_lo.push_back(0.0);
-        for (double lo : seg_lo[k]) b.jac.e_coef_lo.push_back(lo);
-    }
-    if (b.jac.e_coef_lo.size() != b.jac.e_coef.size()) return -1;
+    build_stream(segs, b.jac, &seg_lo);
     // dX/dt (calculate_dxdt, jac_rhs.F90:250-272): roles k1 -k2, k3 -1, k5 +k6, k7 +k8, times A
     std::vector<std::vector<std::pair<int, double>>> dsegs(n);
     for (int r = 0; r < n; ++r) {
@@ -494,15 +515,15 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
         }
     }
     build_stream(dsegs, b.dxdt);
-    // residual rows: element = CSR position k, operand = column
+    // residual rows: the CSR entries of each row, operand = column
     std::vector<std::vector<std::pair<int, double>>> rsegs(n);
     for (int i = 0; i < n; ++i)
-        for (int k = b.csr_ptr[i]; k < b.csr_ptr[i + 1]; ++k) rsegs[i].push_back({k, 1.0});
+        for (int k = b.csr_ptr[i]; k < b.csr_ptr[i + 1]; ++k) rsegs[i].push_back({b.csr_col[k], 1.0});
     build_stream(rsegs, b.resid);
-    // every row holds its diagonal, so no dummy terms: the residual kernel reads term e as CSR
-    // entry e
-    for (size_t e = 0; e < b.resid.e_idx.size(); ++e)
-        if ((b.resid.e_idx[e] & 0x7fffffff) != (int) e) return -1;
+    // every row holds its diagonal, so no dummy terms: logical term k is CSR entry k, and A's
+    // entry k is also stored at the term's physical position for the residual
+    if ((int) b.resid.phys.size() != nnz) return -1;
+    b.resid_pos = b.resid.phys;
     return 0;
 }
 

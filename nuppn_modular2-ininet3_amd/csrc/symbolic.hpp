@@ -32,23 +32,30 @@ struct Symbolic {
 // builds everything but the column order; returns 0 or a negative error
 int build_symbolic(const HostNetwork &net, Symbolic &s);
 
-// Segmented sums as the device streams them: the terms of all segments back to back, in segment
-// order and, inside a segment, in summation order; the last term of every segment carries bit 31
-// in its operand index, and an empty segment gets one zero term.  The terms are cut into
-// kSegRounds rounds at segment boundaries (near-even shares), and every round into 64 lane
-// ranges: lane l of round r sums terms lane_t0[64 r + l] .. lane_t0[64 r + l + 1] - 1, and
-// lane_s0[64 r + l] is the segment holding its first term.  A round is the work of one wavefront
-// pass; a zone's single wavefront takes the rounds one after another, the kSegRounds wavefronts of
-// a multi-wavefront zone one each.  Both sum every segment in the same pieces and order, so the
-// two give the same bits.
+// Segmented sums as the device streams them.  In logical order the terms of all segments lie
+// back to back, in segment order and, inside a segment, in summation order; the last term of
+// every segment carries bit 31 in its operand index, and an empty segment gets one zero term.
+// The logical terms are cut into kSegRounds rounds at segment boundaries (near-even shares), and
+// every round into 64 lane ranges of near-even length: lane l of round r sums lane_len[64 r + l]
+// consecutive logical terms, starting in segment lane_s0[64 r + l].  A round is the work of one
+// wavefront pass; a zone's single wavefront takes the rounds one after another, the wavefronts
+// of a multi-wavefront zone share them.  All sum every segment in the same pieces and order, so
+// they give the same bits.
+// Storage is interleaved by lane: the j-th term of lane l of round r sits at physical position
+// round_base[r] + 64 j + l, so the 64 lanes' j-th terms are adjacent and every load of the
+// stream's operand indices and coefficients is one contiguous access (padding positions hold
+// operand 0, coefficient 0 and are never summed).  phys[t] is the physical position of logical
+// term t (host side only).
 constexpr int kSegRounds = 4;
 struct SegStream {
     int nseg = 0;
-    std::vector<int> e_idx;       // operand index, | (1 << 31) on a segment's last term
-    std::vector<double> e_coef;   // coefficient
+    std::vector<int> e_idx;         // operand index, | (1 << 31) on a segment's last term
+    std::vector<double> e_coef;     // coefficient
     std::vector<double> e_coef_lo;  // J only: the coefficient's low part (hi + lo its exact value)
-    std::vector<int> lane_t0;     // [64 kSegRounds + 1]
-    std::vector<int> lane_s0;     // [64 kSegRounds]
+    std::vector<int> round_base;    // [kSegRounds + 1]
+    std::vector<int> lane_len;      // [64 kSegRounds]
+    std::vector<int> lane_s0;       // [64 kSegRounds]
+    std::vector<int> phys;          // [logical terms]
 };
 
 // Static block structure of the reduced matrix for the pivot-free batched factorisation.
@@ -88,8 +95,10 @@ struct BlockSym {
     std::vector<unsigned char> csr_isdiag;
     // chunked sums: J entries (CSR order) from the per-reaction stems [st1 | st2] (2 nre),
     // dX/dt of reduced species from the fluxes vx (nre, coefficients include the mass number),
-    // the residual rows from the CSR entries (element = CSR position, operand = column)
+    // the residual rows from the CSR entries (operand = column; the coefficient of the term at
+    // physical position P is A's entry stored there, see resid_pos)
     SegStream jac, dxdt, resid;
+    std::vector<int> resid_pos;   // [nnzA] physical position of each CSR entry in the residual stream
 };
 
 // Builds the block structure for block size NB (32 or 64); returns 0, or -1 when the interior
