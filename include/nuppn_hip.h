@@ -179,6 +179,11 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
  * twice the device's compute-unit count; 0: never.
  * team_waves: the wavefronts of such a workgroup, 2 or 4; 0 (default): 4 when the list has no
  * more zones than the device has compute units, else 2.
+ * queue_passes: controller/worker passes the host queues before it reads the work-list counts
+ * back (1..16, default 16; the counts stay on the device, so queued passes need no host round
+ * trip).  While the list holds more zones than team_zones the host still reads back every pass.
+ * use_graph: 1 (default): a chunk of 16 passes with a short list is captured once as a hipGraph
+ * and replayed while the list stays within the grids it was captured with; 0: plain launches.
  * Every choice gives the same results bit for bit. */
 typedef struct {
     int factor_precision;   /* 32 (default) or 64 */
@@ -187,6 +192,8 @@ typedef struct {
     double accept_tol;      /* 0 (off) */
     int team_zones;         /* -1 (default), 0 or a zone count */
     int team_waves;         /* 0 (default), 2 or 4 */
+    int queue_passes;       /* 16 (default): 1..16 */
+    int use_graph;          /* 1 (default) or 0 */
 } nuppn_batch_opts;
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o);
@@ -213,7 +220,8 @@ int nuppn_integrate_batch_mixed_dev(nuppn_ctx *ctx, int nz, double *yps, const d
 
 /* Timing of the last batch call from HIP events on its stream: out[0] rate kernel ms,
  * out[1] whole integration ms, out[2] summed Newton-kernel ms, out[3] summed control-kernel
- * ms, out[4] Newton-kernel launches, out[5] controller passes.  nout: entries wanted (<= 6). */
+ * ms, out[4] Newton-kernel launches, out[5] controller passes, out[6] launches of captured
+ * pass graphs.  nout: entries wanted (<= 7). */
 int nuppn_last_timing(nuppn_ctx *ctx, double *out, int nout);
 
 /* Link-level drop-ins with the reference signatures and semantics.

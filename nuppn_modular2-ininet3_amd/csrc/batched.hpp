@@ -486,7 +486,8 @@ __global__ void __launch_bounds__(64, W) zone_newton_block_kernel(KArgs a, BKArg
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x)
         zone_newton_block<NB, Team<1>>(a, b, slot, lds, a.Z.active[i]);
 }
 
@@ -501,7 +502,8 @@ __global__ void __launch_bounds__(64 * NW, 1) zone_newton_team_kernel(KArgs a, B
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x)
         zone_newton_block<NB, Team<NW>>(a, b, slot, lds, a.Z.active[i]);
 }
 
@@ -512,7 +514,8 @@ __global__ void __launch_bounds__(64) zone_bd_block_kernel(KArgs a, BKArgs b) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
-    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x)
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x)
         zone_bd_block<NB>(a, b, slot, lds, a.Z.active[i]);
 }
 

@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) {
 // zones not on it are done)
 __global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    for (int i = blockIdx.x; i < a.Z.nctl; i += gridDim.x)
+    const int nctl = list_len(a.Z.nctl_dev, a.Z.nctl);
+    for (int i = blockIdx.x; i < nctl; i += gridDim.x)
         zone_control(a, a.Z.ctl ? a.Z.ctl[i] : i, (NUPPN_LDS double *) smem);
 }
 
@@ -116,7 +117,8 @@ __global__ void __launch_bounds__(64) zone_newton_kernel(KArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const LUWork w = make_work(smem, a.N.n);
     const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
-    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x) zone_newton(a, S, w, a.Z.active[i]);
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x) zone_newton(a, S, w, a.Z.active[i]);
 }
 
 // one Bader-Deuflhard step for every zone in PH_NEWTON (integration_method 1)
@@ -124,7 +126,8 @@ __global__ void __launch_bounds__(64) zone_bd_kernel(KArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const LUWork w = make_work(smem, a.N.n);
     const Slot S = make_slot(a.slot_base + (size_t) blockIdx.x * a.slot_bytes, a.off);
-    for (int i = blockIdx.x; i < a.Z.nact; i += gridDim.x) zone_bd(a, S, w, a.Z.active[i]);
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x) zone_bd(a, S, w, a.Z.active[i]);
 }
 
 __global__ void flag_reset_kernel(int *flag) {
@@ -212,6 +215,28 @@ void dev_free(void *p) {
     if (p) (void) hipFree(p);
 }
 
+// passes queued ahead of the host: each has its own flag pair on the device and event set
+constexpr int kPassRing = 16;
+
+// a chunk of kPassRing passes built as a graph (and its executable), replayed while the work list
+// stays within the bound its grids were sized for; freed at the end of a batch call
+struct PassGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int bound = -1;
+    // the kernel arguments of the graph's passes, kept for the graph's lifetime
+    KArgs ka[kPassRing];
+    int *flag[kPassRing];
+    void reset() {
+        if (exec) (void) hipGraphExecDestroy(exec);
+        if (graph) (void) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        bound = -1;
+    }
+    ~PassGraph() { reset(); }
+};
+
 }  // namespace
 
 struct nuppn_ctx {
@@ -273,9 +298,13 @@ struct nuppn_ctx {
     int bgrid[3] = {0, 0, 0};   // resident wavefronts of the worker held to 1 / 2 per SIMD
     char *d_bslots = nullptr;
     hipEvent_t ev[4] = {};
-    hipEvent_t evn[2] = {}, evc[2] = {};
+    // per pass of a chunk (pass % kPassRing): control start / end, worker end
+    hipEvent_t evc[kPassRing][2] = {}, evn[kPassRing][1] = {};
+    // the same for captured pass graphs (events a graph records are used by graphs only)
+    hipEvent_t gevc[kPassRing][2] = {}, gevn[kPassRing][1] = {};
     double newton_ms = 0.0, control_ms = 0.0;
-    long long newton_launches = 0;
+    long long newton_launches = 0, graph_launches = 0;
+    hipGraphNode_t glast = nullptr;     // the last node of a pass graph being built
     long long last_passes = 0;
     std::mutex mu;
     ~nuppn_ctx();
@@ -306,10 +335,18 @@ nuppn_ctx::~nuppn_ctx() {
         if (p) (void) hipFree(p);
     for (auto &e : ev)
         if (e) (void) hipEventDestroy(e);
-    for (auto &e : evn)
-        if (e) (void) hipEventDestroy(e);
-    for (auto &e : evc)
-        if (e) (void) hipEventDestroy(e);
+    for (auto &r : evn)
+        for (auto &e : r)
+            if (e) (void) hipEventDestroy(e);
+    for (auto &r : evc)
+        for (auto &e : r)
+            if (e) (void) hipEventDestroy(e);
+    for (auto &r : gevn)
+        for (auto &e : r)
+            if (e) (void) hipEventDestroy(e);
+    for (auto &r : gevc)
+        for (auto &e : r)
+            if (e) (void) hipEventDestroy(e);
     if (h_flag) (void) hipHostFree(h_flag);
 }
 
@@ -720,7 +757,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         dev_upload(&c->d_isdiag, s.A_isdiag) ||
         dev_upload(&c->d_perm_c, (const int *) nullptr, (size_t) s.n) ||
         dev_upload(&c->d_iperm_c, (const int *) nullptr, (size_t) s.n) ||
-        dev_upload(&c->d_flag, (const int *) nullptr, 2) ||
+        dev_upload(&c->d_flag, (const int *) nullptr, 2 * kPassRing) ||
         dev_upload(&c->d_status, (const int *) nullptr, 4))
         return nullptr;
     if (upload_order(c.get())) return nullptr;
@@ -774,11 +811,20 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     c->max_slots = std::max(1, std::min(ncu * per_cu, by_mem));
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    for (auto &e : c->evn)
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    for (auto &e : c->evc)
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    if (hipHostMalloc((void **) &c->h_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    for (auto &r : c->evn)
+        for (auto &e : r)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    for (auto &r : c->evc)
+        for (auto &e : r)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    for (auto &r : c->gevn)
+        for (auto &e : r)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    for (auto &r : c->gevc)
+        for (auto &e : r)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipHostMalloc((void **) &c->h_flag, 2 * kPassRing * sizeof(int), hipHostMallocDefault) !=
+        hipSuccess)
         return nullptr;
     if (ensure_slots(c.get(), 1)) return nullptr;
     if (setup_batched(c.get())) return nullptr;
@@ -1010,71 +1056,194 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     HIP_OK(hipEventRecord(c->ev[2], s));
     hipLaunchKernelGGL(zone_init_kernel, dim3(nz), dim3(64), c->sort_lds, s, a);
     HIP_OK(hipGetLastError());
-    // each pass advances every zone by at least one Newton iteration or finishes it
+    // Each pass is a flag reset, a control launch and a worker launch; it advances every zone by
+    // at least one Newton iteration or finishes it, so the work-list counts never grow.  The
+    // counts stay on the device (pass p uses flag pair p % kPassRing): the control kernel reads
+    // the previous pass's count and the worker its own, and the host queues up to kPassRing
+    // passes ahead, sizing their grids by the last count it has read (an upper bound; surplus
+    // workgroups find no list entry).  It reads the counts back after each chunk of passes.
     const long long max_pass = (long long) c->kn.max_sub_steps * (c->kn.ittd + 2) + 16;
-    long long pass = 0;
-    bool newton_pending = false;
+    const bool bd = c->kn.integration_method == 1;
+    const int queue = block ? std::max(1, std::min(bo->queue_passes, kPassRing)) : kPassRing;
+    const int team_max = block ? (bo->team_zones < 0 ? 2 * c->ncu : bo->team_zones) : 0;
     c->newton_ms = 0.0;
     c->control_ms = 0.0;
     c->newton_launches = 0;
-    for (;; ++pass) {
-        // this pass builds work list pass % 2; its control pass takes the previous list
-        a.Z.active = c->d_active + (size_t) (pass & 1) * nz;
-        hipLaunchKernelGGL(flag_reset_kernel, dim3(1), dim3(1), 0, s, c->d_flag);
-        HIP_OK(hipEventRecord(c->evc[0], s));
-        hipLaunchKernelGGL(zone_control_kernel, dim3(std::max(1, a.Z.nctl)), dim3(64), c->sort_lds,
-                           s, a);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipEventRecord(c->evc[1], s));
-        HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        float ms = 0.f;
-        if (newton_pending) {
-            HIP_OK(hipEventElapsedTime(&ms, c->evn[0], c->evn[1]));
-            c->newton_ms += ms;
-            newton_pending = false;
-        }
-        HIP_OK(hipEventElapsedTime(&ms, c->evc[0], c->evc[1]));
-        c->control_ms += ms;
-        if (c->h_flag[0] == 0) break;
-        if (pass >= max_pass) return NUPPN_ERR_HIP;
-        HIP_OK(hipEventRecord(c->evn[0], s));
-        const bool bd = c->kn.integration_method == 1;
-        const int active = std::min(c->h_flag[1], nz);   // the control pass's work list
-        a.Z.nact = active;
-        // the next control pass takes this list
-        a.Z.ctl = a.Z.active;
-        a.Z.nctl = active;
-        if (!block) {
-            void *args[] = {&a};
-            HIP_OK(hipLaunchKernel(bd ? (const void *) zone_bd_kernel
-                                      : (const void *) zone_newton_kernel,
-                                   dim3(std::min(grid_n, active)), dim3(64), args, c->lds, s));
+    c->graph_launches = 0;
+    PassGraph graph;
+    bool graph_off = false;   // set when a capture fails
+    long long pass = 0, done_pass = -1;
+    int bound = nz;   // upper bound of the zones the next control pass takes
+    // one pass: its kernels and timing events (on a capturing stream they become graph nodes)
+    // (st: the stream launched on; capturing: st records into a graph, where launch errors
+    // surface at the end of the capture)
+    const char *failed_at = "";
+    // one pass: its kernels and timing events, launched on stream s, or (g set) appended to
+    // graph g as a chain of kernel and event-record nodes (built node by node: HIP runtimes that
+    // cannot capture event records into a graph still take event-record nodes)
+    auto enqueue_pass = [&](long long p, int bnd, hipGraph_t g) -> hipError_t {
+        hipEvent_t(&evc)[kPassRing][2] = g ? c->gevc : c->evc;
+        hipEvent_t(&evn)[kPassRing][1] = g ? c->gevn : c->evn;
+        hipGraphNode_t last = g ? c->glast : nullptr;   // chain after the chunk's previous pass
+        auto kernel = [&](const void *f, dim3 grid, dim3 blk, void **args, size_t lds) {
+            if (!g) return hipLaunchKernel(f, grid, blk, args, lds, s);
+            hipKernelNodeParams kp;
+            memset(&kp, 0, sizeof(kp));
+            kp.func = const_cast<void *>(f);
+            kp.gridDim = grid;
+            kp.blockDim = blk;
+            kp.sharedMemBytes = (unsigned) lds;
+            kp.kernelParams = args;
+            hipGraphNode_t n = nullptr;
+            const hipError_t e = hipGraphAddKernelNode(&n, g, last ? &last : nullptr,
+                                                       last ? 1 : 0, &kp);
+            if (e == hipSuccess) last = n;
+            return e;
+        };
+        auto event = [&](hipEvent_t ev) {
+            if (!g) return hipEventRecord(ev, s);
+            hipGraphNode_t n = nullptr;
+            const hipError_t e = hipGraphAddEventRecordNode(&n, g, last ? &last : nullptr,
+                                                            last ? 1 : 0, ev);
+            if (e == hipSuccess) last = n;
+            return e;
+        };
+        const int slot = (int) (p % kPassRing);
+        KArgs ka_local;
+        int *flag_local = nullptr;
+        KArgs &ka = g ? graph.ka[slot] : ka_local;
+        int *&flag = g ? graph.flag[slot] : flag_local;
+        flag = c->d_flag + 2 * slot;
+        ka = a;
+        ka.Z.flag = flag;
+        ka.Z.active = c->d_active + (size_t) (p & 1) * nz;
+        if (p == 0) {
+            ka.Z.ctl = nullptr;
+            ka.Z.nctl = nz;
+            ka.Z.nctl_dev = nullptr;
         } else {
-            void *args[] = {&a, &b};
+            ka.Z.ctl = c->d_active + (size_t) ((p - 1) & 1) * nz;
+            ka.Z.nctl = 0;
+            ka.Z.nctl_dev = c->d_flag + 2 * (int) ((p - 1) % kPassRing) + 1;
+        }
+        ka.Z.nact = 0;
+        ka.Z.nact_dev = flag + 1;
+        failed_at = "flag reset";
+        void *fargs[] = {&flag};
+        hipError_t e = kernel((const void *) flag_reset_kernel, dim3(1), dim3(1), fargs, 0);
+        if (e != hipSuccess) return e;
+        failed_at = "control start event";
+        if ((e = event(evc[slot][0])) != hipSuccess) return e;
+        failed_at = "control launch";
+        void *cargs[] = {&ka};
+        e = kernel((const void *) zone_control_kernel, dim3(std::max(1, bnd)), dim3(64), cargs,
+                   c->sort_lds);
+        if (e != hipSuccess) return e;
+        failed_at = "control end event";
+        if ((e = event(evc[slot][1])) != hipSuccess) return e;
+        failed_at = "worker launch";
+        const int nl = std::max(1, std::min(bnd, grid_n));
+        if (!block) {
+            void *args[] = {&ka};
+            e = kernel(bd ? (const void *) zone_bd_kernel : (const void *) zone_newton_kernel,
+                       dim3(nl), dim3(64), args, c->lds);
+        } else {
+            void *args[] = {&ka, &b};
             // few zones: several wavefronts per zone shorten each one's iteration (four when the
             // list fits one zone per CU, else two)
-            const int team_max = bo->team_zones < 0 ? 2 * c->ncu : bo->team_zones;
-            int nw = bo->team_waves ? bo->team_waves : (active <= c->ncu ? 4 : 2);
+            int nw = bo->team_waves ? bo->team_waves : (bnd <= c->ncu ? 4 : 2);
             if (!c->team_ok[nw]) nw = c->team_ok[2] ? 2 : 0;
-            if (!bd && nw && active <= team_max) {
-                HIP_OK(hipLaunchKernel(team_kernel_ptr(c->bnb, nw), dim3(std::min(grid_n, active)),
-                                       dim3(64 * nw), args, c->blds_team, s));
+            if (!bd && nw && bnd <= team_max) {
+                e = kernel(team_kernel_ptr(c->bnb, nw), dim3(nl), dim3(64 * nw), args,
+                           c->blds_team);
             } else {
                 // the register-lean two-per-SIMD variant only when the single-wavefront one
                 // cannot give every zone of the list its own wavefront
-                const int waves = (!bd && active > c->bgrid[1]) ? 2 : 1;
-                const int grid = std::min({grid_n, c->bgrid[waves], active});
-                HIP_OK(hipLaunchKernel(block_kernel_ptr(c->bnb, bd ? 1 : 0, waves), dim3(grid),
-                                       dim3(64), args, c->blds64, s));
+                const int waves = (!bd && bnd > c->bgrid[1]) ? 2 : 1;
+                const int grid = std::max(1, std::min(nl, c->bgrid[waves]));
+                e = kernel(block_kernel_ptr(c->bnb, bd ? 1 : 0, waves), dim3(grid), dim3(64),
+                           args, c->blds64);
             }
         }
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipEventRecord(c->evn[1], s));
-        newton_pending = true;
-        c->newton_launches++;
+        if (e != hipSuccess) return e;
+        failed_at = "worker end event";
+        if ((e = event(evn[slot][0])) != hipSuccess) return e;
+        if (g) c->glast = last;
+        return hipSuccess;
+    };
+    while (done_pass < 0) {
+        if (pass >= max_pass) return NUPPN_ERR_HIP;
+        // chunk length: one pass at a time while the lists are long (long passes; their grids
+        // and kernel variants follow the counts exactly), more as they shorten
+        int ch = 1;
+        if (pass > 0 && (!block || bound <= team_max))
+            ch = (!block || bound <= c->ncu) ? queue : std::min(queue, 4);
+        // graph chunks start at a ring boundary: a shorter chunk first reaches one
+        if (block && bo->use_graph && ch == kPassRing && pass % kPassRing != 0)
+            ch = kPassRing - (int) (pass % kPassRing);
+        // a chunk that starts at a ring boundary with a short list can be replayed from a
+        // prebuilt graph: pass p's arguments depend only on p % kPassRing, except the grids
+        const bool use_graph = block && bo->use_graph && !graph_off && ch == kPassRing &&
+                               pass % kPassRing == 0;
+        if (use_graph) {
+            // grids sized for the next power of two at or above the count (at most the CU
+            // count, which a graph chunk's count never exceeds), so that a built chunk
+            // serves until the count halves
+            int gb = 1;
+            while (gb < bound) gb <<= 1;
+            gb = std::max(bound, std::min(gb, c->ncu));
+            if (!graph.exec || graph.bound != gb) {
+                graph.reset();
+                hipGraph_t g = nullptr;
+                HIP_OK(hipGraphCreate(&g, 0));
+                graph.graph = g;
+                c->glast = nullptr;
+                hipError_t e = hipSuccess;
+                for (int q = 0; q < ch && e == hipSuccess; ++q) e = enqueue_pass(pass + q, gb, g);
+                hipError_t e3 = hipSuccess;
+                if (e == hipSuccess) e3 = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
+                if (e != hipSuccess || e3 != hipSuccess) {
+                    // the rest of the call runs on plain launches
+                    fprintf(stderr, "nuppn_hip: pass graph not built (%s: %s; "
+                            "instantiation: %s); plain launches follow\n", failed_at,
+                            hipGetErrorString(e), hipGetErrorString(e3));
+                    (void) hipGetLastError();
+                    graph.reset();
+                    graph_off = true;
+                }
+                graph.bound = gb;
+            }
+        }
+        if (use_graph && !graph_off) {
+            HIP_OK(hipGraphLaunch(graph.exec, s));
+            c->graph_launches++;
+        } else {
+            for (int q = 0; q < ch; ++q) HIP_OK(enqueue_pass(pass + q, bound, nullptr));
+            HIP_OK(hipGetLastError());
+        }
+        HIP_OK(hipMemcpyAsync(c->h_flag, c->d_flag, 2 * kPassRing * sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        const bool graphed = use_graph && !graph_off;
+        hipEvent_t(&evc)[kPassRing][2] = graphed ? c->gevc : c->evc;
+        hipEvent_t(&evn)[kPassRing][1] = graphed ? c->gevn : c->evn;
+        for (int q = 0; q < ch; ++q) {
+            const int slot = (int) ((pass + q) % kPassRing);
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, evc[slot][0], evc[slot][1]));
+            c->control_ms += ms;
+            if (c->h_flag[2 * slot] == 0) {   // no zone needed a Newton iteration: all done
+                done_pass = pass + q;
+                break;
+            }
+            HIP_OK(hipEventElapsedTime(&ms, evc[slot][1], evn[slot][0]));
+            c->newton_ms += ms;
+            c->newton_launches++;
+            bound = std::min(c->h_flag[2 * slot + 1], nz);
+        }
+        pass += ch;
     }
-    c->last_passes = pass;
+    c->last_passes = done_pass;
     HIP_OK(hipEventRecord(c->ev[3], s));
     if (c->kn.integration_method == 1) {
         // the state the reference would carry into its next call: the last zone's (exact for
@@ -1106,9 +1275,9 @@ int nuppn_last_timing(nuppn_ctx *c, double *out, int nout) {
     HIP_OK(hipEventElapsedTime(&r, c->ev[0], c->ev[1]));
     HIP_OK(hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
     const double nms = c->newton_ms < 0.0 ? (double) t : c->newton_ms;
-    const double vals[6] = {r, t, nms, c->control_ms, (double) c->newton_launches,
-                            (double) c->last_passes};
-    for (int i = 0; i < nout && i < 6; ++i) out[i] = vals[i];
+    const double vals[7] = {r, t, nms, c->control_ms, (double) c->newton_launches,
+                            (double) c->last_passes, (double) c->graph_launches};
+    for (int i = 0; i < nout && i < 7; ++i) out[i] = vals[i];
     return NUPPN_OK;
 }
 
@@ -1145,7 +1314,8 @@ static int mixed_opts(const nuppn_batch_opts *opts, nuppn_batch_opts *o) {
     if (opts) *o = *opts;
     else nuppn_default_batch_opts(o);
     if ((o->factor_precision != 32 && o->factor_precision != 64) || o->max_refine < 1 ||
-        (o->team_waves != 0 && o->team_waves != 2 && o->team_waves != 4))
+        (o->team_waves != 0 && o->team_waves != 2 && o->team_waves != 4) ||
+        o->queue_passes < 1 || o->queue_passes > 16 || (o->use_graph != 0 && o->use_graph != 1))
         return NUPPN_ERR_ARG;
     return NUPPN_OK;
 }
@@ -1165,6 +1335,8 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->accept_tol = 0.0;
     o->team_zones = -1;
     o->team_waves = 0;
+    o->queue_passes = 16;
+    o->use_graph = 1;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {

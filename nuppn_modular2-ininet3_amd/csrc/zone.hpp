@@ -67,12 +67,18 @@ struct ZoneData {
     int *flag;          // [0] set when some zone needs a Newton iteration or a BD step,
                         // [1] the number of such zones
     int *active;        // [nz] those zones, in no particular order (the workers' work list)
-    int nact;           // entries of active the worker launch takes (set per pass)
+    int nact;           // entries of active the worker launch takes, or, when nact_dev is set,
+    const int *nact_dev;   // read on the device from there (the control pass's count)
     const int *ctl;     // the zones the control pass takes: the previous pass's work list, or
-    int nctl;           // every zone (ctl null) in the first pass
+    int nctl;           // every zone (ctl null) in the first pass; when nctl_dev is set, the
+    const int *nctl_dev;   // count is read on the device from there (the previous pass's)
     int nz;
     int init_prec;      // factor precision a zone starts with on the batched path (0 FP32, 1 FP64)
 };
+
+// the length of a work list: the host's value, or the count a previous kernel left on the device
+// (passes are queued ahead of the host knowing their counts)
+__device__ __forceinline__ int list_len(const int *dev, int n) { return dev ? *dev : n; }
 
 struct KArgs {
     DevNet N;

@@ -49,7 +49,8 @@ class BatchOpts(C.Structure):
     """nuppn_batch_opts: factor precision (32 / 64) and the FP64 refinement tolerances."""
     _fields_ = [("factor_precision", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("accept_tol", C.c_double),
-                ("team_zones", C.c_int), ("team_waves", C.c_int)]
+                ("team_zones", C.c_int), ("team_waves", C.c_int),
+                ("queue_passes", C.c_int), ("use_graph", C.c_int)]
 
 
 class Network(C.Structure):
@@ -324,12 +325,14 @@ class Context:
             raise RuntimeError(f"nuppn_integrate_batch_mixed_dev failed ({rc})")
 
     def last_timing(self):
-        """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes"""
-        out = np.zeros(6)
-        rc = load().nuppn_last_timing(self.h, out, 6)
+        """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes,
+        graph_launches"""
+        out = np.zeros(7)
+        rc = load().nuppn_last_timing(self.h, out, 7)
         if rc != OK:
             raise RuntimeError(f"nuppn_last_timing failed ({rc})")
-        keys = ("rates_ms", "total_ms", "newton_ms", "control_ms", "newton_launches", "passes")
+        keys = ("rates_ms", "total_ms", "newton_ms", "control_ms", "newton_launches", "passes",
+                "graph_launches")
         return dict(zip(keys, out.tolist()))
 
 

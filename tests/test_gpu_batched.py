@@ -132,6 +132,39 @@ def test_team_kernel_equals_single_wavefront(c1_full, hip):
         np.testing.assert_array_equal(ye4, ye1)
 
 
+def test_queued_and_graph_passes_equal_one_pass_at_a_time(c1_full, hip):
+    """Passes queued ahead of the host (the counts read on the device) and chunks of 16 passes
+    replayed from a captured hipGraph give the results of one pass per host round trip, bit for
+    bit; the graph path runs twice in a row (replays of one call's graphs after the previous
+    call's), with the program's own synchronising work in between."""
+    w, ctx = c1_full
+    sel = np.array(MIXED_ZONES)
+    args = _args(w, sel)
+    ref = ctx.integrate_batch_mixed(w.x0[sel], *args,
+                                    opts=hip.batch_opts(queue_passes=1, use_graph=0))
+    t_ref = ctx.last_timing()
+    assert t_ref["graph_launches"] == 0
+    runs = [("queued", hip.batch_opts(use_graph=0)), ("graph", hip.batch_opts()),
+            ("graph again", hip.batch_opts())]
+    for name, o in runs:
+        ie, y, ye, st = ctx.integrate_batch_mixed(w.x0[sel], *args, opts=o)
+        t = ctx.last_timing()
+        print(f"{name}: passes {t['passes']:.0f} newton launches {t['newton_launches']:.0f} graph "
+              f"launches {t['graph_launches']:.0f} total {t['total_ms']:.1f} ms "
+              f"(one pass at a time: {t_ref['total_ms']:.1f} ms)", flush=True)
+        np.testing.assert_array_equal(ie, ref[0])
+        np.testing.assert_array_equal(y, ref[1])
+        np.testing.assert_array_equal(ye, ref[2])
+        for f in ("nsubt", "n_newton", "nsucc", "nfail", "titers", "n_factor", "n_solve"):
+            np.testing.assert_array_equal(st[f], ref[3][f], err_msg=f)
+        assert t["passes"] == t_ref["passes"]
+        assert t["newton_launches"] == t_ref["newton_launches"]
+        if name.startswith("graph"):
+            assert t["graph_launches"] > 0
+        # the program's own eager work between the calls
+        _ = ctx.rates(args[0], args[2])
+
+
 def _exact_oracle(oracle, net):
     return oracle.Oracle(net, oracle.default_knobs(cyminformal=-1.0), oracle.LS_EXACT)
 
