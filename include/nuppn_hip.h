@@ -33,6 +33,8 @@ extern "C" {
                                          the reference stops ends the zone with NUPPN_BAD_STEP */
 #define NUPPN_ERR_SINGULAR     (-4)   /* exactly zero pivot (slu_solve returns info > 0 there) */
 #define NUPPN_ERR_UNSUPPORTED  (-5)   /* configuration outside the static-network path */
+#define NUPPN_ERR_MIXING       (-6)   /* the mixing step stops where mixing.F90 does: a zero zone
+                                         width or shell mass, or a zero pivot of tridag */
 
 /* Reaction network (physics/source/rates.F90:13-16, ppn_physics.F90:746-815 conventions).
  * k[m*nre + l], m = 0..7 holds k1..k8 of reaction l: k1,k3,k5,k7 are 0-based species indices,
@@ -217,6 +219,43 @@ int nuppn_integrate_batch_mixed_dev(nuppn_ctx *ctx, int nz, double *yps, const d
                                     double *ye, const double *dtwant, int *ierr,
                                     nuppn_zone_stats *stats, const nuppn_batch_opts *opts,
                                     void *stream);
+
+/* One mppnp cycle (frames/mppnp/source/mppnp.F90:637-781) on m >= 2 zones of a stellar model
+ * that share this network, zone-major yps[m][nsp] (Fortran yps(msl, nsp) holds it the other way:
+ * pass the transpose):
+ *   - zoneselect (637, 1080-1097): the zones with t9 > t9threshold are integrated, all in one batch
+ *     (path 1: the batched path with bopts, as nuppn_integrate_batch_mixed; path 0: the plain FP64
+ *     path), at constant T9 and rho over dzeit, Ye from the zone's abundances (719-724);
+ *   - per integrated zone, renorm_1d when |1 - sum X| > 5e-7 (737-738); ierr[z] and stats[z]
+ *     (zeros for the zones not selected).  Any non-zero ierr ends the call here (the reference
+ *     stops, 726-735): the zones keep their integrated abundances and mixing is not applied;
+ *   - do_mixing (765; mixing.F90:30-176 with idifmix 1): D_lagr from the Eulerian coefficient
+ *     d_eul, radius rad (solar units) and density rho, the implicit diffusion matrix over the mass
+ *     coordinates xm (code_source 0: GNV / EVL, matrixdiffus) or the shell masses dq (1: MES,
+ *     matrixdiffus_mesa) with coefficients limited at sig_term_limit, one tridiagonal solve per
+ *     considered species 1..nsp-1 (the last species is not mixed, as in mixdiff08);
+ *   - renorm_2d of every zone when the largest |1 - sum X| reaches 1e-6, then max(yps, 1e-99)
+ *     (768-781).
+ * Returns NUPPN_OK, NUPPN_ERR_MIXING, NUPPN_ERR_UNSUPPORTED (path 1 without a batched path) or a
+ * library error.  Replaces the master/worker cycle of mppnp.F90:653-781. */
+typedef struct {
+    double t9threshold;      /* zoneselect's threshold (ppn_frame.input t9threshold) */
+    double sig_term_limit;   /* 1e10 (ppn_frame.input) */
+    int code_source;         /* 0: GNV / EVL (mass coordinates), 1: MES (shell masses) */
+    int path;                /* 1 (default): batched path; 0: plain FP64 path */
+} nuppn_mppnp_opts;
+
+void nuppn_default_mppnp_opts(nuppn_mppnp_opts *o);
+int nuppn_mppnp_cycle(nuppn_ctx *ctx, int m, double *yps, const double *t9, const double *rho,
+                      const double *xm, const double *dq, const double *rad, const double *d_eul,
+                      double dzeit, const nuppn_mppnp_opts *opts, const nuppn_batch_opts *bopts,
+                      int *ierr, nuppn_zone_stats *stats);
+/* Same with device-resident arrays, enqueued on `stream` (a hipStream_t, NULL = default). */
+int nuppn_mppnp_cycle_dev(nuppn_ctx *ctx, int m, double *yps, const double *t9, const double *rho,
+                          const double *xm, const double *dq, const double *rad,
+                          const double *d_eul, double dzeit, const nuppn_mppnp_opts *opts,
+                          const nuppn_batch_opts *bopts, int *ierr, nuppn_zone_stats *stats,
+                          void *stream);
 
 /* Timing of the last batch call from HIP events on its stream: out[0] rate kernel ms,
  * out[1] whole integration ms, out[2] summed Newton-kernel ms, out[3] summed control-kernel

@@ -27,6 +27,7 @@
 #include "integrate.hpp"
 #include "zone.hpp"
 #include "batched.hpp"
+#include "mppnp.hpp"
 #include "symbolic.hpp"
 
 using namespace nuppn;
@@ -305,6 +306,13 @@ struct nuppn_ctx {
     double newton_ms = 0.0, control_ms = 0.0;
     long long newton_launches = 0, graph_launches = 0;
     hipGraphNode_t glast = nullptr;     // the last node of a pass graph being built
+    // the mppnp cycle (zone selection, batch gather / scatter, mixing): device buffers for m zones
+    int mp_cap = 0;
+    int *mp_zidx = nullptr, *mp_ierrb = nullptr, *mp_small = nullptr;   // small: count, bad
+    double *mp_yb = nullptr, *mp_t9b = nullptr, *mp_rhob = nullptr, *mp_dtb = nullptr;
+    double *mp_yeb = nullptr, *mp_dl = nullptr, *mp_a = nullptr, *mp_b = nullptr, *mp_c = nullptr;
+    double *mp_bet = nullptr, *mp_gam = nullptr, *mp_res = nullptr, *mp_max = nullptr;
+    ZoneStats *mp_stb = nullptr;
     long long last_passes = 0;
     std::mutex mu;
     ~nuppn_ctx();
@@ -328,6 +336,11 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
                     (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9,
                     (void *) d_b_resid_pos})
+        if (p) (void) hipFree(p);
+    for (void *p : {(void *) mp_zidx, (void *) mp_ierrb, (void *) mp_small, (void *) mp_yb,
+                    (void *) mp_t9b, (void *) mp_rhob, (void *) mp_dtb, (void *) mp_yeb,
+                    (void *) mp_dl, (void *) mp_a, (void *) mp_b, (void *) mp_c, (void *) mp_bet,
+                    (void *) mp_gam, (void *) mp_res, (void *) mp_max, (void *) mp_stb})
         if (p) (void) hipFree(p);
     for (void *p : d_seg_i)
         if (p) (void) hipFree(p);
@@ -1281,6 +1294,90 @@ int nuppn_last_timing(nuppn_ctx *c, double *out, int nout) {
     return NUPPN_OK;
 }
 
+// the mppnp cycle's device buffers for m zones (kept between calls)
+static int ensure_mp(nuppn_ctx *c, int m) {
+    if (m <= c->mp_cap) return 0;
+    for (void *p : {(void *) c->mp_zidx, (void *) c->mp_ierrb, (void *) c->mp_small,
+                    (void *) c->mp_yb, (void *) c->mp_t9b, (void *) c->mp_rhob, (void *) c->mp_dtb,
+                    (void *) c->mp_yeb, (void *) c->mp_dl, (void *) c->mp_a, (void *) c->mp_b,
+                    (void *) c->mp_c, (void *) c->mp_bet, (void *) c->mp_gam, (void *) c->mp_res,
+                    (void *) c->mp_max, (void *) c->mp_stb})
+        dev_free(p);
+    c->mp_cap = 0;
+    const size_t nsp = c->net.nsp;
+    HIP_OK(hipMalloc((void **) &c->mp_zidx, sizeof(int) * m));
+    HIP_OK(hipMalloc((void **) &c->mp_ierrb, sizeof(int) * m));
+    HIP_OK(hipMalloc((void **) &c->mp_small, sizeof(int) * 4));
+    HIP_OK(hipMalloc((void **) &c->mp_yb, sizeof(double) * nsp * m));
+    for (double **p : {&c->mp_t9b, &c->mp_rhob, &c->mp_dtb, &c->mp_yeb, &c->mp_dl, &c->mp_a,
+                       &c->mp_b, &c->mp_c, &c->mp_bet, &c->mp_gam, &c->mp_res})
+        HIP_OK(hipMalloc((void **) p, sizeof(double) * m));
+    HIP_OK(hipMalloc((void **) &c->mp_max, sizeof(double)));
+    HIP_OK(hipMalloc((void **) &c->mp_stb, sizeof(ZoneStats) * m));
+    c->mp_cap = m;
+    return 0;
+}
+
+// One mppnp cycle on device arrays (frames/mppnp/source/mppnp.F90:637-781): zoneselect, the
+// selected zones integrated as one batch (bo null: the plain FP64 path, else the batched path),
+// renorm_1d above 5e-7 per zone, then, unless a zone failed (the reference stops there), the
+// mixing of all m zones, renorm_2d when the largest residual reaches SUM_TOL, and the clip.
+static int run_mppnp(nuppn_ctx *c, int m, double *yps, const double *t9, const double *rho,
+                     const double *xm, const double *dq, const double *rad, const double *d_eul,
+                     double dzeit, const nuppn_mppnp_opts &mo, const nuppn_batch_opts *bo,
+                     int *ierr, nuppn_zone_stats *stats, hipStream_t s) {
+    const int nsp = c->net.nsp;
+    if ((size_t) nsp * 8 > 64 * 1024) return NUPPN_ERR_UNSUPPORTED;
+    if (ensure_mp(c, m)) return NUPPN_ERR_HIP;
+    ZoneStats *st = (ZoneStats *) stats;
+    int *cnt = c->mp_small, *bad = c->mp_small + 1;
+    hipLaunchKernelGGL(mp_select_kernel, dim3(1), dim3(64), 0, s, m, t9, mo.t9threshold,
+                       c->mp_zidx, cnt);
+    const int g = std::max(1, std::min(4096, (int) (((size_t) m * nsp + 255) / 256)));
+    hipLaunchKernelGGL(mp_gather_kernel, dim3(g), dim3(256), 0, s, m, nsp, c->mp_zidx, cnt, yps,
+                       t9, rho, dzeit, c->mp_yb, c->mp_t9b, c->mp_rhob, c->mp_dtb, c->mp_yeb,
+                       ierr, st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(c->h_flag, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const int n = c->h_flag[0];
+    const size_t lds = (size_t) nsp * 8;
+    if (n > 0) {
+        const int rc = run_batch(c, n, c->mp_yb, c->mp_t9b, c->mp_t9b, c->mp_rhob, c->mp_rhob,
+                                 c->mp_yeb, c->mp_dtb, c->mp_ierrb,
+                                 (nuppn_zone_stats *) c->mp_stb, bo, s);
+        if (rc != NUPPN_OK) return rc;
+        hipLaunchKernelGGL(mp_scatter_kernel, dim3(std::min(n, 4096)), dim3(64), lds, s, nsp,
+                           c->mp_zidx, cnt, c->mp_yb, yps, c->mp_ierrb, c->mp_stb, ierr, st);
+        HIP_OK(hipGetLastError());
+        // mppnp stops on any integration error (mppnp.F90:726-735): no mixing then
+        std::vector<int> eb(n);
+        HIP_OK(hipMemcpyAsync(eb.data(), c->mp_ierrb, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        for (int k = 0; k < n; ++k)
+            if (eb[k] != 0) return NUPPN_OK;
+    }
+    // do_mixing (mixing.F90:30-64)
+    HIP_OK(hipMemsetAsync(bad, 0, sizeof(int), s));
+    const int gm = std::max(1, std::min(1024, (m + 255) / 256));
+    hipLaunchKernelGGL(mp_dlagr_kernel, dim3(gm), dim3(256), 0, s, m, rho, rad, d_eul, c->mp_dl);
+    hipLaunchKernelGGL(mp_matrix_kernel, dim3(gm), dim3(256), 0, s, m, dzeit, xm, dq, c->mp_dl,
+                       mo.sig_term_limit, mo.code_source, c->mp_a, c->mp_b, c->mp_c, bad);
+    hipLaunchKernelGGL(mp_thomas_kernel, dim3(1), dim3(64), 0, s, m, c->mp_a, c->mp_b, c->mp_c,
+                       c->mp_bet, c->mp_gam, bad);
+    hipLaunchKernelGGL(mp_species_kernel, dim3((nsp + 255) / 256), dim3(256), 0, s, m, nsp,
+                       c->net.igamma, yps, c->mp_a, c->mp_bet, c->mp_gam, bad);
+    hipLaunchKernelGGL(mp_residual_kernel, dim3(std::min(m, 4096)), dim3(64), lds, s, m, nsp, yps,
+                       c->mp_res);
+    hipLaunchKernelGGL(mp_max_kernel, dim3(1), dim3(64), 0, s, m, c->mp_res, c->mp_max);
+    hipLaunchKernelGGL(mp_renorm_clip_kernel, dim3(std::min(m, 4096)), dim3(64), lds, s, m, nsp,
+                       yps, c->mp_max);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(c->h_flag + 1, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return c->h_flag[1] ? NUPPN_ERR_MIXING : NUPPN_OK;
+}
+
 // A host-array batch call: the staging buffers are filled, the batch run and the results read
 // back under the context's lock, so that concurrent calls on one context serialise whole.
 static int host_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, const double *t9_1,
@@ -1337,6 +1434,83 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o) {
     o->team_waves = 0;
     o->queue_passes = 16;
     o->use_graph = 1;
+}
+
+void nuppn_default_mppnp_opts(nuppn_mppnp_opts *o) {
+    o->t9threshold = 0.0;
+    o->sig_term_limit = 1.0e10;
+    o->code_source = 0;
+    o->path = 1;
+}
+
+static int mppnp_checked(nuppn_ctx *c, int m, const nuppn_mppnp_opts *mo_in,
+                         const nuppn_batch_opts *bo_in, nuppn_mppnp_opts *mo,
+                         nuppn_batch_opts *bo) {
+    if (!c || m < 2) return NUPPN_ERR_ARG;
+    if (mo_in) *mo = *mo_in;
+    else nuppn_default_mppnp_opts(mo);
+    if ((mo->code_source != 0 && mo->code_source != 1) || (mo->path != 0 && mo->path != 1))
+        return NUPPN_ERR_ARG;
+    if (mo->path == 1) {
+        if (mixed_opts(bo_in, bo) != NUPPN_OK) return NUPPN_ERR_ARG;
+        if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
+    }
+    return NUPPN_OK;
+}
+
+int nuppn_mppnp_cycle_dev(nuppn_ctx *c, int m, double *yps, const double *t9, const double *rho,
+                          const double *xm, const double *dq, const double *rad,
+                          const double *d_eul, double dzeit, const nuppn_mppnp_opts *opts,
+                          const nuppn_batch_opts *bopts, int *ierr, nuppn_zone_stats *stats,
+                          void *stream) {
+    nuppn_mppnp_opts mo;
+    nuppn_batch_opts bo;
+    const int rc = mppnp_checked(c, m, opts, bopts, &mo, &bo);
+    if (rc != NUPPN_OK) return rc;
+    if (!yps || !t9 || !rho || !xm || !dq || !rad || !d_eul || !ierr) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    return run_mppnp(c, m, yps, t9, rho, xm, dq, rad, d_eul, dzeit, mo,
+                     mo.path == 1 ? &bo : nullptr, ierr, stats, (hipStream_t) stream);
+}
+
+int nuppn_mppnp_cycle(nuppn_ctx *c, int m, double *yps, const double *t9, const double *rho,
+                      const double *xm, const double *dq, const double *rad, const double *d_eul,
+                      double dzeit, const nuppn_mppnp_opts *opts, const nuppn_batch_opts *bopts,
+                      int *ierr, nuppn_zone_stats *stats) {
+    nuppn_mppnp_opts mo;
+    nuppn_batch_opts bo;
+    int rc = mppnp_checked(c, m, opts, bopts, &mo, &bo);
+    if (rc != NUPPN_OK) return rc;
+    if (!yps || !t9 || !rho || !xm || !dq || !rad || !d_eul || !ierr) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t nsp = c->net.nsp, md = (size_t) m * sizeof(double);
+    // one staging block: yps, t9, rho, xm, dq, rad, d_eul, then ierr and the statistics
+    char *d = nullptr;
+    const size_t bytes = nsp * md + 6 * md + m * (sizeof(int) + sizeof(ZoneStats)) + 64;
+    HIP_OK(hipMalloc((void **) &d, bytes));
+    double *dy = (double *) d, *dv = dy + nsp * m;
+    const double *src[6] = {t9, rho, xm, dq, rad, d_eul};
+    int *di = (int *) (dv + 6 * (size_t) m);
+    ZoneStats *ds = (ZoneStats *) (((uintptr_t) (di + m) + 15) & ~(uintptr_t) 15);
+    rc = NUPPN_ERR_HIP;
+    if (hipMemcpy(dy, yps, nsp * md, hipMemcpyHostToDevice) == hipSuccess) {
+        bool ok = true;
+        for (int k = 0; k < 6 && ok; ++k)
+            ok = hipMemcpy(dv + k * (size_t) m, src[k], md, hipMemcpyHostToDevice) == hipSuccess;
+        if (ok) {
+            rc = run_mppnp(c, m, dy, dv, dv + m, dv + 2 * (size_t) m, dv + 3 * (size_t) m,
+                           dv + 4 * (size_t) m, dv + 5 * (size_t) m, dzeit, mo,
+                           mo.path == 1 ? &bo : nullptr, di, (nuppn_zone_stats *) ds, nullptr);
+            if ((rc == NUPPN_OK || rc == NUPPN_ERR_MIXING) &&
+                (hipMemcpy(yps, dy, nsp * md, hipMemcpyDeviceToHost) != hipSuccess ||
+                 hipMemcpy(ierr, di, sizeof(int) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+                 (stats && hipMemcpy(stats, ds, sizeof(ZoneStats) * m,
+                                     hipMemcpyDeviceToHost) != hipSuccess)))
+                rc = NUPPN_ERR_HIP;
+        }
+    }
+    (void) hipFree(d);
+    return rc;
 }
 
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {

@@ -30,7 +30,8 @@ module nuppn_hip
        NUPPN_MAX_SUBSTEPS = 2, NUPPN_BAD_STEP = 3, NUPPN_NEG_ABUNDS = 4, &
        NUPPN_HUGE_ABUNDS = 5, NUPPN_SUM_X = 6, NUPPN_GOT_NAN = 7, NUPPN_SMALL_DT = 11
   integer(c_int), parameter, public :: NUPPN_ERR_ARG = -1, NUPPN_ERR_HIP = -2, &
-       NUPPN_ERR_PATTERN = -3, NUPPN_ERR_SINGULAR = -4, NUPPN_ERR_UNSUPPORTED = -5
+       NUPPN_ERR_PATTERN = -3, NUPPN_ERR_SINGULAR = -4, NUPPN_ERR_UNSUPPORTED = -5, &
+       NUPPN_ERR_MIXING = -6
 
   ! reaction network: k(nre, 8) holds k1..k8 (0-based species indices and counts), rtype 0
   ! two-body, 1 photodisintegration, 2 decay; coef(nre, 7) the REACLIB a0..a6; aw / zw mass
@@ -62,13 +63,20 @@ module nuppn_hip
      integer(c_int) :: team_zones, team_waves, queue_passes, use_graph
   end type
 
+  ! options of the mppnp cycle (nuppn_mppnp_cycle)
+  type, bind(c), public :: nuppn_mppnp_opts
+     real(c_double) :: t9threshold, sig_term_limit
+     integer(c_int) :: code_source, path
+  end type
+
   public :: nuppn_default_knobs, nuppn_ctx_create, nuppn_ctx_destroy, nuppn_set_column_order, &
        nuppn_get_column_order, nuppn_column_order, nuppn_reduced_size, nuppn_reduced_pattern, &
        nuppn_rates_eval, nuppn_set_rate_cache, nuppn_rate_cache_table, nuppn_dxdt, &
        nuppn_jacobian_reduced, nuppn_lu_solve_reduced, nuppn_integrate_zone, &
        nuppn_integrate_batch, nuppn_integrate_batch_dev, nuppn_default_batch_opts, &
        nuppn_batch_block_info, nuppn_integrate_batch_mixed, nuppn_integrate_batch_mixed_dev, &
-       nuppn_last_timing, slu_solve, slu_solve_noalloc
+       nuppn_last_timing, slu_solve, slu_solve_noalloc, nuppn_default_mppnp_opts, &
+       nuppn_mppnp_cycle, nuppn_mppnp_cycle_dev
 
   interface
      subroutine nuppn_default_knobs(k) bind(c, name="nuppn_default_knobs")
@@ -239,6 +247,37 @@ module nuppn_hip
        integer(c_int), value :: nz
        type(c_ptr), value    :: yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, opts, &
                                 stream
+     end function
+
+     subroutine nuppn_default_mppnp_opts(o) bind(c, name="nuppn_default_mppnp_opts")
+       import :: nuppn_mppnp_opts
+       type(nuppn_mppnp_opts), intent(out) :: o
+     end subroutine
+
+     ! one mppnp cycle (frames/mppnp/source/mppnp.F90:637-781); yps is zone-major, yps(nsp, m)
+     integer(c_int) function nuppn_mppnp_cycle(ctx, m, yps, t9, rho, xm, dq, rad, d_eul, dzeit, &
+                                               opts, bopts, ierr, stats) &
+          bind(c, name="nuppn_mppnp_cycle")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value            :: ctx
+       integer(c_int), value         :: m
+       real(c_double), intent(inout) :: yps(*)
+       real(c_double), intent(in)    :: t9(*), rho(*), xm(*), dq(*), rad(*), d_eul(*)
+       real(c_double), value         :: dzeit
+       type(c_ptr), value            :: opts, bopts   ! c_null_ptr: defaults
+       integer(c_int), intent(out)   :: ierr(*)
+       type(c_ptr), value            :: stats         ! c_loc(stats(1)) or c_null_ptr
+     end function
+
+     integer(c_int) function nuppn_mppnp_cycle_dev(ctx, m, yps, t9, rho, xm, dq, rad, d_eul, &
+                                                   dzeit, opts, bopts, ierr, stats, stream) &
+          bind(c, name="nuppn_mppnp_cycle_dev")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: m
+       type(c_ptr), value    :: yps, t9, rho, xm, dq, rad, d_eul
+       real(c_double), value :: dzeit
+       type(c_ptr), value    :: opts, bopts, ierr, stats, stream
      end function
 
      integer(c_int) function nuppn_last_timing(ctx, out, nout) bind(c, name="nuppn_last_timing")

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnuppn_hip.so")
 
 OK = 0
-ERR_ARG, ERR_HIP, ERR_PATTERN, ERR_SINGULAR, ERR_UNSUPPORTED = -1, -2, -3, -4, -5
+ERR_ARG, ERR_HIP, ERR_PATTERN, ERR_SINGULAR, ERR_UNSUPPORTED, ERR_MIXING = -1, -2, -3, -4, -5, -6
 IERR_NAMES = {0: "OK", 1: "CONVERGENCE", 2: "MAX_STEPS", 3: "BAD_STEP", 4: "NEGATIVE_X",
               5: "HUGE_X", 6: "SUM_X", 7: "NAN", 11: "DT_UNDRFLOW", -3: "PATTERN",
               -4: "SINGULAR"}
@@ -51,6 +51,12 @@ class BatchOpts(C.Structure):
                 ("refine_tol", C.c_double), ("accept_tol", C.c_double),
                 ("team_zones", C.c_int), ("team_waves", C.c_int),
                 ("queue_passes", C.c_int), ("use_graph", C.c_int)]
+
+
+class MppnpOpts(C.Structure):
+    """nuppn_mppnp_opts: zoneselect threshold, mixing limit and matrix form, integration path."""
+    _fields_ = [("t9threshold", C.c_double), ("sig_term_limit", C.c_double),
+                ("code_source", C.c_int), ("path", C.c_int)]
 
 
 class Network(C.Structure):
@@ -104,6 +110,13 @@ def load():
                                         _ip, C.c_void_p]
     L.nuppn_integrate_batch_dev.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
     L.nuppn_last_timing.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.nuppn_default_mppnp_opts.argtypes = [C.POINTER(MppnpOpts)]
+    L.nuppn_mppnp_cycle.argtypes = ([C.c_void_p, C.c_int] + [_dp] * 7 +
+                                    [C.c_double, C.POINTER(MppnpOpts), C.POINTER(BatchOpts), _ip,
+                                     C.c_void_p])
+    L.nuppn_mppnp_cycle_dev.argtypes = ([C.c_void_p, C.c_int] + [C.c_void_p] * 7 +
+                                        [C.c_double, C.POINTER(MppnpOpts), C.POINTER(BatchOpts),
+                                         C.c_void_p, C.c_void_p, C.c_void_p])
     L.nuppn_default_batch_opts.argtypes = [C.POINTER(BatchOpts)]
     L.nuppn_batch_block_info.argtypes = [C.c_void_p, _ip, C.c_int]
     L.nuppn_integrate_batch_mixed.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -136,6 +149,14 @@ def batch_opts(**over) -> BatchOpts:
     load().nuppn_default_batch_opts(C.byref(o))
     for key, val in over.items():
         setattr(o, key, val)
+    return o
+
+
+def mppnp_opts(**over) -> MppnpOpts:
+    o = MppnpOpts()
+    load().nuppn_default_mppnp_opts(C.byref(o))
+    for k, v in over.items():
+        setattr(o, k, v)
     return o
 
 
@@ -323,6 +344,23 @@ class Context:
                                                     dtwant, ierr, stats, C.byref(o), stream)
         if rc != OK:
             raise RuntimeError(f"nuppn_integrate_batch_mixed_dev failed ({rc})")
+
+    def mppnp_cycle(self, yps, t9, rho, xm, dq, rad, d_eul, dzeit, opts=None, bopts=None):
+        """One mppnp cycle (zoneselect, batch integration, renormalisation, mixing) on host
+        arrays, yps[m][nsp] zone-major.  Returns (rc, ierr, yps, stats); rc is NUPPN_OK or
+        NUPPN_ERR_MIXING (other codes raise)."""
+        y = _f64(yps).copy()
+        m = y.shape[0]
+        ierr = np.zeros(m, np.int32)
+        stats = np.zeros(m, STATS_DTYPE)
+        mo = opts if opts is not None else mppnp_opts()
+        bo = bopts if bopts is not None else batch_opts()
+        rc = load().nuppn_mppnp_cycle(self.h, m, y, _f64(t9), _f64(rho), _f64(xm), _f64(dq),
+                                      _f64(rad), _f64(d_eul), float(dzeit), C.byref(mo),
+                                      C.byref(bo), ierr, stats.ctypes.data_as(C.c_void_p))
+        if rc not in (OK, ERR_MIXING):
+            raise RuntimeError(f"nuppn_mppnp_cycle failed ({rc})")
+        return rc, ierr, y, stats
 
     def last_timing(self):
         """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes,

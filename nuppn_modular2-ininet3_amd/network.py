@@ -260,3 +260,45 @@ def config(cfg: int, nz: Optional[int] = None, seed: Optional[int] = None) -> Wo
         return Workload("c3", net, np.stack([t9, t9], 1), np.stack([rho, rho], 1),
                         np.full(n, 1e4), x0)
     raise ValueError(f"unknown workload {cfg}")
+
+
+@dataclasses.dataclass
+class StellarModel:
+    """The per-zone structure an mppnp cycle reads besides the abundances (frames/mppnp/source/
+    mppnp.F90:600-640, from the stellar-evolution input): mass coordinate ``xm`` (solar masses,
+    increasing outward), shell masses ``dq`` (MESA convention, dq_reverse), radius ``rad`` (solar
+    radii), density ``rho`` (g/cc), temperature ``t9`` and the Eulerian diffusion coefficient
+    ``d_eul`` (cm^2/s: large in the convective zones, 0 in the radiative ones)."""
+
+    xm: np.ndarray
+    dq: np.ndarray
+    rad: np.ndarray
+    rho: np.ndarray
+    t9: np.ndarray
+    d_eul: np.ndarray
+
+    @property
+    def m(self):
+        return self.xm.size
+
+
+def stellar_model(m: int, seed: int = SEED_BASE + 7, t9_center: float = 2.0,
+                  t9_surface: float = 0.01) -> StellarModel:
+    """A synthetic star of ``m`` zones: a hot, dense, convective core, a radiative envelope with
+    one convective shell, and a cool convective surface layer.  T9 falls from ``t9_center`` to
+    ``t9_surface`` so that zoneselect keeps the inner part of the star; shell masses are uneven
+    (log-normal scatter around an even split of one solar mass)."""
+    rng = np.random.default_rng([seed, m])
+    dq = np.exp(rng.normal(0.0, 0.3, m))
+    dq /= dq.sum()
+    xm = np.cumsum(dq)
+    q = xm / xm[-1]
+    rad = 0.02 + 0.98 * q ** 1.5
+    rho = 10.0 ** (2.5 - 6.0 * q)
+    t9 = t9_center * (t9_surface / t9_center) ** q
+    d_eul = np.zeros(m)
+    d_eul[q < 0.15] = 10.0 ** rng.uniform(14.0, 15.0, int((q < 0.15).sum()))
+    shell = (q > 0.45) & (q < 0.55)
+    d_eul[shell] = 10.0 ** rng.uniform(12.0, 13.0, int(shell.sum()))
+    d_eul[q > 0.9] = 10.0 ** rng.uniform(15.0, 16.0, int((q > 0.9).sum()))
+    return StellarModel(xm, dq, rad, rho, t9, d_eul)

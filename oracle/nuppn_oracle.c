@@ -1443,3 +1443,239 @@ out:
     free(yps_old);
     return ierr;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* The mppnp cycle around the integration: zone selection, per-zone renormalisation, diffusive
+ * mixing over the zones (frames/mppnp/source/mppnp.F90:637-781, 1080-1097;
+ * frames/mppnp/source/mixing.F90; utils/source/utils.F90:387-469,670-684) */
+
+/* zoneselect, mppnp.F90:1080-1097: zones hotter than the threshold; returns mmax (1-based) */
+int orc_zoneselect(double t9threshold, const double *t9, int m, unsigned char *sel)
+{
+    int mmax = 0;
+    for (int j = 1; j <= m; ++j) {
+        sel[j - 1] = 0;
+        if (t9[j - 1] > t9threshold) {
+            sel[j - 1] = 1;
+            mmax = j;
+        }
+    }
+    return mmax;
+}
+
+/* ksum (Kahan summation), utils.F90:670-684 */
+double orc_ksum(int n, const double *a)
+{
+    double c = 0.0, s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double y = a[i] - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    return s;
+}
+
+/* check_1d_mass_cons without sorting, utils.F90:387-397: |1 - sum(yps)| */
+double orc_mass_residual(int n, const double *y)
+{
+    return fabs(1.0 - plain_sum(n, y));
+}
+
+/* renorm_1d, utils.F90:433-444 (the message aside): yps = yps / ksum(yps) */
+void orc_renorm_1d(int n, double *y)
+{
+    const double s = orc_ksum(n, y);
+    for (int i = 0; i < n; ++i) y[i] = y[i] / s;
+}
+
+/* mix_calculate_D_Lagrangian, mixing.F90:68-110: D_lagr = D_eul (4 pi Rsun^2 / Msun)^2
+ * exp(2 (2 ln r + ln rho)), r and rho averaged in logarithm over the zone and its inner
+ * neighbour; 0 where D_eul <= 0.  (dconst ** 2 is the compile-time constant dconst * dconst.) */
+void orc_mix_d_lagrangian(int m, const double *rho, const double *rad, const double *d_eul,
+                          double *d_lagr)
+{
+    const double dconst = 3.0589703419947736e-11;
+    const double dc2 = dconst * dconst;
+    for (int j = 0; j < m; ++j) {
+        d_lagr[j] = 0.0;
+        if (d_eul[j] <= 0.0) continue;
+        double lnrho, lnrad;
+        if (j == 0) {
+            lnrho = log(rho[j]);
+            lnrad = log(rad[j]);
+        } else {
+            lnrho = (log(rho[j]) + log(rho[j - 1])) / 2.0;
+            lnrad = (log(rad[j]) + log(rad[j - 1])) / 2.0;
+        }
+        d_lagr[j] = d_eul[j] * dc2 * exp(2.0 * (2.0 * lnrad + lnrho));
+    }
+}
+
+/* matrixdiffus, mixing.F90:180-250 (code_source GNV / EVL): the implicit diffusion matrix on
+ * mass coordinates xm; D is limited in place where D(j+1) h / (xmm xr) exceeds sig_term_limit.
+ * Returns 0, or 1 where the reference stops (a zero zone width). */
+int orc_matrixdiffus(int m, double h, const double *xm, double *D, double sig, double *a,
+                     double *b, double *c)
+{
+    /* Fortran indices 1..m as [j - 1] */
+    double xr = xm[1] - xm[0];
+    double alpha = D[1] * h / (xr * xr);
+    if (sig < alpha) alpha = sig;
+    a[0] = 0.0;
+    b[0] = 1.0 + alpha;
+    c[0] = -1.0 * alpha;
+    double xl = 0.0;
+    for (int j = 2; j <= m - 1; ++j) {
+        xl = xm[j - 1] - xm[j - 2];
+        xr = xm[j] - xm[j - 1];
+        const double xmm = 0.5 * (xm[j] - xm[j - 2]);
+        alpha = h / xmm;
+        if (sig < D[j] * alpha / xr) D[j] = sig / alpha * xr;
+        a[j - 1] = -1.0 * D[j - 1] * alpha / xl;
+        b[j - 1] = 1.0 + (D[j] / xr + D[j - 1] / xl) * alpha;
+        c[j - 1] = -1.0 * D[j] * alpha / xr;
+        if (xl == 0.0 || xr == 0.0) return 1;
+    }
+    xl = xm[m - 1] - xm[m - 2];
+    alpha = D[m - 1] * h / (xl * xl);
+    a[m - 1] = -1.0 * alpha;
+    b[m - 1] = 1.0 + alpha;
+    c[m - 1] = 0.0;
+    if (xl == 0.0 || xr == 0.0) return 1;
+    return 0;
+}
+
+/* reduce_term, mixing.F90:325-340 */
+static void reduce_term(double alpha, double onedxr, double *d_coeff, double sig)
+{
+    if (alpha * onedxr * *d_coeff > sig) *d_coeff = sig / (alpha * onedxr);
+}
+
+/* matrixdiffus_mesa, mixing.F90:254-322 (code_source MES): the same on shell masses dq, with
+ * the limited coefficients kept apart (d_reduce) */
+void orc_matrixdiffus_mesa(int m, double h, const double *dq, const double *d, double sig,
+                           double *a, double *b, double *c)
+{
+    double *dr = calloc((size_t) m + 1, sizeof(double));
+    dr[0] = d[0];
+    double xr = dq[0];
+    double alpha = h / xr;
+    double term_c = 1.0 / xr;
+    double d_coeff = d[1];
+    reduce_term(alpha, term_c, &d_coeff, sig);
+    dr[1] = d_coeff;
+    a[0] = 0.0;
+    b[0] = 1.0 + term_c * alpha * dr[1];
+    c[0] = -1.0 * term_c * alpha * dr[1];
+    for (int j = 2; j <= m - 1; ++j) {
+        const double xl = dq[j - 2];
+        xr = dq[j - 1];
+        const double xmm = 0.5 * (dq[j - 1] + dq[j - 2]);
+        alpha = h / xmm;
+        const double term_a = 1.0 / xl;
+        term_c = 1.0 / xr;
+        d_coeff = d[j];
+        reduce_term(alpha, term_c, &d_coeff, sig);
+        dr[j] = d_coeff;
+        const double term_b = term_c * dr[j] + term_a * dr[j - 1];
+        a[j - 1] = -1.0 * term_a * dr[j - 1] * alpha;
+        b[j - 1] = 1.0 + term_b * alpha;
+        c[j - 1] = -1.0 * term_c * dr[j] * alpha;
+    }
+    const double xl = dq[m - 1];
+    alpha = h / (xl * xl);
+    a[m - 1] = -1.0 * alpha * d[m - 1];
+    b[m - 1] = 1.0 + alpha * d[m - 1];
+    c[m - 1] = 0.0;
+    free(dr);
+}
+
+/* tridag, mixing.F90:140-176 (Numerical Recipes' Thomas algorithm).  Returns 0, or 1 where the
+ * reference stops (a zero pivot). */
+int orc_tridag(int n, const double *a, const double *b, const double *c, const double *r,
+               double *u)
+{
+    double *gam = malloc(sizeof(double) * (size_t) (n + 1));
+    if (b[0] == 0.0) { free(gam); return 1; }
+    double bet = b[0];
+    u[0] = r[0] / bet;
+    for (int j = 1; j < n; ++j) {
+        gam[j] = c[j - 1] / bet;
+        bet = b[j] - a[j] * gam[j];
+        if (bet == 0.0) { free(gam); return 1; }
+        u[j] = (r[j] - a[j] * u[j - 1]) / bet;
+    }
+    for (int j = n - 2; j >= 0; --j) u[j] = u[j] - gam[j + 1] * u[j + 1];
+    free(gam);
+    return 0;
+}
+
+/* do_mixing with idifmix = 1 (mixing.F90:30-64): D_lagr from D_eul, the matrix
+ * (matrixdiffus, or matrixdiffus_mesa for code_source MES), and one tridag per considered
+ * species over the zones, species 1..NSP-1 (the loop of mixdiff08, mixing.F90:118-130, leaves
+ * the last species out).  yps[m][nsp] zone-major; consider[nsp].  Returns 0 or 1 (stop). */
+int orc_do_mixing(int m, int nsp, double *yps, const unsigned char *consider, const double *xm,
+                  const double *dq, const double *rho, const double *rad, const double *d_eul,
+                  double dzeit, int mesa, double sig)
+{
+    double *dl = malloc(sizeof(double) * (size_t) m), *a = malloc(sizeof(double) * (size_t) m);
+    double *b = malloc(sizeof(double) * (size_t) m), *c = malloc(sizeof(double) * (size_t) m);
+    double *yn = malloc(sizeof(double) * (size_t) m), *u = malloc(sizeof(double) * (size_t) m);
+    int rc = 0;
+    orc_mix_d_lagrangian(m, rho, rad, d_eul, dl);
+    if (mesa) {
+        if (!(dq[0] > 0.0)) rc = 1;
+        else orc_matrixdiffus_mesa(m, dzeit, dq, dl, sig, a, b, c);
+    } else {
+        rc = orc_matrixdiffus(m, dzeit, xm, dl, sig, a, b, c);
+    }
+    for (int s = 0; s < nsp - 1 && rc == 0; ++s) {
+        if (!consider[s]) continue;
+        for (int j = 0; j < m; ++j) yn[j] = yps[(size_t) j * nsp + s];
+        rc = orc_tridag(m, a, b, c, yn, u);
+        if (rc == 0)
+            for (int j = 0; j < m; ++j) yps[(size_t) j * nsp + s] = u[j];
+    }
+    free(dl); free(a); free(b); free(c); free(yn); free(u);
+    return rc;
+}
+
+/* One mppnp cycle (mppnp.F90:637-781) on zones that share this network: zoneselect; for every
+ * selected zone calculate_ye + evaluate_all_rates at (T9, rho) + integrate_network over dzeit
+ * at constant T9 and rho (719-724), then the mass check and renorm_1d above 5e-7 (737-738);
+ * a zone's integration error ends the cycle there (the reference stops, 726-735);
+ * do_mixing over all m zones (765), the 2-d mass check and renorm_2d when the largest residual
+ * reaches SUM_TOL (768-773), and the clip max(yps, 1e-99) (781).  yps[m][nsp]; ierr[m] (0 for
+ * unselected zones); stats[m] may be NULL.  Returns 0, or 1 where mixing stops. */
+int orc_mppnp_cycle(orc_ctx *c, int m, double *yps, const double *t9, const double *rho,
+                    const double *xm, const double *dq, const double *rad, const double *d_eul,
+                    double dzeit, double t9threshold, int mesa, double sig, int *ierr,
+                    orc_stats *stats)
+{
+    const int nsp = c->nsp;
+    unsigned char *sel = malloc((size_t) m);
+    orc_zoneselect(t9threshold, t9, m, sel);
+    for (int j = 0; j < m; ++j) {
+        ierr[j] = 0;
+        if (stats) memset(&stats[j], 0, sizeof(orc_stats));
+        if (!sel[j]) continue;
+        double *y0 = yps + (size_t) j * nsp;
+        double ye = 0.0;
+        ierr[j] = orc_integrate_zone(c, y0, t9[j], t9[j], rho[j], rho[j], &ye, dzeit,
+                                     stats ? &stats[j] : NULL);
+        if (orc_mass_residual(nsp, y0) > 5.0e-7) orc_renorm_1d(nsp, y0);
+    }
+    free(sel);
+    /* any integration error stops mppnp before mixing (mppnp.F90:726-735) */
+    for (int j = 0; j < m; ++j)
+        if (ierr[j] != 0) return 0;
+    int rc = orc_do_mixing(m, nsp, yps, c->consider, xm, dq, rho, rad, d_eul, dzeit, mesa, sig);
+    if (rc) return rc;
+    double res = 0.0;
+    for (int j = 0; j < m; ++j) res = fmax(res, orc_mass_residual(nsp, yps + (size_t) j * nsp));
+    if (res >= SUM_TOL)
+        for (int j = 0; j < m; ++j) orc_renorm_1d(nsp, yps + (size_t) j * nsp);
+    for (size_t i = 0; i < (size_t) m * nsp; ++i) yps[i] = fmax(yps[i], 1.0e-99);
+    return 0;
+}

@@ -125,6 +125,28 @@ double orc_sortedsum(int n, const double *a);
 double orc_calculate_ye(const orc_ctx *c, const double *x);
 void   orc_ypsplus(int n, double *y);
 
+/* The mppnp cycle around the integration (frames/mppnp/source/mppnp.F90:637-781,
+ * mixing.F90, utils.F90:387-469,670-684).  yps[m][nsp] zone-major. */
+int    orc_zoneselect(double t9threshold, const double *t9, int m, unsigned char *sel);
+double orc_ksum(int n, const double *a);
+double orc_mass_residual(int n, const double *y);
+void   orc_renorm_1d(int n, double *y);
+void   orc_mix_d_lagrangian(int m, const double *rho, const double *rad, const double *d_eul,
+                            double *d_lagr);
+int    orc_matrixdiffus(int m, double h, const double *xm, double *D, double sig, double *a,
+                        double *b, double *c);
+void   orc_matrixdiffus_mesa(int m, double h, const double *dq, const double *d, double sig,
+                             double *a, double *b, double *c);
+int    orc_tridag(int n, const double *a, const double *b, const double *c, const double *r,
+                  double *u);
+int    orc_do_mixing(int m, int nsp, double *yps, const unsigned char *consider, const double *xm,
+                     const double *dq, const double *rho, const double *rad, const double *d_eul,
+                     double dzeit, int mesa, double sig);
+int    orc_mppnp_cycle(orc_ctx *c, int m, double *yps, const double *t9, const double *rho,
+                       const double *xm, const double *dq, const double *rad, const double *d_eul,
+                       double dzeit, double t9threshold, int mesa, double sig, int *ierr,
+                       orc_stats *stats);
+
 /* Bader-Deuflhard tables, bader_deuflhard.F90:66-124 */
 void orc_bd_tables(int *nseq9, double *a9, double *alf64, int *kopt, int *kmax);
 

@@ -30,5 +30,8 @@ int main(void) {
            O(nuppn_batch_opts, refine_tol), O(nuppn_batch_opts, accept_tol),
            O(nuppn_batch_opts, team_zones), O(nuppn_batch_opts, team_waves),
            O(nuppn_batch_opts, queue_passes), O(nuppn_batch_opts, use_graph));
+    printf("nuppn_mppnp_opts %d %d %d %d %d\n", (int) sizeof(nuppn_mppnp_opts),
+           O(nuppn_mppnp_opts, t9threshold), O(nuppn_mppnp_opts, sig_term_limit),
+           O(nuppn_mppnp_opts, code_source), O(nuppn_mppnp_opts, path));
     return 0;
 }

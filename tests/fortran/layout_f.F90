@@ -9,7 +9,8 @@ program layout_f
   type(nuppn_knobs), target :: k
   type(nuppn_zone_stats), target :: s
   type(nuppn_batch_opts), target :: o
-  type(c_funptr) :: f(24)
+  type(nuppn_mppnp_opts), target :: q
+  type(c_funptr) :: f(27)
   integer(c_intptr_t) :: b
 
   b = transfer(c_loc(n), b)
@@ -33,6 +34,9 @@ program layout_f
        off(c_loc(o%max_refine)), off(c_loc(o%refine_tol)), off(c_loc(o%accept_tol)), &
        off(c_loc(o%team_zones)), off(c_loc(o%team_waves)), off(c_loc(o%queue_passes)), &
        off(c_loc(o%use_graph))
+  b = transfer(c_loc(q), b)
+  write(*, '(a, 5(1x, i0))') 'nuppn_mppnp_opts', c_sizeof(q), off(c_loc(q%t9threshold)), &
+       off(c_loc(q%sig_term_limit)), off(c_loc(q%code_source)), off(c_loc(q%path))
 
   f(1) = c_funloc(nuppn_default_knobs)
   f(2) = c_funloc(nuppn_ctx_create)
@@ -58,7 +62,10 @@ program layout_f
   f(22) = c_funloc(nuppn_last_timing)
   f(23) = c_funloc(slu_solve)
   f(24) = c_funloc(slu_solve_noalloc)
-  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 24)])
+  f(25) = c_funloc(nuppn_default_mppnp_opts)
+  f(26) = c_funloc(nuppn_mppnp_cycle)
+  f(27) = c_funloc(nuppn_mppnp_cycle_dev)
+  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 27)])
 contains
   integer(c_intptr_t) function off(p)
     type(c_ptr), intent(in) :: p

@@ -97,6 +97,23 @@ def lib():
     L.orc_lu_solve.argtypes = [C.c_void_p, _dp]
     L.orc_lu_info.argtypes = [C.c_void_p, _ip, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.orc_lu_free.argtypes = [C.c_void_p]
+    _bp1 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+    L.orc_zoneselect.argtypes = [C.c_double, _dp, C.c_int, _bp1]
+    L.orc_ksum.restype = C.c_double
+    L.orc_ksum.argtypes = [C.c_int, _dp]
+    L.orc_mass_residual.restype = C.c_double
+    L.orc_mass_residual.argtypes = [C.c_int, _dp]
+    L.orc_renorm_1d.argtypes = [C.c_int, _dp]
+    L.orc_mix_d_lagrangian.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    L.orc_matrixdiffus.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double, _dp, _dp, _dp]
+    L.orc_matrixdiffus_mesa.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double, _dp, _dp,
+                                        _dp]
+    L.orc_tridag.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_do_mixing.argtypes = [C.c_int, C.c_int, _dp, _bp1, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                C.c_int, C.c_double]
+    L.orc_mppnp_cycle.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                  C.c_double, C.c_double, C.c_int, C.c_double, _ip,
+                                  C.POINTER(Stats)]
     L.orc_ref_open.argtypes = [C.c_char_p]
     L.orc_ref_slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                                     C.POINTER(C.c_int)]
@@ -254,6 +271,29 @@ class Oracle:
                                               float(rho_0), float(rho_1), C.byref(yeo),
                                               float(dtwant), C.byref(st))
         return ierr, y, yeo.value, st
+
+
+def mppnp_cycle(o, yps, t9, rho, xm, dq, rad, d_eul, dzeit, t9threshold=0.0, mesa=False,
+                sig=1.0e10):
+    """The oracle's mppnp cycle (orc_mppnp_cycle) on Oracle o: (rc, ierr, yps, stats list)."""
+    y = f64(yps).copy()
+    m = y.shape[0]
+    ierr = np.zeros(m, np.int32)
+    st = (Stats * m)()
+    rc = lib().orc_mppnp_cycle(o.h, m, y, f64(t9), f64(rho), f64(xm), f64(dq), f64(rad),
+                               f64(d_eul), float(dzeit), float(t9threshold), int(mesa),
+                               float(sig), ierr, st)
+    return rc, ierr, y, [st[i] for i in range(m)]
+
+
+def do_mixing(yps, consider, xm, dq, rho, rad, d_eul, dzeit, mesa=False, sig=1.0e10):
+    """orc_do_mixing on a copy of yps[m][nsp]: (rc, yps)."""
+    y = f64(yps).copy()
+    m, nsp = y.shape
+    rc = lib().orc_do_mixing(m, nsp, y, np.ascontiguousarray(consider, np.uint8), f64(xm),
+                             f64(dq), f64(rho), f64(rad), f64(d_eul), float(dzeit), int(mesa),
+                             float(sig))
+    return rc, y
 
 
 def csr_to_csc(n, rowptr, colind, val):
