@@ -96,6 +96,42 @@ void nuppn_default_knobs(nuppn_knobs *k);
 nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, int *status);
 void nuppn_ctx_destroy(nuppn_ctx *ctx);
 
+/* Rate physics beyond the bare REACLIB sum, as evaluate_all_rates applies it
+ * (physics/source/evaluate_rates.F90:14-172; the reference's defaults detailed_balance = .true.,
+ * screen_option = CHUGUNOV, physics_knobs.F90:110-111):
+ *   - partition functions on REACLIB's T9 grid (pf_t9 [pf_n] ascending, pf_tab [nsp][pf_n]),
+ *     interpolated linearly at the clipped temperature (reaclib.F90:586-605), multiply the REACLIB
+ *     reverse ('v') rates by their chapter's Gfac (reaclib.F90:512-557): rev_ch [nre] is the
+ *     chapter 1..11 of such a rate (0 otherwise), rev_part [6][nre] its participants e1..e6
+ *     (species, -1 none);
+ *   - detailed_balance: every forward reaction paired as reverse_init pairs it (reverse.F90:25-105)
+ *     sets its partner's rate from its own (reverse.F90:107-337): g_spin [nsp] spin factors,
+ *     mass_g [nsp] nuclear masses (g), q_mev [nsp] binding energies (MeV);
+ *   - screen 1: Chugunov, DeWitt & Yakovlev (2007) screening of every reaction between charged
+ *     reactants at the zone's abundances (screening_chugunov.F90:86-235).
+ * The batch and zone entry points then evaluate the rates at each zone's abundances.  pf_n = 0
+ * and screen = 0 (or a null pointer) switch it all off: the bare REACLIB sum. */
+typedef struct {
+    int pf_n;
+    const double *pf_t9;
+    const double *pf_tab;
+    const int *rev_ch;
+    const int *rev_part;
+    int detailed_balance;
+    const double *g_spin;
+    const double *mass_g;
+    const double *q_mev;
+    int screen;
+} nuppn_rate_physics;
+
+int nuppn_set_rate_physics(nuppn_ctx *ctx, const nuppn_rate_physics *rp);
+/* reverse_init's pairing in use: irev[nre] = the reverse partner of a forward reaction, -1 */
+int nuppn_reverse_pairs(nuppn_ctx *ctx, int *irev);
+/* evaluate_all_rates (evaluate_rates.F90:14-172) for nz states at abundances x [nz][nsp]:
+ * v [nz][nre]; equals nuppn_rates_eval without rate physics */
+int nuppn_evaluate_all_rates(nuppn_ctx *ctx, int nz, const double *t9, const double *rho,
+                             const double *x, double *v);
+
 /* Column order used by the factorisation, SuperLU convention (perm_c[i] = position of reduced
  * column i).  set: caller-supplied order, like SuperLU's MY_PERMC; get: the order in use. */
 int nuppn_set_column_order(nuppn_ctx *ctx, const int *perm_c);

@@ -153,7 +153,8 @@ __device__ __forceinline__ T9Pow t9_powers(double t9in) {
 // one rate: REACLIB sum in the reference's order, exp, density scaling by reaction type,
 // the 1e-99 floors of reaclib.F90:581 and evaluate_rates.F90:46,172
 __device__ __forceinline__ double reaclib_rate(const double *__restrict__ coef, int nre, int l,
-                                               int rtype, const T9Pow &p, double rho) {
+                                               int rtype, const T9Pow &p, double rho,
+                                               double gfac = 1.0, bool reverse = false) {
     double r0 = coef[l] + coef[nre + l] * p.t9inv;
     r0 = r0 + coef[2 * nre + l] * p.p13inv;
     r0 = r0 + coef[3 * nre + l] * p.p13;
@@ -161,10 +162,81 @@ __device__ __forceinline__ double reaclib_rate(const double *__restrict__ coef, 
     r0 = r0 + coef[5 * nre + l] * p.p53;
     r0 = r0 + coef[6 * nre + l] * p.lnt9;
     double reacv = 0.0 + exp(r0);
+    if (reverse) reacv = reacv * gfac;   // a REACLIB reverse rate's partition functions
     double rv = (rtype == 0) ? reacv * rho : reacv;
     rv = fmax(rv, 1.0e-99);
     rv = fmax(rv * 1.0, 1.0e-99);
     return fmax(rv, 1.0e-99);
+}
+
+// Rate physics beyond the bare REACLIB sum (physics/source/evaluate_rates.F90:14-172), all off
+// when pf_n and screen are 0: partition functions on REACLIB's T9 grid and the factors of its
+// reverse ('v') rates (reaclib.F90:512-557, 586-605), detailed-balance reverse rates of the
+// paired reactions (reverse.F90:107-337), Chugunov screening (screening_chugunov.F90).
+struct DevPhys {
+    int pf_n;                // grid points (0: no partition functions, no detailed balance)
+    const double *pf_t9;     // [pf_n]
+    const double *pf_tab;    // [nsp][pf_n]
+    const int *rev_ch;       // [nre] chapter of a REACLIB reverse rate, 0 otherwise
+    const int *rev_part;     // [6][nre] participants e1..e6 (species or -1)
+    int ndb;                 // detailed-balance pairs
+    const int *db_fwd;       // [ndb] forward reaction
+    const int *db_rev;       // [ndb] its reverse partner
+    const double *g_spin, *mass_g, *q_mev;   // [nsp]
+    int screen;              // 1: Chugunov screening
+};
+
+// reaclib_interpolate_partition_functions (reaclib.F90:586-605) with bsearch_r8
+// (utils.F90:687-722): the interval of the clipped temperature and the fraction within it
+struct PfPos {
+    int i0;
+    double f;
+};
+
+__device__ inline PfPos pf_pos(const DevPhys &P, double t9_in) {
+    const int n = P.pf_n;
+    const double *rt9 = P.pf_t9;
+    const double t9 = fmax(rt9[0], fmin(t9_in, rt9[n - 1]));
+    int a = 1, b = n, r = b - a, m = (a + b) / 2;
+    while (t9 != rt9[m - 1] && r > 1) {
+        if (rt9[m - 1] > t9) b = m;
+        else a = m;
+        r = b - a;
+        m = (a + b) / 2;
+    }
+    if (rt9[m - 1] > t9) m = m - 1;
+    const double t90 = rt9[m - 1], t91 = rt9[m];
+    return {m - 1, (t9 - t90) / (t91 - t90)};
+}
+
+__device__ __forceinline__ double pf_value(const DevPhys &P, int s, PfPos q) {
+    const double *p = P.pf_tab + (size_t) s * P.pf_n;
+    return p[q.i0] + (p[q.i0 + 1] - p[q.i0]) * q.f;
+}
+
+// Gfac of a REACLIB reverse rate by chapter (reaclib.F90:512-557); 1 for the others
+__device__ inline double reverse_gfac(const DevPhys &P, int nre, int l, PfPos q) {
+    const int ch = P.rev_ch[l];
+    double g[6];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int sp = P.rev_part[e * nre + l];
+        g[e] = sp >= 0 ? pf_value(P, sp, q) : 1.0;
+    }
+    switch (ch) {
+    case 1: return g[1] / g[0];
+    case 2: return g[1] * g[2] / g[0];
+    case 3: return g[1] * g[2] * g[3] / g[0];
+    case 4: return g[2] / (g[0] * g[1]);
+    case 5: return g[2] * g[3] / (g[0] * g[1]);
+    case 6: return g[2] * g[3] * g[4] / (g[0] * g[1]);
+    case 7: return g[2] * g[3] * g[4] * g[5] / (g[0] * g[1]);
+    case 8: return g[3] / (g[0] * g[1] * g[2]);
+    case 9: return g[3] * g[4] / (g[0] * g[1] * g[2]);
+    case 10: return g[4] * g[5] / (g[0] * g[1] * g[2] * g[3]);
+    case 11: return g[1] * g[2] * g[3] * g[4] * g[5] / g[0];
+    default: return 1.0;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -194,6 +266,7 @@ struct DevNet {
     const double *rcache;
     const double *rc_t9;
     double rc_dt9;
+    DevPhys P;
 };
 
 // num_t9_preproc (physics/source/reaclib.F90:155): the cache's grid, T9 = 0.01 .. 10

@@ -104,17 +104,199 @@ __device__ __forceinline__ double ypsplus_value(double t) {
 }
 
 
-// rates of one state into v (one wavefront): the REACLIB formula, or the T9 cache when it is on
-__device__ inline void zone_rates(const DevNet &N, double t9, double rho, double *v) {
+// physical constants, utils/source/constants.F90:11-45 (the compile-time values)
+namespace phys {
+constexpr double kEv2Erg = 1.602177e-12;
+constexpr double kMev2Erg = kEv2Erg * 1.0e6;
+constexpr double kErg2Mev = 1.0 / kMev2Erg;
+constexpr double kPi = 3.141592653589793;
+constexpr double kAvogadro = 6.022140857e23;
+constexpr double kBoltzmann = 1.3806485279e-16;
+constexpr double kPlanckMev = 4.135667662e-21;
+constexpr double kPlanckErg = kPlanckMev * kMev2Erg;
+constexpr double kHbarErg = kPlanckErg / 2.0 / kPi;
+constexpr double kAmu = 1.6605402e-24;
+constexpr double kQe = 4.80320425e-10;
+constexpr double kQe2 = kQe * kQe;
+}  // namespace phys
+
+// calculate_reverse_rates + reverse_rate (reverse.F90:107-337; partition functions on, no
+// Coulomb corrections): each pair's reverse rate from its forward rate when that exceeds 1e-97,
+// G = partition function x spin factor (nse_swj.F90:613-625).  Pairs write distinct reverse
+// rates and read forward ones only, so lanes take them in any order.
+__device__ inline void zone_detailed_balance(const DevNet &N, double t9, double rho, double *v) {
+    using namespace phys;
+    const DevPhys &P = N.P;
+    const int nre = N.nre;
+    const PfPos q = pf_pos(P, t9);
+    const double kTerg = kBoltzmann * t9 * 1.0e9;
+    const double kTMeV = kTerg * kErg2Mev;
+    auto G = [&](int s) { return pf_value(P, s, q) * P.g_spin[s]; };
+    for (int d = lane_id(); d < P.ndb; d += WAVE) {
+        const int i = P.db_fwd[d];
+        const double vi = v[i];
+        if (!(vi > 1.0e-97)) continue;
+        const int a1 = N.k[i], na1 = N.k[nre + i], a2 = N.k[2 * nre + i];
+        const int b1 = N.k[4 * nre + i], nb1 = N.k[5 * nre + i], b2 = N.k[6 * nre + i];
+        const int nb2 = N.k[7 * nre + i];
+        const double G_a1 = G(a1), m_a1 = P.mass_g[a1];
+        double G_a2, m_a2, G_b1, m_b1;
+        if (na1 == 1) {
+            G_a2 = G(a2);
+            m_a2 = P.mass_g[a2];
+        } else if (na1 == 2) {
+            G_a2 = G_a1;
+            m_a2 = m_a1;
+        } else {
+            continue;
+        }
+        const double G_b2 = G(b2), m_b2 = P.mass_g[b2];
+        if (nb2 == 1) {
+            if (nb1 == 1) {
+                G_b1 = G(b1);
+                m_b1 = P.mass_g[b1];
+            } else {
+                G_b1 = 0.0;
+                m_b1 = 0.0;
+            }
+        } else if (nb2 == 2) {
+            G_b1 = G_b2;
+            m_b1 = m_b2;
+        } else {
+            continue;
+        }
+        double pfac, mfac, Q, rhofac;
+        if (b1 == N.igamma) {   // (n,g), (p,g), (a,g)
+            pfac = G_a1 * G_a2 / G_b2;
+            mfac = pow(m_a1 * m_a2 / m_b2 * kTerg * 2.0 * kPi / kPlanckErg / kPlanckErg, 1.5);
+            Q = -P.q_mev[a1] - P.q_mev[a2] + P.q_mev[b2];
+            rhofac = 1.0 / rho / kAvogadro;
+        } else {
+            pfac = G_a1 * G_a2 / G_b1 / G_b2;
+            mfac = pow(m_a1 * m_a2 / m_b1 / m_b2, 1.5);
+            Q = -P.q_mev[a1] - P.q_mev[a2] + P.q_mev[b1] + P.q_mev[b2];
+            rhofac = 1.0;
+        }
+        const double x = fmin(-Q / kTMeV, 200.0);
+        const double efac = exp(x);
+        double fac = mfac * efac * rhofac;
+        fac = fac * pfac;
+        fac = fmin(fmax(fac, 1.0e-99), 1.0e99);
+        v[P.db_rev[d]] = fmax(vi * fac, 1.0e-99);
+    }
+    wave_sync();
+}
+
+// Chugunov, DeWitt & Yakovlev (2007) screening as evaluate_all_rates applies it
+// (evaluate_rates.F90:65-121; screening_chugunov.F90:86-235; calculate_azbar utils.F90:504-525):
+// the zone's mean ion weight and charge from x (sequential sums in species order, staged in
+// buf >= nsp doubles), the plasma temperature with its fade below 0.2 T_p, then each reaction's
+// factor from its reactants' charges (an identical pair, k2 = 2, screens k1 with itself)
+__device__ inline void zone_screening(const DevNet &N, double t9, double rho, const double *x,
+                                      double *v, NUPPN_LDS double *buf) {
+    using namespace phys;
+    const int nsp = N.nsp, nre = N.nre;
+    const double third = 1.0 / 3.0;
+    double sums[3];
+    for (int k = 0; k < 3; ++k) {
+        for (int i = lane_id(); i < nsp; i += WAVE) {
+            double t = 0.0;
+            if (i != N.igamma) {
+                const double y = x[i] / N.aw[i];
+                t = k == 0 ? y : (k == 1 ? N.aw[i] * y : N.zw[i] * y);
+            }
+            buf[i] = t;
+        }
+        wave_sync();
+        sums[k] = seq_sum(buf, nsp);
+        wave_sync();
+    }
+    const double abar = sums[1] / sums[0], zbar = sums[2] / sums[0];
+    const double fourov3pi = 4.0 / (3.0 * kPi * kPi);
+    const double fourpiqe2 = 4.0 * kPi * kQe2;
+    const double tkin = t9 * 1.0e9;
+    const double ntot = rho / (abar * kAmu);
+    const double denom = pow(3.0 / (4.0 * kPi * zbar * ntot), third);
+    const double a_av = denom * pow(zbar, third);
+    const double wp = pow(fourpiqe2 * zbar * zbar * ntot / (abar * kAmu), 0.5);
+    const double tp = kHbarErg * wp / kBoltzmann;
+    double tn = tkin / tp, tk;
+    const double t1s = 0.1, t2s = 0.2, t0s = (t1s + t2s) * 0.5;
+    if (tn >= 2.0 * t2s) {
+        tk = tkin;
+    } else if (tn <= 0.5 * t1s) {
+        tk = t1s * tp;
+    } else {
+        const double sg = 1.0 / (1.0 + exp(-(tn - t0s) / 0.01));
+        tn = (1.0 - sg) * t1s + sg * tn;
+        tk = tn * tp;
+    }
+    const double gamfac = kQe2 / (a_av * tk) / kBoltzmann;
+    const double zeta = pow(fourov3pi * tp * tp / tk / tk, third);
+    const double c_a1 = 2.7822, c_a2 = 98.34, c_a3 = sqrt(3.0) - c_a1 / sqrt(c_a2);
+    const double c_b1 = -1.7476, c_b2 = 66.07, c_b3 = 1.12, c_b4 = 65.0, alfa = 0.022;
+    for (int i = lane_id(); i < nre; i += WAVE) {
+        const double vi = v[i];
+        if (vi <= 1.0e-99) continue;
+        const int r1 = N.k[i], r2 = N.k[nre + i] == 2 ? N.k[i] : N.k[2 * nre + i];
+        const double z1 = N.zw[r1], z2 = N.zw[r2];
+        double screen = 1.0;
+        if (z1 > 0.0 && z2 > 0.0) {
+            const double gam = z1 * z2 * gamfac;
+            const double beta = 0.41 - 0.6 / gam;
+            const double gama = 0.06 + 2.2 / gam;
+            double gamtild = gam / pow(1.0 + alfa * zeta + beta * (zeta * zeta) +
+                                       gama * (zeta * zeta * zeta), third);
+            if (gamtild >= 650.0) {
+                gamtild = 600.0;
+            } else if (gamtild > 550.0) {
+                const double sg = 1.0 / (1.0 + exp(-(gamtild - 599.0) / 1.0));
+                gamtild = sg * 600.0 + (1.0 - sg) * gamtild;
+            }
+            const double gamtild2 = gamtild * gamtild;
+            const double t1 = pow(gamtild, 3.0 * 0.5) *
+                              (c_a1 / sqrt(c_a2 + gamtild) + c_a3 / (1.0 + gamtild));
+            const double t2 = c_b1 * gamtild2 / (c_b2 + gamtild);
+            const double t3 = c_b3 * gamtild2 / (c_b4 + gamtild2);
+            screen = fmin(exp(t1 + t2 + t3), 1.0e30);
+        }
+        screen = fmin(screen, 1.0e30);
+        v[i] = vi * screen;
+    }
+    wave_sync();
+}
+
+// evaluate_all_rates of one state into v (one wavefront; evaluate_rates.F90:14-172): the
+// REACLIB formula (or the T9 cache when it is on) with the partition-function factors of the
+// REACLIB reverse rates, then detailed balance and screening at abundances x (buf >= nsp
+// doubles of LDS), then the clip
+__device__ inline void zone_rates(const DevNet &N, double t9, double rho, double *v,
+                                  const double *x, NUPPN_LDS double *buf) {
+    const DevPhys &P = N.P;
     if (N.rcache) {
         const CachePos c = cache_pos(N, t9);
         for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = cached_rate(N, l, N.rtype[l], c, rho);
     } else {
         const T9Pow p = t9_powers(t9);
-        for (int l = lane_id(); l < N.nre; l += WAVE)
-            v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho);
+        if (P.pf_n > 0) {
+            const PfPos q = pf_pos(P, t9);
+            for (int l = lane_id(); l < N.nre; l += WAVE) {
+                const bool rev = P.rev_ch[l] > 0;
+                v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho,
+                                    rev ? reverse_gfac(P, N.nre, l, q) : 1.0, rev);
+            }
+        } else {
+            for (int l = lane_id(); l < N.nre; l += WAVE)
+                v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho);
+        }
     }
     wave_sync();
+    if (P.pf_n > 0 && P.ndb > 0) zone_detailed_balance(N, t9, rho, v);
+    if (P.screen) zone_screening(N, t9, rho, x, v, buf);
+    if (P.ndb > 0 || P.screen) {
+        for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = fmax(v[l], 1.0e-99);
+        wave_sync();
+    }
 }
 
 // -------------------------------------------------------------------------------------------

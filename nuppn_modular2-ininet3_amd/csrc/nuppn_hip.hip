@@ -70,6 +70,16 @@ __global__ void __launch_bounds__(256) rates_kernel(DevNet N, int nz,
             const int l = l0 + r * blockDim.x;
             if (l < nre) vz[l] = cached_rate(N, l, N.rtype[l], c, rho);
         }
+    } else if (N.P.pf_n > 0) {
+        const T9Pow p = t9_powers(t9);
+        const PfPos q = pf_pos(N.P, t9);
+        for (int r = 0; r < RATE_PER_THREAD; ++r) {
+            const int l = l0 + r * blockDim.x;
+            if (l >= nre) break;
+            const bool rev = N.P.rev_ch[l] > 0;
+            vz[l] = reaclib_rate(N.coef, nre, l, N.rtype[l], p, rho,
+                                 rev ? reverse_gfac(N.P, nre, l, q) : 1.0, rev);
+        }
     } else {
         const T9Pow p = t9_powers(t9);
 #pragma unroll
@@ -77,6 +87,24 @@ __global__ void __launch_bounds__(256) rates_kernel(DevNet N, int nz,
             const int l = l0 + r * blockDim.x;
             if (l < nre) vz[l] = reaclib_rate(N.coef, nre, l, N.rtype[l], p, rho);
         }
+    }
+}
+
+// evaluate_all_rates per zone with the rate physics on (partition functions, detailed balance,
+// screening at the zone's abundances yps): one wavefront per zone, LDS >= nsp doubles
+__global__ void __launch_bounds__(64) rates_phys_kernel(DevNet N, int nz,
+                                                        const double *__restrict__ t9_0,
+                                                        const double *__restrict__ t9_1,
+                                                        const double *__restrict__ rho_0,
+                                                        const double *__restrict__ rho_1,
+                                                        const double *__restrict__ yps,
+                                                        size_t nsp_stride, double *v) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    NUPPN_LDS double *buf = (NUPPN_LDS double *) smem;
+    for (int z = blockIdx.x; z < nz; z += gridDim.x) {
+        const double t9 = 0.5 * (t9_0[z] + t9_1[z]);
+        const double rho = 0.5 * (rho_0[z] + rho_1[z]);
+        zone_rates(N, t9, rho, v + (size_t) z * N.nre, yps + (size_t) z * nsp_stride, buf);
     }
 }
 
@@ -88,7 +116,11 @@ __global__ void __launch_bounds__(256) rate_cache_kernel(DevNet N, const double 
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kNT9 || l >= N.nre) return;
     const T9Pow p = t9_powers(t9grid[i]);
-    cache[(size_t) i * N.nre + l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, 1.0);
+    // the preprocessor interpolates the partition functions at every grid point
+    // (reaclib.F90:627): the reverse rates are stored with their factors
+    const bool rev = N.P.pf_n > 0 && N.P.rev_ch[l] > 0;
+    const double gf = rev ? reverse_gfac(N.P, N.nre, l, pf_pos(N.P, t9grid[i])) : 1.0;
+    cache[(size_t) i * N.nre + l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, 1.0, gf, rev);
 }
 
 static dim3 rates_grid(int nre, int nz) {
@@ -288,6 +320,11 @@ struct nuppn_ctx {
     double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
     double *d_jac_lo = nullptr;   // jac: e_coef_lo
     double *d_rcache = nullptr, *d_rc_t9 = nullptr;   // REACLIB T9 cache, grid
+    // rate physics (nuppn_set_rate_physics)
+    int pf_n = 0, ndb = 0, screen = 0;
+    double *d_pf_t9 = nullptr, *d_pf_tab = nullptr, *d_g_spin = nullptr, *d_mass_g = nullptr;
+    double *d_q_mev = nullptr;
+    int *d_rev_ch = nullptr, *d_rev_part = nullptr, *d_db_fwd = nullptr, *d_db_rev = nullptr;
     double rc_dt9 = 0.0;
     int use_cache = 0;
     BLayout blayout{};
@@ -336,6 +373,10 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
                     (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9,
                     (void *) d_b_resid_pos})
+        if (p) (void) hipFree(p);
+    for (void *p : {(void *) d_pf_t9, (void *) d_pf_tab, (void *) d_g_spin, (void *) d_mass_g,
+                    (void *) d_q_mev, (void *) d_rev_ch, (void *) d_rev_part, (void *) d_db_fwd,
+                    (void *) d_db_rev})
         if (p) (void) hipFree(p);
     for (void *p : {(void *) mp_zidx, (void *) mp_ierrb, (void *) mp_small, (void *) mp_yb,
                     (void *) mp_t9b, (void *) mp_rhob, (void *) mp_dtb, (void *) mp_yeb,
@@ -388,7 +429,36 @@ static DevNet dev_net(const nuppn_ctx *c) {
     N.rcache = c->use_cache ? c->d_rcache : nullptr;
     N.rc_t9 = c->d_rc_t9;
     N.rc_dt9 = c->rc_dt9;
+    DevPhys &P = N.P;
+    P.pf_n = c->pf_n;
+    P.pf_t9 = c->d_pf_t9;
+    P.pf_tab = c->d_pf_tab;
+    P.rev_ch = c->d_rev_ch;
+    P.rev_part = c->d_rev_part;
+    P.ndb = c->ndb;
+    P.db_fwd = c->d_db_fwd;
+    P.db_rev = c->d_db_rev;
+    P.g_spin = c->d_g_spin;
+    P.mass_g = c->d_mass_g;
+    P.q_mev = c->d_q_mev;
+    P.screen = c->screen;
     return N;
+}
+
+static bool rate_physics_on(const nuppn_ctx *c) { return c->pf_n > 0 || c->screen; }
+
+// rates of a batch at its cycle mid-points into c->d_v: the REACLIB kernel over (reaction,
+// zone) or, with the rate physics on, one wavefront per zone at the zones' abundances
+static void launch_rates(nuppn_ctx *c, int nz, const double *t9_0, const double *t9_1,
+                         const double *rho_0, const double *rho_1, const double *yps,
+                         hipStream_t s) {
+    if (rate_physics_on(c))
+        hipLaunchKernelGGL(rates_phys_kernel, dim3(std::min(nz, 8192)), dim3(64),
+                           (size_t) c->net.nsp * 8, s, dev_net(c), nz, t9_0, t9_1, rho_0, rho_1,
+                           yps, (size_t) c->net.nsp, c->d_v);
+    else
+        hipLaunchKernelGGL(rates_kernel, rates_grid(c->net.nre, nz), dim3(256), 0, s, dev_net(c),
+                           nz, t9_0, t9_1, rho_0, rho_1, c->d_v);
 }
 
 static DevKnobs dev_knobs(const nuppn_knobs &k) {
@@ -886,6 +956,113 @@ int nuppn_set_rate_cache(nuppn_ctx *c, int use_cache) {
     return NUPPN_OK;
 }
 
+// reverse_init (reverse.F90:25-105): each forward reaction (not a REACLIB reverse rate, not
+// weak, at most one heavy product of a kind) paired with the first reaction of mirrored
+// topology, kept when that one is a REACLIB reverse rate (ground states only here)
+static void reverse_pairs(const HostNetwork &h, const int *rev_ch, std::vector<int> &fwd,
+                          std::vector<int> &rev) {
+    const int nre = h.nre;
+    const int *k = h.k.data();
+    fwd.clear();
+    rev.clear();
+    for (int i = 0; i < nre; ++i) {
+        if (h.rtype[i] == 2 || k[7 * nre + i] > 1 || rev_ch[i] > 0) continue;
+        for (int j = 0; j < nre; ++j) {
+            const int ri[8] = {k[i], k[nre + i], k[2 * nre + i], k[3 * nre + i], k[4 * nre + i],
+                               k[5 * nre + i], k[6 * nre + i], k[7 * nre + i]};
+            const int rj[8] = {k[6 * nre + j], k[7 * nre + j], k[4 * nre + j], k[5 * nre + j],
+                               k[2 * nre + j], k[3 * nre + j], k[j], k[nre + j]};
+            if (std::equal(ri, ri + 8, rj)) {
+                if (rev_ch[j] > 0) {
+                    fwd.push_back(i);
+                    rev.push_back(j);
+                }
+                break;
+            }
+        }
+    }
+}
+
+int nuppn_set_rate_physics(nuppn_ctx *c, const nuppn_rate_physics *rp) {
+    if (!c) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    const int nsp = c->net.nsp, nre = c->net.nre;
+    if (rp && rp->pf_n > 0) {
+        if (rp->pf_n < 2 || !rp->pf_t9 || !rp->pf_tab || !rp->rev_ch || !rp->rev_part ||
+            (rp->detailed_balance && (!rp->g_spin || !rp->mass_g || !rp->q_mev)))
+            return NUPPN_ERR_ARG;
+        for (int i = 1; i < rp->pf_n; ++i)
+            if (!(rp->pf_t9[i] > rp->pf_t9[i - 1])) return NUPPN_ERR_ARG;
+        for (size_t e = 0; e < 6 * (size_t) nre; ++e)
+            if (rp->rev_part[e] < -1 || rp->rev_part[e] >= nsp) return NUPPN_ERR_ARG;
+    }
+    for (void *p : {(void *) c->d_pf_t9, (void *) c->d_pf_tab, (void *) c->d_g_spin,
+                    (void *) c->d_mass_g, (void *) c->d_q_mev, (void *) c->d_rev_ch,
+                    (void *) c->d_rev_part, (void *) c->d_db_fwd, (void *) c->d_db_rev})
+        dev_free(p);
+    c->d_pf_t9 = c->d_pf_tab = c->d_g_spin = c->d_mass_g = c->d_q_mev = nullptr;
+    c->d_rev_ch = c->d_rev_part = c->d_db_fwd = c->d_db_rev = nullptr;
+    c->pf_n = c->ndb = c->screen = 0;
+    if (rp) {
+        c->screen = rp->screen ? 1 : 0;
+        if (rp->pf_n > 0) {
+            if (dev_upload(&c->d_pf_t9, rp->pf_t9, rp->pf_n) ||
+                dev_upload(&c->d_pf_tab, rp->pf_tab, (size_t) nsp * rp->pf_n) ||
+                dev_upload(&c->d_rev_ch, rp->rev_ch, nre) ||
+                dev_upload(&c->d_rev_part, rp->rev_part, 6 * (size_t) nre))
+                return NUPPN_ERR_HIP;
+            c->pf_n = rp->pf_n;
+            if (rp->detailed_balance) {
+                std::vector<int> fwd, rev;
+                reverse_pairs(c->net, rp->rev_ch, fwd, rev);
+                if (!fwd.empty() &&
+                    (dev_upload(&c->d_db_fwd, fwd) || dev_upload(&c->d_db_rev, rev) ||
+                     dev_upload(&c->d_g_spin, rp->g_spin, nsp) ||
+                     dev_upload(&c->d_mass_g, rp->mass_g, nsp) ||
+                     dev_upload(&c->d_q_mev, rp->q_mev, nsp)))
+                    return NUPPN_ERR_HIP;
+                c->ndb = (int) fwd.size();
+            }
+        }
+    }
+    // a T9 cache built before holds the rates without the partition-function factors
+    if (c->d_rcache) {
+        hipLaunchKernelGGL(rate_cache_kernel, dim3((nre + 255) / 256, kNT9), dim3(256), 0, 0,
+                           dev_net(c), c->d_rc_t9, c->d_rcache);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipDeviceSynchronize());
+    }
+    return NUPPN_OK;
+}
+
+int nuppn_reverse_pairs(nuppn_ctx *c, int *irev) {
+    if (!c || !irev) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    const int nre = c->net.nre;
+    for (int i = 0; i < nre; ++i) irev[i] = -1;
+    if (c->ndb == 0) return NUPPN_OK;
+    std::vector<int> fwd(c->ndb), rev(c->ndb);
+    HIP_OK(hipMemcpy(fwd.data(), c->d_db_fwd, sizeof(int) * c->ndb, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(rev.data(), c->d_db_rev, sizeof(int) * c->ndb, hipMemcpyDeviceToHost));
+    for (int d = 0; d < c->ndb; ++d) irev[fwd[d]] = rev[d];
+    return NUPPN_OK;
+}
+
+int nuppn_evaluate_all_rates(nuppn_ctx *c, int nz, const double *t9, const double *rho,
+                             const double *x, double *v) {
+    if (!c || nz <= 0 || !t9 || !rho || !x || !v) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (ensure_zone_buffers(c, nz) || ensure_rate_buffer(c, nz)) return NUPPN_ERR_HIP;
+    const size_t nsp = c->net.nsp, nre = c->net.nre;
+    HIP_OK(hipMemcpy(c->d_t9_0, t9, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_0, rho, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_yps, x, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
+    launch_rates(c, nz, c->d_t9_0, c->d_t9_0, c->d_rho_0, c->d_rho_0, c->d_yps, nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(v, c->d_v, sizeof(double) * nre * (size_t) nz, hipMemcpyDeviceToHost));
+    return NUPPN_OK;
+}
+
 int nuppn_rate_cache_table(nuppn_ctx *c, double *table) {
     if (!c || !table) return NUPPN_ERR_ARG;
     if (!c->d_rcache) return NUPPN_ERR_UNSUPPORTED;
@@ -1019,8 +1196,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     if (ensure_rate_buffer(c, nz) || ensure_zone_state(c, nz)) return NUPPN_ERR_HIP;
     const int nre = c->net.nre;
     HIP_OK(hipEventRecord(c->ev[0], s));
-    hipLaunchKernelGGL(rates_kernel, rates_grid(nre, nz), dim3(256), 0, s, dev_net(c), nz, t9_0,
-                       t9_1, rho_0, rho_1, c->d_v);
+    launch_rates(c, nz, t9_0, t9_1, rho_0, rho_1, yps, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(c->ev[1], s));
     KArgs a = kargs(c);

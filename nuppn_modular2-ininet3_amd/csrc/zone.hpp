@@ -363,7 +363,7 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
             continue;
         }
         if (my_t9 != s.my_t9_last || my_rho != s.my_rho_last) {
-            zone_rates(N, my_t9, my_rho, v);
+            zone_rates(N, my_t9, my_rho, v, yps, sb);
             s.my_rho_last = my_rho;
             s.my_t9_last = my_t9;
         }

@@ -69,6 +69,15 @@ module nuppn_hip
      integer(c_int) :: code_source, path
   end type
 
+  ! rate physics beyond the bare REACLIB sum (nuppn_set_rate_physics)
+  type, bind(c), public :: nuppn_rate_physics
+     integer(c_int) :: pf_n
+     type(c_ptr)    :: pf_t9, pf_tab, rev_ch, rev_part
+     integer(c_int) :: detailed_balance
+     type(c_ptr)    :: g_spin, mass_g, q_mev
+     integer(c_int) :: screen
+  end type
+
   public :: nuppn_default_knobs, nuppn_ctx_create, nuppn_ctx_destroy, nuppn_set_column_order, &
        nuppn_get_column_order, nuppn_column_order, nuppn_reduced_size, nuppn_reduced_pattern, &
        nuppn_rates_eval, nuppn_set_rate_cache, nuppn_rate_cache_table, nuppn_dxdt, &
@@ -76,7 +85,8 @@ module nuppn_hip
        nuppn_integrate_batch, nuppn_integrate_batch_dev, nuppn_default_batch_opts, &
        nuppn_batch_block_info, nuppn_integrate_batch_mixed, nuppn_integrate_batch_mixed_dev, &
        nuppn_last_timing, slu_solve, slu_solve_noalloc, nuppn_default_mppnp_opts, &
-       nuppn_mppnp_cycle, nuppn_mppnp_cycle_dev
+       nuppn_mppnp_cycle, nuppn_mppnp_cycle_dev, nuppn_set_rate_physics, nuppn_reverse_pairs, &
+       nuppn_evaluate_all_rates
 
   interface
      subroutine nuppn_default_knobs(k) bind(c, name="nuppn_default_knobs")
@@ -247,6 +257,28 @@ module nuppn_hip
        integer(c_int), value :: nz
        type(c_ptr), value    :: yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, opts, &
                                 stream
+     end function
+
+     integer(c_int) function nuppn_set_rate_physics(ctx, rp) bind(c, name="nuppn_set_rate_physics")
+       import :: c_ptr, c_int, nuppn_rate_physics
+       type(c_ptr), value                   :: ctx
+       type(nuppn_rate_physics), intent(in) :: rp
+     end function
+
+     integer(c_int) function nuppn_reverse_pairs(ctx, irev) bind(c, name="nuppn_reverse_pairs")
+       import :: c_ptr, c_int
+       type(c_ptr), value          :: ctx
+       integer(c_int), intent(out) :: irev(*)
+     end function
+
+     ! evaluate_all_rates (physics/source/evaluate_rates.F90:14-172) at abundances x(nsp, nz)
+     integer(c_int) function nuppn_evaluate_all_rates(ctx, nz, t9, rho, x, v) &
+          bind(c, name="nuppn_evaluate_all_rates")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value          :: ctx
+       integer(c_int), value       :: nz
+       real(c_double), intent(in)  :: t9(*), rho(*), x(*)
+       real(c_double), intent(out) :: v(*)
      end function
 
      subroutine nuppn_default_mppnp_opts(o) bind(c, name="nuppn_default_mppnp_opts")

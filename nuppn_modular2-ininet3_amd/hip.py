@@ -59,6 +59,14 @@ class MppnpOpts(C.Structure):
                 ("code_source", C.c_int), ("path", C.c_int)]
 
 
+class RatePhysicsC(C.Structure):
+    """nuppn_rate_physics"""
+    _fields_ = [("pf_n", C.c_int), ("pf_t9", C.c_void_p), ("pf_tab", C.c_void_p),
+                ("rev_ch", C.c_void_p), ("rev_part", C.c_void_p), ("detailed_balance", C.c_int),
+                ("g_spin", C.c_void_p), ("mass_g", C.c_void_p), ("q_mev", C.c_void_p),
+                ("screen", C.c_int)]
+
+
 class Network(C.Structure):
     _fields_ = [("nsp", C.c_int), ("nre", C.c_int), ("k", C.POINTER(C.c_int)),
                 ("rtype", C.POINTER(C.c_int)), ("coef", C.POINTER(C.c_double)),
@@ -111,6 +119,9 @@ def load():
     L.nuppn_integrate_batch_dev.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 10
     L.nuppn_last_timing.argtypes = [C.c_void_p, _dp, C.c_int]
     L.nuppn_default_mppnp_opts.argtypes = [C.POINTER(MppnpOpts)]
+    L.nuppn_set_rate_physics.argtypes = [C.c_void_p, C.c_void_p]
+    L.nuppn_reverse_pairs.argtypes = [C.c_void_p, _ip]
+    L.nuppn_evaluate_all_rates.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.nuppn_mppnp_cycle.argtypes = ([C.c_void_p, C.c_int] + [_dp] * 7 +
                                     [C.c_double, C.POINTER(MppnpOpts), C.POINTER(BatchOpts), _ip,
                                      C.c_void_p])
@@ -240,6 +251,38 @@ class Context:
         rc = load().nuppn_rates_eval(self.h, t9.size, t9, rho, None, v)
         if rc != OK:
             raise RuntimeError(f"nuppn_rates_eval failed ({rc})")
+        return v.reshape(t9.size, self.net.nre)
+
+    def set_rate_physics(self, rp):
+        """rp: network.RatePhysics, or None for the bare REACLIB sum"""
+        if rp is None:
+            rc = load().nuppn_set_rate_physics(self.h, None)
+        else:
+            keep = [_f64(rp.pf_t9), _f64(rp.pf_tab.reshape(-1)), _i32(rp.rev_ch),
+                    _i32(rp.rev_part.reshape(-1)), _f64(rp.g_spin), _f64(rp.mass_g),
+                    _f64(rp.q_mev)]
+            c = RatePhysicsC(rp.pf_t9.size, *(a.ctypes.data for a in keep[:4]),
+                             int(rp.detailed_balance), *(a.ctypes.data for a in keep[4:]),
+                             int(rp.screen))
+            rc = load().nuppn_set_rate_physics(self.h, C.byref(c))
+        if rc != OK:
+            raise RuntimeError(f"nuppn_set_rate_physics failed ({rc})")
+
+    def reverse_pairs(self):
+        out = np.zeros(self.net.nre, np.int32)
+        rc = load().nuppn_reverse_pairs(self.h, out)
+        if rc != OK:
+            raise RuntimeError(f"nuppn_reverse_pairs failed ({rc})")
+        return out
+
+    def evaluate_all_rates(self, t9, rho, x):
+        t9 = np.atleast_1d(_f64(t9))
+        rho = np.atleast_1d(_f64(rho))
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        v = np.zeros(t9.size * self.net.nre)
+        rc = load().nuppn_evaluate_all_rates(self.h, t9.size, t9, rho, x, v)
+        if rc != OK:
+            raise RuntimeError(f"nuppn_evaluate_all_rates failed ({rc})")
         return v.reshape(t9.size, self.net.nre)
 
     def dxdt(self, x, v):

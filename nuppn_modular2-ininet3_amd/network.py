@@ -302,3 +302,74 @@ def stellar_model(m: int, seed: int = SEED_BASE + 7, t9_center: float = 2.0,
     d_eul[shell] = 10.0 ** rng.uniform(12.0, 13.0, int(shell.sum()))
     d_eul[q > 0.9] = 10.0 ** rng.uniform(15.0, 16.0, int((q > 0.9).sum()))
     return StellarModel(xm, dq, rad, rho, t9, d_eul)
+
+
+# REACLIB's partition-function temperature grid (the 24 points of its winvn table)
+PF_T9 = np.array([0.1, 0.15, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.5, 2.0, 2.5, 3.0,
+                  3.5, 4.0, 4.5, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0])
+
+
+@dataclasses.dataclass
+class RatePhysics:
+    """Nuclear data for the rate physics beyond the bare REACLIB sum (physics/source/
+    evaluate_rates.F90:14-172): partition functions on REACLIB's T9 grid, the REACLIB reverse
+    ('v') rates with their chapter and participants (reaclib.F90:512-557), spin factors, nuclear
+    masses and binding energies for detailed balance (reverse.F90), and the switches (Chugunov
+    screening and detailed balance are the reference's defaults, physics_knobs.F90:110-111)."""
+
+    pf_t9: np.ndarray        # [24]
+    pf_tab: np.ndarray       # [nsp, 24]
+    rev_ch: np.ndarray       # int32 [nre]: chapter of a REACLIB reverse rate, 0 otherwise
+    rev_part: np.ndarray     # int32 [6, nre]: participants e1..e6 (species, -1 none)
+    g_spin: np.ndarray       # [nsp] 2J + 1
+    mass_g: np.ndarray       # [nsp] nuclear masses (g)
+    q_mev: np.ndarray        # [nsp] binding energies (MeV)
+    screen: int = 1
+    detailed_balance: int = 1
+
+
+def rate_physics(net: Network, seed: int = SEED_BASE + 11, screen: int = 1,
+                 detailed_balance: int = 1) -> RatePhysics:
+    """Synthetic nuclear data for ``net``: partition functions 1 + c (T9 / 10)^q rising with
+    temperature (c log-uniform up to A / 5, flat 1 for gamma, n, p, alpha); ground-state spins
+    by the parity of Z and N; binding energies 8 MeV per nucleon +- 1 MeV (alpha's measured
+    28.296 MeV, 0 for n and p); masses Z m_p + N m_n - B / c^2.  The photodisintegrations
+    are REACLIB reverse rates of chapter 2 (target -> ejectile + residual)."""
+    rng = np.random.default_rng([seed, net.nsp])
+    nsp, nre = net.nsp, net.nre
+    z = net.zn[:, 0].astype(np.float64)
+    n = net.zn[:, 1].astype(np.float64)
+    a = z + n
+    pf = np.ones((nsp, PF_T9.size))
+    heavy = a > 4
+    cfac = np.exp(rng.uniform(math.log(1e-3), 0.0, nsp)) * np.maximum(a, 1.0) / 5.0
+    qexp = rng.uniform(1.5, 4.0, nsp)
+    pf[heavy] = 1.0 + cfac[heavy, None] * (PF_T9[None, :] / 10.0) ** qexp[heavy, None]
+    g = np.ones(nsp)
+    zi, ni = net.zn[:, 0], net.zn[:, 1]
+    odd_a = ((zi + ni) % 2) == 1
+    odd_odd = ((zi % 2) == 1) & ((ni % 2) == 1)
+    g[odd_a] = 2.0 * rng.integers(1, 6, nsp)[odd_a]
+    g[odd_odd] = 2.0 * rng.integers(0, 5, nsp)[odd_odd] + 1.0
+    g[0] = 1.0
+    g[(a == 1)] = 2.0       # n and p: J = 1/2
+    # binding energies of about 8 MeV per nucleon with a +-1 MeV scatter: n, p and alpha
+    # captures release a few MeV (separation energies 6..10 MeV, alpha Q ~ 4 MeV)
+    b = 8.0 * a + rng.uniform(-1.0, 1.0, nsp)
+    b[a <= 1] = 0.0
+    b[(zi == 2) & (ni == 2)] = 28.2957
+    b[0] = 0.0
+    mev2erg = 1.602177e-12 * 1.0e6
+    c2 = 2.99792458e10 ** 2
+    mass = (z * 938.27231 + n * 939.56563 - b) * mev2erg / c2
+    mass[0] = 0.0
+    rev_ch = np.zeros(nre, np.int32)
+    rev_part = np.full((6, nre), -1, np.int32)
+    photo = net.rtype == RTYPE_PHOTO
+    rev_ch[photo] = 2
+    k = net.k
+    rev_part[0, photo] = k[0, photo]
+    # the ejectile and residual (a folded identical pair keeps both on k7)
+    rev_part[1, photo] = np.where(k[4, photo] == GAMMA, k[6, photo], k[4, photo])
+    rev_part[2, photo] = k[6, photo]
+    return RatePhysics(PF_T9.copy(), pf, rev_ch, rev_part, g, mass, b, screen, detailed_balance)

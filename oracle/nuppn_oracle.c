@@ -54,6 +54,15 @@ struct orc_ctx {
     int use_cache;
     double *rcache;         /* [ORC_NT9][nre] */
     double rc_t9[ORC_NT9], rc_dt9;
+    /* rate physics beyond the bare REACLIB sum (orc_set_rate_physics): partition functions on
+     * REACLIB's T9 grid and the factors of its reverse rates, Chugunov screening, detailed
+     * balance; all off by default */
+    int pf_n;
+    double *pf_t9, *pf_tab;  /* [pf_n], [nsp][pf_n] */
+    int *rev_ch, *rev_part;  /* [nre], [6*nre] */
+    int screen, detailed_balance;
+    int *irev;               /* [nre] reverse_init's pairing, -1 */
+    double *g_spin, *mass_g, *q_mev;   /* [nsp] */
 };
 
 /* ------------------------------------------------------------------------------------------ */
@@ -339,6 +348,10 @@ void orc_bd_set_frst(orc_ctx *c, int frst) { c->frst = frst; }
 
 void orc_destroy(orc_ctx *c)
 {
+    if (c) {
+        free(c->pf_t9); free(c->pf_tab); free(c->rev_ch); free(c->rev_part); free(c->irev);
+        free(c->g_spin); free(c->mass_g); free(c->q_mev);
+    }
     if (!c) return;
     free(c->k); free(c->rtype); free(c->coef); free(c->aw); free(c->zw); free(c->consider);
     free(c->rowptr); free(c->colind); free(c->row_of); free(c->map); free(c->diag);
@@ -398,8 +411,58 @@ void orc_set_rate_cache(orc_ctx *c, int use_cache)
 
 const double *orc_rate_cache(const orc_ctx *c) { return c->rcache; }
 
+/* reaclib_interpolate_partition_functions (reaclib.F90:586-605) with bsearch_r8
+ * (utils.F90:687-722): the temperature clipped to the grid, the interval by bisection, linear
+ * interpolation of every species' table */
+static void partition_functions(const orc_ctx *c, double t9_in, double *pi)
+{
+    const int n = c->pf_n;
+    const double *rt9 = c->pf_t9;
+    const double t9 = fmax(rt9[0], fmin(t9_in, rt9[n - 1]));
+    /* bsearch_r8, 1-based a, b, m */
+    int a = 1, b = n, r = b - a, m = (a + b) / 2;
+    while (t9 != rt9[m - 1] && r > 1) {
+        if (rt9[m - 1] > t9) b = m;
+        else a = m;
+        r = b - a;
+        m = (a + b) / 2;
+    }
+    if (rt9[m - 1] > t9) m = m - 1;
+    const int i0 = m - 1, i1 = m;   /* 0-based i0 = m - 1, i1 = i0 + 1 */
+    const double t90 = rt9[i0], t91 = rt9[i1];
+    const double dt9 = (t9 - t90) / (t91 - t90);
+    for (int s = 0; s < c->nsp; ++s) {
+        const double *p = c->pf_tab + (size_t) s * n;
+        pi[s] = p[i0] + (p[i1] - p[i0]) * dt9;
+    }
+}
+
+/* Gfac of a REACLIB reverse ('v') rate by chapter, reaclib.F90:512-557 */
+static double reverse_gfac(int chapter, const double *g)
+{
+    switch (chapter) {
+    case 1: return g[1] / g[0];
+    case 2: return g[1] * g[2] / g[0];
+    case 3: return g[1] * g[2] * g[3] / g[0];
+    case 4: return g[2] / (g[0] * g[1]);
+    case 5: return g[2] * g[3] / (g[0] * g[1]);
+    case 6: return g[2] * g[3] * g[4] / (g[0] * g[1]);
+    case 7: return g[2] * g[3] * g[4] * g[5] / (g[0] * g[1]);
+    case 8: return g[3] / (g[0] * g[1] * g[2]);
+    case 9: return g[3] * g[4] / (g[0] * g[1] * g[2]);
+    case 10: return g[4] * g[5] / (g[0] * g[1] * g[2] * g[3]);
+    case 11: return g[1] * g[2] * g[3] * g[4] * g[5] / g[0];
+    default: return 1.0;
+    }
+}
+
 static void rates_direct(const orc_ctx *c, double t9in, double rho, double *v)
 {
+    double *pinter = NULL;
+    if (c->pf_n > 0) {
+        pinter = malloc(sizeof(double) * c->nsp);
+        partition_functions(c, t9in, pinter);   /* reaclib_wrapper, reaclib.F90:177-190 */
+    }
     const double t9 = fmax(t9in, 0.01);          /* t9_lowlimit_reaclib, reaclib.F90:443 */
     const double t9inv = 1.0 / t9;
     const double t9pow13 = pow(t9, 1.0 / 3.0);
@@ -417,11 +480,252 @@ static void rates_direct(const orc_ctx *c, double t9in, double rho, double *v)
         r0 = r0 + a[5 * nre + l] * t9pow53;
         r0 = r0 + a[6 * nre + l] * lnt9;
         double reacv = 0.0 + exp(r0);
+        if (pinter && c->rev_ch[l] > 0) {
+            double g[6];
+            for (int e = 0; e < 6; ++e) {
+                const int sp = c->rev_part[e * nre + l];
+                g[e] = sp >= 0 ? pinter[sp] : 1.0;
+            }
+            reacv = reacv * reverse_gfac(c->rev_ch[l], g);
+        }
         double rv = (c->rtype[l] == 0) ? reacv * rho : reacv;
         rv = fmax(rv, 1.0e-99);                     /* reaclib.F90:581 */
         rv = fmax(rv * 1.0, 1.0e-99);               /* rfac, evaluate_rates.F90:46 */
         v[l] = fmax(rv, 1.0e-99);                   /* evaluate_rates.F90:172 */
     }
+    free(pinter);
+}
+
+/* physical constants, utils/source/constants.F90:11-45 (compile-time values) */
+#define C_EV2ERG 1.602177e-12
+#define C_MEV2ERG (C_EV2ERG * 1.0e6)
+#define C_ERG2MEV (1.0 / C_MEV2ERG)
+#define C_PI 3.141592653589793
+#define C_AVOGADRO 6.022140857e23
+#define C_BOLTZMANN 1.3806485279e-16
+#define C_PLANCK_MEV 4.135667662e-21
+#define C_PLANCK_ERG (C_PLANCK_MEV * C_MEV2ERG)
+#define C_HBAR_ERG (C_PLANCK_ERG / 2.0 / C_PI)
+#define C_AMU 1.6605402e-24
+#define C_QE 4.80320425e-10
+#define C_QE2 (C_QE * C_QE)
+
+/* calculate_reverse_rates + reverse_rate (reverse.F90:107-337, use_partition_functions true,
+ * no Coulomb corrections): every forward reaction i paired by reverse_init whose rate exceeds
+ * 1e-97 sets its reverse partner's rate v(irev(i)) = max(v(i) fac, 1e-99), with G the
+ * partition functions times the spin factors (nse_swj.F90:613-625) */
+static void detailed_balance(const orc_ctx *c, double rho, double t9, double *v)
+{
+    const int nre = c->nre;
+    double *G = malloc(sizeof(double) * c->nsp);
+    partition_functions(c, t9, G);
+    for (int s = 0; s < c->nsp; ++s) G[s] = G[s] * c->g_spin[s];
+    const double kTerg = C_BOLTZMANN * t9 * 1.0e9;
+    const double kTMeV = kTerg * C_ERG2MEV;
+    for (int i = 0; i < nre; ++i) {
+        if (c->irev[i] < 0 || !(v[i] > 1.0e-97)) continue;
+        const int *k = c->k;
+        const int a1 = k[0 * nre + i], na1 = k[1 * nre + i], a2 = k[2 * nre + i];
+        const int b1 = k[4 * nre + i], nb1 = k[5 * nre + i], b2 = k[6 * nre + i];
+        const int nb2 = k[7 * nre + i];
+        double G_a1 = G[a1], m_a1 = c->mass_g[a1], G_a2, m_a2, G_b1, m_b1, G_b2, m_b2;
+        if (na1 == 1) { G_a2 = G[a2]; m_a2 = c->mass_g[a2]; }
+        else if (na1 == 2) { G_a2 = G_a1; m_a2 = m_a1; }
+        else continue;
+        G_b2 = G[b2];
+        m_b2 = c->mass_g[b2];
+        if (nb2 == 1) {
+            if (nb1 == 1) { G_b1 = G[b1]; m_b1 = c->mass_g[b1]; }
+            else { G_b1 = 0.0; m_b1 = 0.0; }
+        } else if (nb2 == 2) {
+            G_b1 = G_b2; m_b1 = m_b2;
+        } else {
+            continue;
+        }
+        double pfac, mfac, Q, rhofac;
+        if (b1 == c->igamma) {   /* (n,g), (p,g), (a,g): one product */
+            pfac = G_a1 * G_a2 / G_b2;
+            mfac = pow(m_a1 * m_a2 / m_b2 * kTerg * 2.0 * C_PI / C_PLANCK_ERG / C_PLANCK_ERG, 1.5);
+            Q = -c->q_mev[a1] - c->q_mev[a2] + c->q_mev[b2];
+            rhofac = 1.0 / rho / C_AVOGADRO;
+        } else {
+            pfac = G_a1 * G_a2 / G_b1 / G_b2;
+            mfac = pow(m_a1 * m_a2 / m_b1 / m_b2, 1.5);
+            Q = -c->q_mev[a1] - c->q_mev[a2] + c->q_mev[b1] + c->q_mev[b2];
+            rhofac = 1.0;
+        }
+        const double x = fmin(-Q / kTMeV, 200.0);
+        const double efac = exp(x);
+        double fac = mfac * efac * rhofac;
+        fac = fac * pfac;
+        fac = fmin(fmax(fac, 1.0e-99), 1.0e99);
+        v[c->irev[i]] = fmax(v[i] * fac, 1.0e-99);
+    }
+    free(G);
+}
+
+/* Chugunov, DeWitt & Yakovlev screening (screening_chugunov.F90): the plasma quantities of the
+ * zone (screen_precalc_chugunov, 86-152, with calculate_azbar utils.F90:504-525), then every
+ * reaction's factor (calculate_screening_chugunov, 159-235) from the charges of its reactants,
+ * as evaluate_all_rates loops (evaluate_rates.F90:65-121; the networks here have none of the
+ * reference's fixed-index special cases 43 and 109) */
+static void screening(const orc_ctx *c, double t9, double rho, const double *x, double *v)
+{
+    const double third = 1.0 / 3.0;
+    const double fourov3pi = 4.0 / (3.0 * C_PI * C_PI);
+    const double fourpiqe2 = 4.0 * C_PI * C_QE2;
+    const double tkin = t9 * 1.0e9;
+    double ytot = 0.0, sa = 0.0, sz = 0.0;
+    for (int s = 0; s < c->nsp; ++s) {
+        if (!c->consider[s]) continue;
+        const double y = x[s] / c->aw[s];
+        ytot = ytot + y;
+    }
+    for (int s = 0; s < c->nsp; ++s) {
+        if (!c->consider[s]) continue;
+        const double y = x[s] / c->aw[s];
+        sa = sa + c->aw[s] * y;
+    }
+    for (int s = 0; s < c->nsp; ++s) {
+        if (!c->consider[s]) continue;
+        const double y = x[s] / c->aw[s];
+        sz = sz + c->zw[s] * y;
+    }
+    const double abar = sa / ytot, zbar = sz / ytot;
+    const double ntot = rho / (abar * C_AMU);
+    const double denom = pow(3.0 / (4.0 * C_PI * zbar * ntot), third);
+    const double a_av = denom * pow(zbar, third);
+    const double wp = pow(fourpiqe2 * zbar * zbar * ntot / (abar * C_AMU), 0.5);
+    const double tp = C_HBAR_ERG * wp / C_BOLTZMANN;
+    double tn = tkin / tp, tk;
+    const double t1s = 0.1, t2s = 0.2, t0s = (t1s + t2s) * 0.5;
+    if (tn >= 2.0 * t2s) {
+        tk = tkin;
+    } else if (tn <= 0.5 * t1s) {
+        tk = t1s * tp;
+    } else {
+        const double sg = 1.0 / (1.0 + exp(-(tn - t0s) / 0.01));
+        tn = (1.0 - sg) * t1s + sg * tn;
+        tk = tn * tp;
+    }
+    const double gamfac = C_QE2 / (a_av * tk) / C_BOLTZMANN;
+    const double zeta = pow(fourov3pi * tp * tp / tk / tk, third);
+    const double c_a1 = 2.7822, c_a2 = 98.34, c_a3 = sqrt(3.0) - c_a1 / sqrt(c_a2);
+    const double c_b1 = -1.7476, c_b2 = 66.07, c_b3 = 1.12, c_b4 = 65.0, alfa = 0.022;
+    const int nre = c->nre;
+    for (int i = 0; i < nre; ++i) {
+        if (v[i] <= 1.0e-99) continue;
+        const int r1 = c->k[i], r2 = c->k[1 * nre + i] == 2 ? c->k[i] : c->k[2 * nre + i];
+        const double z1 = c->zw[r1], z2 = c->zw[r2];
+        double screen = 1.0;
+        if (z1 > 0.0 && z2 > 0.0) {
+            const double z1z2 = z1 * z2;
+            const double gam = z1z2 * gamfac;
+            const double beta = 0.41 - 0.6 / gam;
+            const double gama = 0.06 + 2.2 / gam;
+            double gamtild = gam / pow(1.0 + alfa * zeta + beta * (zeta * zeta) +
+                                       gama * (zeta * zeta * zeta), third);
+            if (gamtild >= 650.0) {
+                gamtild = 600.0;
+            } else if (gamtild <= 550.0) {
+            } else {
+                const double sg = 1.0 / (1.0 + exp(-(gamtild - 599.0) / 1.0));
+                gamtild = sg * 600.0 + (1.0 - sg) * gamtild;
+            }
+            const double gamtild2 = gamtild * gamtild;
+            const double t1 = pow(gamtild, 3.0 * 0.5) *
+                              (c_a1 / sqrt(c_a2 + gamtild) + c_a3 / (1.0 + gamtild));
+            const double t2 = c_b1 * gamtild2 / (c_b2 + gamtild);
+            const double t3 = c_b3 * gamtild2 / (c_b4 + gamtild2);
+            const double h0fit = t1 + t2 + t3;
+            screen = fmin(exp(h0fit), 1.0e30);
+        }
+        screen = fmin(screen, 1.0e30);
+        v[i] = v[i] * screen;
+    }
+}
+
+/* evaluate_all_rates (evaluate_rates.F90:14-172) at the abundances x: the REACLIB rates (with
+ * the partition-function factors of its reverse rates), detailed balance, screening, the clip.
+ * Without rate physics this is orc_rates. */
+void orc_rates_full(const orc_ctx *c, double t9, double rho, double ye, const double *x,
+                    double *v)
+{
+    orc_rates(c, t9, rho, ye, v);
+    if (c->pf_n > 0 && c->detailed_balance && c->irev) detailed_balance(c, rho, t9, v);
+    if (c->screen && x) screening(c, t9, rho, x, v);
+    for (int l = 0; l < c->nre; ++l) v[l] = fmax(v[l], 1.0e-99);
+}
+
+/* reverse_init (reverse.F90:25-105) on this network: forward reactions (not REACLIB reverse
+ * rates, not weak, at most one heavy product of a kind) paired with the first reaction of the
+ * mirrored topology, kept when that one is a REACLIB reverse rate (all species in their ground
+ * state here) */
+static void reverse_pairs(orc_ctx *c)
+{
+    const int nre = c->nre, *k = c->k;
+    for (int i = 0; i < nre; ++i) c->irev[i] = -1;
+    for (int i = 0; i < nre; ++i) {
+        if (c->rtype[i] == 2) continue;              /* weak */
+        if (k[7 * nre + i] > 1) continue;
+        if (c->rev_ch[i] > 0) continue;
+        for (int j = 0; j < nre; ++j) {
+            const int ri[8] = {k[i], k[nre + i], k[2 * nre + i], k[3 * nre + i], k[4 * nre + i],
+                               k[5 * nre + i], k[6 * nre + i], k[7 * nre + i]};
+            const int rj[8] = {k[6 * nre + j], k[7 * nre + j], k[4 * nre + j], k[5 * nre + j],
+                               k[2 * nre + j], k[3 * nre + j], k[j], k[nre + j]};
+            int same = 1;
+            for (int e = 0; e < 8; ++e) same = same && ri[e] == rj[e];
+            if (same) {
+                if (c->rev_ch[j] > 0) c->irev[i] = j;
+                break;
+            }
+        }
+    }
+}
+
+void orc_set_rate_physics(orc_ctx *c, int pf_n, const double *pf_t9, const double *pf_tab,
+                          const int *rev_ch, const int *rev_part, int screen,
+                          int detailed_balance, const double *g_spin, const double *mass_g,
+                          const double *q_mev)
+{
+    const int nsp = c->nsp, nre = c->nre;
+    free(c->pf_t9); free(c->pf_tab); free(c->rev_ch); free(c->rev_part); free(c->irev);
+    free(c->g_spin); free(c->mass_g); free(c->q_mev);
+    c->pf_t9 = c->pf_tab = c->g_spin = c->mass_g = c->q_mev = NULL;
+    c->rev_ch = c->rev_part = c->irev = NULL;
+    c->pf_n = pf_n;
+    c->screen = screen;
+    c->detailed_balance = detailed_balance;
+    if (pf_n > 0) {
+        c->pf_t9 = malloc(sizeof(double) * pf_n);
+        memcpy(c->pf_t9, pf_t9, sizeof(double) * pf_n);
+        c->pf_tab = malloc(sizeof(double) * (size_t) nsp * pf_n);
+        memcpy(c->pf_tab, pf_tab, sizeof(double) * (size_t) nsp * pf_n);
+        c->rev_ch = malloc(sizeof(int) * nre);
+        memcpy(c->rev_ch, rev_ch, sizeof(int) * nre);
+        c->rev_part = malloc(sizeof(int) * 6 * (size_t) nre);
+        memcpy(c->rev_part, rev_part, sizeof(int) * 6 * (size_t) nre);
+        c->g_spin = malloc(sizeof(double) * nsp);
+        memcpy(c->g_spin, g_spin, sizeof(double) * nsp);
+        c->mass_g = malloc(sizeof(double) * nsp);
+        memcpy(c->mass_g, mass_g, sizeof(double) * nsp);
+        c->q_mev = malloc(sizeof(double) * nsp);
+        memcpy(c->q_mev, q_mev, sizeof(double) * nsp);
+        c->irev = malloc(sizeof(int) * nre);
+        reverse_pairs(c);
+    }
+    /* a cache built before holds the rates without the partition-function factors */
+    if (c->rcache) {
+        free(c->rcache);
+        c->rcache = NULL;
+        if (c->use_cache) orc_set_rate_cache(c, 1);
+    }
+}
+
+void orc_reverse_pairs(const orc_ctx *c, int *irev)
+{
+    for (int i = 0; i < c->nre; ++i) irev[i] = c->irev ? c->irev[i] : -1;
 }
 
 /* s += t as an unevaluated double-double sum (hi, lo): TwoSum (Knuth), the error into lo */
@@ -1367,7 +1671,7 @@ int orc_integrate_zone_v(orc_ctx *c, const double *v_mid, double *yps, double t9
     double t9m = 0.5 * (t9_0 + t9_1), rhom = 0.5 * (rho_0 + rho_1);
     *ye = orc_calculate_ye(c, yps);
     if (v_mid) memcpy(c->v, v_mid, sizeof(double) * c->nre);
-    else orc_rates(c, t9m, rhom, *ye, c->v);
+    else orc_rates_full(c, t9m, rhom, *ye, yps, c->v);
 
     /* integrate_network, solver.F90:53-238 */
     double dtdone = 0.0, dtrcmd = 0.0, ye_old = *ye;
@@ -1402,7 +1706,7 @@ int orc_integrate_zone_v(orc_ctx *c, const double *v_mid, double *yps, double t9
             continue;
         }
         if (my_t9 != my_t9_last || my_rho != my_rho_last) {
-            orc_rates(c, my_t9, my_rho, *ye, c->v);
+            orc_rates_full(c, my_t9, my_rho, *ye, yps, c->v);
             my_rho_last = my_rho; my_t9_last = my_t9;
         }
         ierr = 0;

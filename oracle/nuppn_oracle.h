@@ -147,6 +147,21 @@ int    orc_mppnp_cycle(orc_ctx *c, int m, double *yps, const double *t9, const d
                        double dzeit, double t9threshold, int mesa, double sig, int *ierr,
                        orc_stats *stats);
 
+/* Rate physics beyond the bare REACLIB sum (evaluate_rates.F90:14-172): partition functions on
+ * REACLIB's T9 grid (pf_t9 [pf_n], pf_tab [nsp][pf_n]) and the factors of its reverse ('v') rates
+ * (rev_ch [nre] chapter 1..11 or 0, rev_part [6][nre] participants e1..e6 or -1; reaclib.F90:
+ * 512-557, 586-605), Chugunov screening (screen 1; screening_chugunov.F90), detailed balance
+ * (reverse.F90:25-337; spin factors g_spin, nuclear masses mass_g in g, q_mev as ppn_q).
+ * pf_n = 0 switches the partition functions and detailed balance off. */
+void orc_set_rate_physics(orc_ctx *c, int pf_n, const double *pf_t9, const double *pf_tab,
+                          const int *rev_ch, const int *rev_part, int screen,
+                          int detailed_balance, const double *g_spin, const double *mass_g,
+                          const double *q_mev);
+void orc_reverse_pairs(const orc_ctx *c, int *irev);
+/* evaluate_all_rates at abundances x (orc_rates without rate physics) */
+void orc_rates_full(const orc_ctx *c, double t9, double rho, double ye, const double *x,
+                    double *v);
+
 /* Bader-Deuflhard tables, bader_deuflhard.F90:66-124 */
 void orc_bd_tables(int *nseq9, double *a9, double *alf64, int *kopt, int *kmax);
 

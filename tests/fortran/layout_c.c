@@ -33,5 +33,12 @@ int main(void) {
     printf("nuppn_mppnp_opts %d %d %d %d %d\n", (int) sizeof(nuppn_mppnp_opts),
            O(nuppn_mppnp_opts, t9threshold), O(nuppn_mppnp_opts, sig_term_limit),
            O(nuppn_mppnp_opts, code_source), O(nuppn_mppnp_opts, path));
+    printf("nuppn_rate_physics %d %d %d %d %d %d %d %d %d %d %d\n",
+           (int) sizeof(nuppn_rate_physics), O(nuppn_rate_physics, pf_n),
+           O(nuppn_rate_physics, pf_t9), O(nuppn_rate_physics, pf_tab),
+           O(nuppn_rate_physics, rev_ch), O(nuppn_rate_physics, rev_part),
+           O(nuppn_rate_physics, detailed_balance), O(nuppn_rate_physics, g_spin),
+           O(nuppn_rate_physics, mass_g), O(nuppn_rate_physics, q_mev),
+           O(nuppn_rate_physics, screen));
     return 0;
 }

@@ -10,7 +10,8 @@ program layout_f
   type(nuppn_zone_stats), target :: s
   type(nuppn_batch_opts), target :: o
   type(nuppn_mppnp_opts), target :: q
-  type(c_funptr) :: f(27)
+  type(nuppn_rate_physics), target :: rp
+  type(c_funptr) :: f(30)
   integer(c_intptr_t) :: b
 
   b = transfer(c_loc(n), b)
@@ -37,6 +38,11 @@ program layout_f
   b = transfer(c_loc(q), b)
   write(*, '(a, 5(1x, i0))') 'nuppn_mppnp_opts', c_sizeof(q), off(c_loc(q%t9threshold)), &
        off(c_loc(q%sig_term_limit)), off(c_loc(q%code_source)), off(c_loc(q%path))
+  b = transfer(c_loc(rp), b)
+  write(*, '(a, 11(1x, i0))') 'nuppn_rate_physics', c_sizeof(rp), off(c_loc(rp%pf_n)), &
+       off(c_loc(rp%pf_t9)), off(c_loc(rp%pf_tab)), off(c_loc(rp%rev_ch)), &
+       off(c_loc(rp%rev_part)), off(c_loc(rp%detailed_balance)), off(c_loc(rp%g_spin)), &
+       off(c_loc(rp%mass_g)), off(c_loc(rp%q_mev)), off(c_loc(rp%screen))
 
   f(1) = c_funloc(nuppn_default_knobs)
   f(2) = c_funloc(nuppn_ctx_create)
@@ -65,7 +71,10 @@ program layout_f
   f(25) = c_funloc(nuppn_default_mppnp_opts)
   f(26) = c_funloc(nuppn_mppnp_cycle)
   f(27) = c_funloc(nuppn_mppnp_cycle_dev)
-  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 27)])
+  f(28) = c_funloc(nuppn_set_rate_physics)
+  f(29) = c_funloc(nuppn_reverse_pairs)
+  f(30) = c_funloc(nuppn_evaluate_all_rates)
+  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 30)])
 contains
   integer(c_intptr_t) function off(p)
     type(c_ptr), intent(in) :: p

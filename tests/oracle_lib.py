@@ -114,6 +114,10 @@ def lib():
     L.orc_mppnp_cycle.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                   C.c_double, C.c_double, C.c_int, C.c_double, _ip,
                                   C.POINTER(Stats)]
+    L.orc_set_rate_physics.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _ip, _ip, C.c_int,
+                                       C.c_int, _dp, _dp, _dp]
+    L.orc_reverse_pairs.argtypes = [C.c_void_p, _ip]
+    L.orc_rates_full.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _dp]
     L.orc_ref_open.argtypes = [C.c_char_p]
     L.orc_ref_slu_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp,
                                     C.POINTER(C.c_int)]
@@ -190,6 +194,29 @@ class Oracle:
     def rates(self, t9, rho, ye=0.5):
         v = np.zeros(self.net.nre)
         lib().orc_rates(self.h, float(t9), float(rho), float(ye), v)
+        return v
+
+    def set_rate_physics(self, rp):
+        """rp: network.RatePhysics (None switches the rate physics off)"""
+        L = lib()
+        if rp is None:
+            z = np.zeros(1)
+            L.orc_set_rate_physics(self.h, 0, z, z, np.zeros(1, np.int32), np.zeros(1, np.int32),
+                                   0, 0, z, z, z)
+            return
+        L.orc_set_rate_physics(self.h, rp.pf_t9.size, f64(rp.pf_t9), f64(rp.pf_tab.reshape(-1)),
+                               i32(rp.rev_ch), i32(rp.rev_part.reshape(-1)), int(rp.screen),
+                               int(rp.detailed_balance), f64(rp.g_spin), f64(rp.mass_g),
+                               f64(rp.q_mev))
+
+    def reverse_pairs(self):
+        out = np.zeros(self.net.nre, np.int32)
+        lib().orc_reverse_pairs(self.h, out)
+        return out
+
+    def rates_full(self, t9, rho, x, ye=0.5):
+        v = np.zeros(self.net.nre)
+        lib().orc_rates_full(self.h, t9, rho, ye, f64(x), v)
         return v
 
     def dxdt(self, x, v):
