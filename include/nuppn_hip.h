@@ -236,6 +236,27 @@ typedef struct {
 
 void nuppn_default_batch_opts(nuppn_batch_opts *o);
 
+/* Autotuner of the batched path (no counterpart in the reference): times one cycle of the
+ * sample batch (host arrays, as nuppn_integrate_batch_mixed) under each candidate shape - block
+ * size 32 and 64 where the network's band allows it, and the multi-wavefront kernels for short
+ * work lists (team_zones default / 0 / 4 x CUs with two wavefronts) - and keeps the fastest: the
+ * context is rebuilt with that block size and its options become the defaults of later calls
+ * with opts NULL.  cache_path (may be NULL): a text file of earlier choices keyed by network
+ * (nsp, nre, nnz) and CU count; a hit applies it without timing, a new choice is appended.
+ * Every candidate gives the same results bit for bit (res->consistent reports it). */
+typedef struct {
+    int block_size, team_zones, team_waves;   /* the choice */
+    int from_cache;                           /* 1: read from cache_path, nothing timed */
+    int ncand;                                /* candidates timed */
+    int consistent;                           /* 1: all gave the same abundances bit for bit */
+    int cand_nb[8], cand_tz[8], cand_tw[8];
+    double ms[8];                             /* their cycle times */
+} nuppn_tune_result;
+
+int nuppn_autotune(nuppn_ctx *ctx, int nz, const double *yps, const double *t9_0,
+                   const double *t9_1, const double *rho_0, const double *rho_1,
+                   const double *dtwant, const char *cache_path, nuppn_tune_result *res);
+
 /* Block structure of the batched path: info[0..6] = block size NB (0: unavailable), blocks,
  * border species, interior species, interior half-bandwidth, resident slots, LDS bytes per
  * slot.  Returns NUPPN_OK or NUPPN_ERR_UNSUPPORTED. */

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -343,6 +344,9 @@ struct nuppn_ctx {
     double newton_ms = 0.0, control_ms = 0.0;
     long long newton_launches = 0, graph_launches = 0;
     hipGraphNode_t glast = nullptr;     // the last node of a pass graph being built
+    // the autotuner's choice, the defaults of the batched path once it has run
+    int tuned = 0;
+    nuppn_batch_opts tuned_opts{};
     // the mppnp cycle (zone selection, batch gather / scatter, mixing): device buffers for m zones
     int mp_cap = 0;
     int *mp_zidx = nullptr, *mp_ierrb = nullptr, *mp_small = nullptr;   // small: count, bad
@@ -566,10 +570,35 @@ static int kernel_occupancy(const void *kfn, size_t lds) {
     return per_cu;
 }
 
-// block structure + device upload + occupancy of the batched Newton kernels (once per context)
-static int setup_batched(nuppn_ctx *c) {
-    const int nb = choose_block_size(c->net, c->sym);
+// the batched path's device arrays (before a rebuild with another block size)
+static void free_batched(nuppn_ctx *c) {
+    for (void **p : {(void **) &c->d_b_vpos, (void **) &c->d_b_diagk, (void **) &c->d_b_ent_ptr,
+                     (void **) &c->d_b_ent_k, (void **) &c->d_b_ent_loc, (void **) &c->d_b_ent_col,
+                     (void **) &c->d_b_csr_col, (void **) &c->d_b_isdiag, (void **) &c->d_b_mass,
+                     (void **) &c->d_b_ent_ys, (void **) &c->d_b_ent_dst, (void **) &c->d_jac_lo,
+                     (void **) &c->d_b_resid_pos, (void **) &c->d_bslots}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+    for (int *&p : c->d_seg_i) {
+        dev_free(p);
+        p = nullptr;
+    }
+    for (double *&p : c->d_seg_c) {
+        dev_free(p);
+        p = nullptr;
+    }
+    c->bslots = 0;
+    c->bnb = 0;
+}
+
+// block structure + device upload + occupancy of the batched Newton kernels (once per context,
+// again when the autotuner tries another block size): force_nb 0 takes the smallest block size
+// whose blocks hold the interior band
+static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
+    const int nb = force_nb ? force_nb : choose_block_size(c->net, c->sym);
     if (nb == 0) return 0;   // the batched path stays unavailable; the plain path is unaffected
+    c->bsym = BlockSym();
     if (build_block_symbolic(c->net, c->sym, nb, c->bsym) != 0 || c->bsym.nbd > 8) return 0;
     const BlockSym &b = c->bsym;
     if (dev_upload(&c->d_b_vpos, b.vpos) || dev_upload(&c->d_b_diagk, b.diagk) ||
@@ -1582,9 +1611,11 @@ static int host_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, con
     return NUPPN_OK;
 }
 
-// the batched path's options, checked (defaults for null)
-static int mixed_opts(const nuppn_batch_opts *opts, nuppn_batch_opts *o) {
+// the batched path's options, checked (defaults for null: the autotuner's choice once it ran)
+static int mixed_opts(const nuppn_batch_opts *opts, nuppn_batch_opts *o,
+                      const nuppn_ctx *c = nullptr) {
     if (opts) *o = *opts;
+    else if (c && c->tuned) *o = c->tuned_opts;
     else nuppn_default_batch_opts(o);
     if ((o->factor_precision != 32 && o->factor_precision != 64) || o->max_refine < 1 ||
         (o->team_waves != 0 && o->team_waves != 2 && o->team_waves != 4) ||
@@ -1628,7 +1659,7 @@ static int mppnp_checked(nuppn_ctx *c, int m, const nuppn_mppnp_opts *mo_in,
     if ((mo->code_source != 0 && mo->code_source != 1) || (mo->path != 0 && mo->path != 1))
         return NUPPN_ERR_ARG;
     if (mo->path == 1) {
-        if (mixed_opts(bo_in, bo) != NUPPN_OK) return NUPPN_ERR_ARG;
+        if (mixed_opts(bo_in, bo, c) != NUPPN_OK) return NUPPN_ERR_ARG;
         if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     }
     return NUPPN_OK;
@@ -1689,6 +1720,138 @@ int nuppn_mppnp_cycle(nuppn_ctx *c, int m, double *yps, const double *t9, const 
     return rc;
 }
 
+// a line of the autotuner's cache file: the network and device it was tuned for, the choice
+struct TuneKey {
+    int nsp, nre, nnz, ncu;
+    bool operator==(const TuneKey &o) const {
+        return nsp == o.nsp && nre == o.nre && nnz == o.nnz && ncu == o.ncu;
+    }
+};
+
+static bool tune_cache_read(const char *path, const TuneKey &k, int *nb, int *tz, int *tw) {
+    FILE *f = path ? fopen(path, "r") : nullptr;
+    if (!f) return false;
+    TuneKey r;
+    int a, b, d;
+    bool hit = false;
+    while (fscanf(f, "%d %d %d %d %d %d %d", &r.nsp, &r.nre, &r.nnz, &r.ncu, &a, &b, &d) == 7)
+        if (r == k) {
+            *nb = a;
+            *tz = b;
+            *tw = d;
+            hit = true;
+        }
+    fclose(f);
+    return hit;
+}
+
+int nuppn_autotune(nuppn_ctx *c, int nz, const double *yps, const double *t9_0,
+                   const double *t9_1, const double *rho_0, const double *rho_1,
+                   const double *dtwant, const char *cache_path, nuppn_tune_result *res) {
+    if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !dtwant || !res)
+        return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
+    memset(res, 0, sizeof(*res));
+    const TuneKey key{c->net.nsp, c->net.nre, c->sym.nnzA, c->ncu};
+    int nb = 0, tz = 0, tw = 0;
+    auto apply = [&](int nb_, int tz_, int tw_) -> int {
+        if (nb_ != c->bnb) {
+            free_batched(c);
+            if (setup_batched(c, nb_) != 0 || c->bnb != nb_) return NUPPN_ERR_HIP;
+        }
+        nuppn_default_batch_opts(&c->tuned_opts);
+        c->tuned_opts.team_zones = tz_;
+        c->tuned_opts.team_waves = tw_;
+        c->tuned = 1;
+        return NUPPN_OK;
+    };
+    if (tune_cache_read(cache_path, key, &nb, &tz, &tw)) {
+        const int rc = apply(nb, tz, tw);
+        res->block_size = nb;
+        res->team_zones = tz;
+        res->team_waves = tw;
+        res->from_cache = 1;
+        res->consistent = 1;
+        return rc;
+    }
+    // stage the sample once; every candidate integrates a fresh copy of it
+    if (ensure_zone_buffers(c, nz)) return NUPPN_ERR_HIP;
+    const size_t nsp = c->net.nsp;
+    double *d_y0 = nullptr;
+    HIP_OK(hipMalloc((void **) &d_y0, sizeof(double) * nsp * nz));
+    std::unique_ptr<double, void (*)(double *)> guard(d_y0, [](double *p) { (void) hipFree(p); });
+    HIP_OK(hipMemcpy(d_y0, yps, sizeof(double) * nsp * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_0, t9_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_t9_1, t9_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_0, rho_0, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
+    std::vector<double> first(nsp * nz), cur(nsp * nz);
+    const int team_choices[3][2] = {{-1, 0}, {0, 0}, {4 * c->ncu, 2}};
+    const int nb0 = c->bnb;
+    double best = 1e300;
+    res->consistent = 1;
+    for (int nbc : {32, 64}) {
+        if (nbc != c->bnb) {
+            BlockSym probe;
+            if (build_block_symbolic(c->net, c->sym, nbc, probe) != 0) continue;
+            free_batched(c);
+            if (setup_batched(c, nbc) != 0 || c->bnb != nbc) {
+                free_batched(c);
+                if (setup_batched(c, nb0) != 0) return NUPPN_ERR_HIP;
+                continue;
+            }
+        }
+        for (const auto &tc : team_choices) {
+            if (res->ncand >= 8) break;
+            nuppn_batch_opts o;
+            nuppn_default_batch_opts(&o);
+            o.team_zones = tc[0];
+            o.team_waves = tc[1];
+            HIP_OK(hipMemcpy(c->d_yps, d_y0, sizeof(double) * nsp * nz, hipMemcpyDeviceToDevice));
+            HIP_OK(hipDeviceSynchronize());
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = run_batch(c, nz, c->d_yps, c->d_t9_0, c->d_t9_1, c->d_rho_0,
+                                     c->d_rho_1, c->d_ye, c->d_dt, c->d_ierr, nullptr, &o, nullptr);
+            HIP_OK(hipDeviceSynchronize());
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+            if (rc != NUPPN_OK) return rc;
+            HIP_OK(hipMemcpy(cur.data(), c->d_yps, sizeof(double) * nsp * nz,
+                             hipMemcpyDeviceToHost));
+            if (res->ncand == 0) first = cur;
+            else if (memcmp(first.data(), cur.data(), sizeof(double) * nsp * nz) != 0)
+                res->consistent = 0;
+            const int k = res->ncand++;
+            res->ms[k] = ms;
+            res->cand_nb[k] = nbc;
+            res->cand_tz[k] = o.team_zones;
+            res->cand_tw[k] = o.team_waves;
+            if (ms < best) {
+                best = ms;
+                nb = nbc;
+                tz = o.team_zones;
+                tw = o.team_waves;
+            }
+        }
+    }
+    const int rc = apply(nb, tz, tw);
+    if (rc != NUPPN_OK) return rc;
+    res->block_size = nb;
+    res->team_zones = tz;
+    res->team_waves = tw;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%d %d %d %d %d %d %d\n", key.nsp, key.nre, key.nnz, key.ncu, nb, tz, tw);
+            fclose(f);
+        }
+    }
+    return NUPPN_OK;
+}
+
 int nuppn_batch_block_info(nuppn_ctx *c, int *info, int ninfo) {
     if (!c || !info || ninfo <= 0) return NUPPN_ERR_ARG;
     const BlockSym &b = c->bsym;
@@ -1706,7 +1869,7 @@ int nuppn_integrate_batch_mixed_dev(nuppn_ctx *c, int nz, double *yps, const dou
         return NUPPN_ERR_ARG;
     if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     nuppn_batch_opts o;
-    if (mixed_opts(opts, &o)) return NUPPN_ERR_ARG;
+    if (mixed_opts(opts, &o, c)) return NUPPN_ERR_ARG;
     std::lock_guard<std::mutex> g(c->mu);
     return run_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, &o,
                      (hipStream_t) stream);
@@ -1720,7 +1883,7 @@ int nuppn_integrate_batch_mixed(nuppn_ctx *c, int nz, double *yps, const double 
         return NUPPN_ERR_ARG;
     if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     nuppn_batch_opts o;
-    if (mixed_opts(opts, &o)) return NUPPN_ERR_ARG;
+    if (mixed_opts(opts, &o, c)) return NUPPN_ERR_ARG;
     return host_batch(c, nz, yps, t9_0, t9_1, rho_0, rho_1, ye, dtwant, ierr, stats, &o);
 }
 

@@ -78,6 +78,13 @@ module nuppn_hip
      integer(c_int) :: screen
   end type
 
+  ! the autotuner's report (nuppn_autotune)
+  type, bind(c), public :: nuppn_tune_result
+     integer(c_int) :: block_size, team_zones, team_waves, from_cache, ncand, consistent
+     integer(c_int) :: cand_nb(8), cand_tz(8), cand_tw(8)
+     real(c_double) :: ms(8)
+  end type
+
   public :: nuppn_default_knobs, nuppn_ctx_create, nuppn_ctx_destroy, nuppn_set_column_order, &
        nuppn_get_column_order, nuppn_column_order, nuppn_reduced_size, nuppn_reduced_pattern, &
        nuppn_rates_eval, nuppn_set_rate_cache, nuppn_rate_cache_table, nuppn_dxdt, &
@@ -86,7 +93,7 @@ module nuppn_hip
        nuppn_batch_block_info, nuppn_integrate_batch_mixed, nuppn_integrate_batch_mixed_dev, &
        nuppn_last_timing, slu_solve, slu_solve_noalloc, nuppn_default_mppnp_opts, &
        nuppn_mppnp_cycle, nuppn_mppnp_cycle_dev, nuppn_set_rate_physics, nuppn_reverse_pairs, &
-       nuppn_evaluate_all_rates
+       nuppn_evaluate_all_rates, nuppn_autotune
 
   interface
      subroutine nuppn_default_knobs(k) bind(c, name="nuppn_default_knobs")
@@ -279,6 +286,18 @@ module nuppn_hip
        integer(c_int), value       :: nz
        real(c_double), intent(in)  :: t9(*), rho(*), x(*)
        real(c_double), intent(out) :: v(*)
+     end function
+
+     ! batched-path shapes chosen by timing a sample batch (no counterpart in the reference)
+     integer(c_int) function nuppn_autotune(ctx, nz, yps, t9_0, t9_1, rho_0, rho_1, dtwant, &
+                                            cache_path, res) bind(c, name="nuppn_autotune")
+       import :: c_ptr, c_int, c_double, nuppn_tune_result
+       type(c_ptr), value                   :: ctx
+       integer(c_int), value                :: nz
+       real(c_double), intent(in)           :: yps(*), t9_0(*), t9_1(*), rho_0(*), rho_1(*)
+       real(c_double), intent(in)           :: dtwant(*)
+       type(c_ptr), value                   :: cache_path   ! c_loc of a NUL-terminated name
+       type(nuppn_tune_result), intent(out) :: res
      end function
 
      subroutine nuppn_default_mppnp_opts(o) bind(c, name="nuppn_default_mppnp_opts")

@@ -67,6 +67,14 @@ class RatePhysicsC(C.Structure):
                 ("screen", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    """nuppn_tune_result"""
+    _fields_ = [("block_size", C.c_int), ("team_zones", C.c_int), ("team_waves", C.c_int),
+                ("from_cache", C.c_int), ("ncand", C.c_int), ("consistent", C.c_int),
+                ("cand_nb", C.c_int * 8), ("cand_tz", C.c_int * 8), ("cand_tw", C.c_int * 8),
+                ("ms", C.c_double * 8)]
+
+
 class Network(C.Structure):
     _fields_ = [("nsp", C.c_int), ("nre", C.c_int), ("k", C.POINTER(C.c_int)),
                 ("rtype", C.POINTER(C.c_int)), ("coef", C.POINTER(C.c_double)),
@@ -120,6 +128,8 @@ def load():
     L.nuppn_last_timing.argtypes = [C.c_void_p, _dp, C.c_int]
     L.nuppn_default_mppnp_opts.argtypes = [C.POINTER(MppnpOpts)]
     L.nuppn_set_rate_physics.argtypes = [C.c_void_p, C.c_void_p]
+    L.nuppn_autotune.argtypes = ([C.c_void_p, C.c_int] + [_dp] * 6 +
+                                 [C.c_char_p, C.POINTER(TuneResult)])
     L.nuppn_reverse_pairs.argtypes = [C.c_void_p, _ip]
     L.nuppn_evaluate_all_rates.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.nuppn_mppnp_cycle.argtypes = ([C.c_void_p, C.c_int] + [_dp] * 7 +
@@ -252,6 +262,23 @@ class Context:
         if rc != OK:
             raise RuntimeError(f"nuppn_rates_eval failed ({rc})")
         return v.reshape(t9.size, self.net.nre)
+
+    def autotune(self, yps, t9_0, t9_1, rho_0, rho_1, dtwant, cache_path=None):
+        """nuppn_autotune on a sample batch: the chosen shapes and the candidates' times"""
+        r = TuneResult()
+        y = _f64(yps)
+        rc = load().nuppn_autotune(self.h, y.shape[0], y, _f64(t9_0), _f64(t9_1), _f64(rho_0),
+                                   _f64(rho_1), _f64(dtwant),
+                                   None if cache_path is None else str(cache_path).encode(),
+                                   C.byref(r))
+        if rc != OK:
+            raise RuntimeError(f"nuppn_autotune failed ({rc})")
+        n = r.ncand
+        return {"block_size": r.block_size, "team_zones": r.team_zones,
+                "team_waves": r.team_waves, "from_cache": bool(r.from_cache),
+                "consistent": bool(r.consistent),
+                "candidates": [(r.cand_nb[i], r.cand_tz[i], r.cand_tw[i], r.ms[i])
+                               for i in range(n)]}
 
     def set_rate_physics(self, rp):
         """rp: network.RatePhysics, or None for the bare REACLIB sum"""

@@ -40,5 +40,12 @@ int main(void) {
            O(nuppn_rate_physics, detailed_balance), O(nuppn_rate_physics, g_spin),
            O(nuppn_rate_physics, mass_g), O(nuppn_rate_physics, q_mev),
            O(nuppn_rate_physics, screen));
+    printf("nuppn_tune_result %d %d %d %d %d %d %d %d %d %d %d\n",
+           (int) sizeof(nuppn_tune_result), O(nuppn_tune_result, block_size),
+           O(nuppn_tune_result, team_zones), O(nuppn_tune_result, team_waves),
+           O(nuppn_tune_result, from_cache), O(nuppn_tune_result, ncand),
+           O(nuppn_tune_result, consistent), O(nuppn_tune_result, cand_nb),
+           O(nuppn_tune_result, cand_tz), O(nuppn_tune_result, cand_tw),
+           O(nuppn_tune_result, ms));
     return 0;
 }

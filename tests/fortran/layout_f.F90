@@ -11,7 +11,8 @@ program layout_f
   type(nuppn_batch_opts), target :: o
   type(nuppn_mppnp_opts), target :: q
   type(nuppn_rate_physics), target :: rp
-  type(c_funptr) :: f(30)
+  type(nuppn_tune_result), target :: tr
+  type(c_funptr) :: f(31)
   integer(c_intptr_t) :: b
 
   b = transfer(c_loc(n), b)
@@ -43,6 +44,11 @@ program layout_f
        off(c_loc(rp%pf_t9)), off(c_loc(rp%pf_tab)), off(c_loc(rp%rev_ch)), &
        off(c_loc(rp%rev_part)), off(c_loc(rp%detailed_balance)), off(c_loc(rp%g_spin)), &
        off(c_loc(rp%mass_g)), off(c_loc(rp%q_mev)), off(c_loc(rp%screen))
+  b = transfer(c_loc(tr), b)
+  write(*, '(a, 11(1x, i0))') 'nuppn_tune_result', c_sizeof(tr), off(c_loc(tr%block_size)), &
+       off(c_loc(tr%team_zones)), off(c_loc(tr%team_waves)), off(c_loc(tr%from_cache)), &
+       off(c_loc(tr%ncand)), off(c_loc(tr%consistent)), off(c_loc(tr%cand_nb)), &
+       off(c_loc(tr%cand_tz)), off(c_loc(tr%cand_tw)), off(c_loc(tr%ms))
 
   f(1) = c_funloc(nuppn_default_knobs)
   f(2) = c_funloc(nuppn_ctx_create)
@@ -74,7 +80,8 @@ program layout_f
   f(28) = c_funloc(nuppn_set_rate_physics)
   f(29) = c_funloc(nuppn_reverse_pairs)
   f(30) = c_funloc(nuppn_evaluate_all_rates)
-  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 30)])
+  f(31) = c_funloc(nuppn_autotune)
+  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 31)])
 contains
   integer(c_intptr_t) function off(p)
     type(c_ptr), intent(in) :: p

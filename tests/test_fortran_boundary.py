@@ -31,8 +31,8 @@ need_fc = pytest.mark.skipif(not os.path.exists(FC), reason="no Fortran compiler
 @need_fc
 def test_bind_c_types_match_c_structs(tmp_path):
     """c_sizeof and the field offsets of nuppn_network, nuppn_knobs, nuppn_zone_stats and
-    nuppn_batch_opts, nuppn_mppnp_opts, nuppn_rate_physics equal the C sizeof / offsetof; all 30
-    interfaces link."""
+    nuppn_batch_opts, nuppn_mppnp_opts, nuppn_rate_physics, nuppn_tune_result equal the C sizeof /
+    offsetof; all 31 interfaces link."""
     if not os.path.exists(os.path.join(LIB, "libnuppn_hip.so")):
         pytest.skip("libnuppn_hip.so not built")
     d = str(tmp_path)
@@ -50,10 +50,11 @@ def test_bind_c_types_match_c_structs(tmp_path):
     f_types = {ln.split()[0]: ln.split()[1:] for ln in f if ln.strip()}
     c_types = {ln.split()[0]: ln.split()[1:] for ln in c if ln.strip()}
     print(f_types)
-    assert f_types.pop("bound_procedures") == ["30"]
+    assert f_types.pop("bound_procedures") == ["31"]
     assert f_types == c_types
     assert set(c_types) == {"nuppn_network", "nuppn_knobs", "nuppn_zone_stats",
-                            "nuppn_batch_opts", "nuppn_mppnp_opts", "nuppn_rate_physics"}
+                            "nuppn_batch_opts", "nuppn_mppnp_opts", "nuppn_rate_physics",
+                            "nuppn_tune_result"}
 
 
 def test_module_binds_every_declared_entry_point():

@@ -341,3 +341,41 @@ def test_uneven_workload_full_size(pkg, hip):
         assert np.all(s <= 1e-12)
     finally:
         ctx.close()
+
+
+def test_autotune_picks_a_shape_and_caches_it(pkg, hip, tmp_path):
+    """nuppn_autotune times one cycle of a sample under block sizes 32 and 64 and the team-kernel
+    settings, keeps the fastest (every candidate gives the same bits), rebuilds the context with
+    it and uses it for calls without options; a second context reads the choice from the cache
+    file without timing and integrates to the same bits."""
+    w = pkg.network.config(1)
+    sel = np.array([0, 2, 5, 6, 439, 554, 1735, 1090])
+    args = _args(w, sel)
+    cache = tmp_path / "tune.txt"
+    ctx = hip.Context(w.net)
+    try:
+        r = ctx.autotune(w.x0[sel], *args, cache_path=cache)
+        for nb, tz, tw, ms in r["candidates"]:
+            print(f"NB {nb} team_zones {tz} team_waves {tw}: {ms:.1f} ms")
+        print("chosen", r["block_size"], r["team_zones"], r["team_waves"])
+        assert r["consistent"] and not r["from_cache"]
+        assert {c[0] for c in r["candidates"]} == {32, 64}
+        best = min(r["candidates"], key=lambda c: c[3])
+        assert (r["block_size"], r["team_zones"], r["team_waves"]) == best[:3]
+        assert ctx.block_info()["NB"] == r["block_size"]
+        ie, y, ye, st = ctx.integrate_batch_mixed(w.x0[sel], *args)
+        np.testing.assert_array_equal(ie, 0)
+    finally:
+        ctx.close()
+    ctx2 = hip.Context(w.net)
+    try:
+        r2 = ctx2.autotune(w.x0[sel], *args, cache_path=cache)
+        assert r2["from_cache"] and r2["block_size"] == r["block_size"]
+        ie2, y2, ye2, st2 = ctx2.integrate_batch_mixed(w.x0[sel], *args)
+        np.testing.assert_array_equal(y2, y)
+        ctx3 = hip.Context(w.net)   # untuned: block size 32, default options
+        ie3, y3, _, _ = ctx3.integrate_batch_mixed(w.x0[sel], *args)
+        ctx3.close()
+        np.testing.assert_array_equal(y3, y)
+    finally:
+        ctx2.close()
