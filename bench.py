@@ -34,19 +34,21 @@ def _args():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--zones", type=int, default=2048)
-    p.add_argument("--plain-sample", type=int, default=48,
-                   help="zones (stratified in T9 below 0.5) on which the plain FP64 path is timed")
-    p.add_argument("--cpu-sample", type=int, default=32, help="zones in the CPU-baseline sample")
+    p.add_argument("--plain-sample", type=int, default=32,
+                   help="zones (stratified in T9 up to 1) on which the plain FP64 path is timed")
+    p.add_argument("--cpu-sample", type=int, default=128, help="zones in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-uneven", action="store_true",
                    help="skip the one-cycle run of the uneven 8,192-zone workload")
+    p.add_argument("--no-extra", action="store_true",
+                   help="skip the mppnp-cycle and rate-physics runs")
     p.add_argument("--save", default="", help="write the step's abundances and counters (npz)")
     return p.parse_args()
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "zone_newton_block_kernel_2048x1100_backward_euler_traffic.json")
+                               "newton_kernels_2048x1100_backward_euler_traffic.json")
 
 
 def measured_traffic():
@@ -240,17 +242,18 @@ def main():
     # --- plain FP64 path (reference pivot order) on a bounded sample of light zones
     plain = None
     if not a.no_plain:
-        # stratified: zones from the cold end up to T9 0.5 (the plain path's one-wavefront GEPP
-        # takes about 0.1 s per Newton iteration, so a hot zone's 700 iterations take minutes)
+        # stratified: zones from the cold end up to T9 1 (the plain path's one-wavefront GEPP
+        # takes about 0.1 s per Newton iteration, so the hottest zones' 1,300 iterations take
+        # minutes; T9 up to 1 keeps zones of up to ~30 sub-steps)
         order = np.argsort(w.t9[:, 0])
-        warm = order[w.t9[order, 0] < 0.5]
+        warm = order[w.t9[order, 0] < 1.0]
         sel = np.sort(warm[np.linspace(0, warm.size - 1, min(a.plain_sample, warm.size)).round()
                            .astype(int)])
         elp, yp, iep, stp, _ = run("plain", sel)
         elb, yb, ieb, stb, _ = run("mixed", sel)
         mm = (yp > 1e-20) | (yb > 1e-20)
         plain = {"zones": int(len(sel)), "selection": "stratified in T9 over the zones with "
-                 f"T9 < 0.5 (T9 {w.t9[sel, 0].min():.3f} .. {w.t9[sel, 0].max():.3f}; the plain "
+                 f"T9 < 1 (T9 {w.t9[sel, 0].min():.3f} .. {w.t9[sel, 0].max():.3f}; the plain "
                  "path's per-iteration latency makes the hottest zones take minutes)",
                  "plain_zone_timesteps_per_s": len(sel) / elp,
                  "mixed_zone_timesteps_per_s": len(sel) / elb, "mixed_over_plain": elp / elb,
@@ -303,6 +306,54 @@ def main():
                   "zones_on_fp64": int((st2["n_factor_fp64"] > 0).sum())}
         del y2, a2, e2, i2, s2
 
+    # --- the mppnp cycle on the device (zoneselect, batch, renormalisation, mixing) and the
+    # same cycle with the rate physics on (partition functions, detailed balance, screening)
+    extra = None
+    if not a.no_extra:
+        sm = pkg.network.stellar_model(nz, t9_center=3.0, t9_surface=0.01)
+        yc = x0.clone()
+        tt = [tens(v) for v in (sm.t9, sm.rho, sm.xm, sm.dq, sm.rad, sm.d_eul)]
+        ie_m = torch.zeros(nz, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = ctx.mppnp_cycle_dev(nz, yc.data_ptr(), *(t.data_ptr() for t in tt), 1e4,
+                                 ie_m.data_ptr(), opts=hip.mppnp_opts(t9threshold=0.05))
+        torch.cuda.synchronize()
+        el_m = time.perf_counter() - t0
+        tm_m = ctx.last_timing()
+        nsel = int((sm.t9 > 0.05).sum())
+        ctx_p = hip.Context(net)
+        ctx_p.set_rate_physics(pkg.network.rate_physics(net))
+        yp = x0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx_p.integrate_batch_mixed_dev(nz, yp.data_ptr(), *(t.data_ptr() for t in (t9_0, t9_1,
+                                        r0, r1)), ye.data_ptr(), dt.data_ptr(), ierr.data_ptr(),
+                                        stats.data_ptr())
+        torch.cuda.synchronize()
+        el_p = time.perf_counter() - t0
+        tm_p = ctx_p.last_timing()
+        ie_p = ierr.cpu().numpy()
+        ctx_p.close()
+        extra = {"mppnp_cycle": {"zones": nz, "integrated": nsel, "rc": int(rc),
+                                 "ierr_nonzero": int((ie_m.cpu().numpy() != 0).sum()),
+                                 "cycle_ms": 1e3 * el_m,
+                                 "integration_ms": tm_m["total_ms"] + tm_m["rates_ms"],
+                                 "selection_mixing_renormalisation_ms":
+                                     1e3 * el_m - tm_m["total_ms"] - tm_m["rates_ms"],
+                                 "model": "synthetic star: T9 3 -> 0.01 centre to surface, "
+                                          "convective core, shell and envelope; zoneselect "
+                                          "T9 > 0.05; matrixdiffus (GNV / EVL form)"},
+                 "rate_physics": {"zone_timesteps_per_s": nz / el_p,
+                                  "rates_ms": tm_p["rates_ms"],
+                                  "rates_ms_bare_reaclib": tm["rates_ms"],
+                                  "ierr_nonzero": int((ie_p != 0).sum()),
+                                  "what": "the same 2,048-zone cycle with partition functions, "
+                                          "detailed balance and Chugunov screening on (the "
+                                          "reference's defaults); the screened rates make the "
+                                          "zones stiffer"}}
+        log(f"mppnp cycle: {el_m * 1e3:.1f} ms; rate physics cycle {el_p * 1e3:.1f} ms")
+
     if a.save:
         np.savez(a.save, y_mixed=y_mixed, y_fp64=y64, nsubt_mixed=st_mixed["nsubt"],
                  nsubt_fp64=st64["nsubt"], newton_mixed=st_mixed["n_newton"],
@@ -328,7 +379,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
-                     "kernel": "zone_newton_block_kernel<32, 1 or 2> (one launch per pass)",
+                     "kernel": "the Newton worker, one launch per pass: "
+                               "zone_newton_block_kernel<32, 1|2> (many zones) or "
+                               "zone_newton_team_kernel<32, 2|4> (short work lists)",
                      "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s,
                      # time-weighted: the same algorithmic bytes over the whole timed step
                      "pipeline_frac": bytes_step / (elapsed / a.steps) / 1e9 / HBM_PEAK_GBS},
@@ -354,6 +407,8 @@ def main():
             "passes": tm["passes"], "algorithmic_bytes_per_step": bytes_step,
             "plain_sample": plain,
             "uneven_8192_zones_one_cycle": uneven,
+            "graph_launches": tm.get("graph_launches"),
+            "extra": extra,
             "cpu_one_core": None if cpu_one is None else {
                 "value": cpu_one["value"], "unit": cpu_one["unit"], "sample": cpu_one["sample"]},
         },

@@ -432,6 +432,18 @@ class Context:
             raise RuntimeError(f"nuppn_mppnp_cycle failed ({rc})")
         return rc, ierr, y, stats
 
+    def mppnp_cycle_dev(self, m, yps, t9, rho, xm, dq, rad, d_eul, dzeit, ierr, stats=None,
+                        opts=None, bopts=None, stream=0):
+        """Device-pointer variant of mppnp_cycle; returns the code (OK or ERR_MIXING)."""
+        mo = opts if opts is not None else mppnp_opts()
+        bo = bopts if bopts is not None else batch_opts()
+        rc = load().nuppn_mppnp_cycle_dev(self.h, m, yps, t9, rho, xm, dq, rad, d_eul,
+                                          float(dzeit), C.byref(mo), C.byref(bo), ierr, stats,
+                                          stream)
+        if rc not in (OK, ERR_MIXING):
+            raise RuntimeError(f"nuppn_mppnp_cycle_dev failed ({rc})")
+        return rc
+
     def last_timing(self):
         """dict: rates_ms, total_ms, newton_ms, control_ms, newton_launches, passes,
         graph_launches"""
