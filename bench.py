@@ -43,6 +43,10 @@ def _args():
                    help="skip the one-cycle run of the uneven 8,192-zone workload")
     p.add_argument("--no-extra", action="store_true",
                    help="skip the mppnp-cycle and rate-physics runs")
+    p.add_argument("--uneven-cycles", type=int, default=1,
+                   help="cycles of the uneven 8,192-zone workload (its definition: 50; the "
+                        "abundances carry over from cycle to cycle); the FP64-factor "
+                        "comparison runs on the first cycle only")
     p.add_argument("--save", default="", help="write the step's abundances and counters (npz)")
     return p.parse_args()
 
@@ -288,6 +292,30 @@ def main():
             res[path] = (el2, y2.cpu().numpy(), i2.cpu().numpy(), st2)
             log(f"uneven 8,192-zone cycle ({path}): {el2 * 1e3:.1f} ms")
         el2, yy, ii, st2 = res["mixed"]
+        multi = None
+        if a.uneven_cycles > 1:
+            # the workload's 50 cycles: each cycle continues from the previous one's abundances
+            y2.copy_(tens(w2.x0))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            per = []
+            for cyc in range(a.uneven_cycles):
+                tc = time.perf_counter()
+                ctx.integrate_batch_mixed_dev(n2, y2.data_ptr(), *(t.data_ptr() for t in a2[:4]),
+                                              e2.data_ptr(), a2[4].data_ptr(), i2.data_ptr(),
+                                              s2.data_ptr())
+                torch.cuda.synchronize()
+                per.append(time.perf_counter() - tc)
+                log(f"uneven cycle {cyc + 1}/{a.uneven_cycles}: {per[-1] * 1e3:.1f} ms")
+                if int((i2 != 0).sum().item()) != 0:
+                    break
+            elm = time.perf_counter() - t0
+            ym = y2.cpu().numpy()
+            multi = {"cycles": len(per), "zone_timesteps_per_s": n2 * len(per) / elm,
+                     "ms_per_cycle_min_max": [1e3 * min(per), 1e3 * max(per)],
+                     "ierr_nonzero_last": int((i2.cpu().numpy() != 0).sum()),
+                     "max_abs_sumX_minus_1_end": float(np.abs(ym.sum(axis=1) - 1.0).max()),
+                     "finite": bool(np.isfinite(ym).all())}
         y64u, st64u = res["fp64"][1], res["fp64"][3]
         mu = (yy > 1e-20) | (y64u > 1e-20)
         relu = np.where(mu, np.abs(yy - y64u) / np.maximum(np.maximum(yy, y64u), 1e-300), 0.0)
@@ -303,7 +331,8 @@ def main():
                   "max_abs_sumX_minus_1": float(np.abs(yy.sum(axis=1) - 1.0).max()),
                   "max_sub_steps": int(st2["nsubt"].max()),
                   "max_newton_per_zone": int(st2["n_newton"].max()),
-                  "zones_on_fp64": int((st2["n_factor_fp64"] > 0).sum())}
+                  "zones_on_fp64": int((st2["n_factor_fp64"] > 0).sum()),
+                  "all_cycles": multi}
         del y2, a2, e2, i2, s2
 
     # --- the mppnp cycle on the device (zoneselect, batch, renormalisation, mixing) and the
