@@ -38,8 +38,10 @@ struct BatchOpts {
 // (B_JAC .. B_SORT: the Bader-Deuflhard step's J, tableau and work vectors)
 // (B_LO: the low parts of the extra-precise residual; B_XLO: those of the solution; B_APHYS:
 // the assembled matrix in the residual stream's physical order)
+// (B_FIRST, B_RFLAG, B_EFLAG: the dynamic network's marks, dynamic.hpp)
 enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_JAC, B_YK, B_YEXT, B_YSCAL,
-                B_DELTA, B_QCOL, B_YERR, B_SORT, B_LO, B_XLO, B_APHYS, B_COUNT };
+                B_DELTA, B_QCOL, B_YERR, B_SORT, B_LO, B_XLO, B_APHYS, B_FIRST, B_RFLAG, B_EFLAG,
+                B_COUNT };
 
 // the factors, in the same slot after the arrays above
 enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
@@ -59,7 +61,7 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
                                  8ull * n, 8ull * n, 8ull * n, 8ull * nnzA,
                                  8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
                                  64ull * nsp, 8ull * pow2, 8ull * n, 8ull * n,
-                                 8ull * nres_phys};
+                                 8ull * nres_phys, 4ull * nsp, (size_t) nre, (size_t) nnzA};
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                   4ull * nbd + 4};
@@ -255,16 +257,20 @@ __device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const doubl
 // precomputed coefficient lists (J, dX/dt and the rhs each rounded once from double-double
 // sums).  lo: [n], jlo: [nnzA] scratch.  Returns 1 where the reference stops (reduced_value).
 template <class TM>
-__device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B, double dt,
+__device__ __forceinline__ int block_assemble(const DevNet &N, const DevKnobs &K,
+                                      const DevBlock &B, double dt,
                                       const double *y_old, const double *y_cur, const double *v,
                                       double *xn, double *vx, double *stem, double *Aval,
                                       double *rhs, double *lop, double *jlo, double *aphys,
-                                      NUPPN_LDS double *red) {
+                                      const DynWork &DW, int &nvar1, NUPPN_LDS double *red) {
     const int t0 = TM::tid();
     molar(N, y_cur, xn, t0, TM::NT);
     TM::sync();
     reaction_terms(N, xn, v, vx, stem, stem + N.nre, t0, TM::NT);
     TM::sync();
+    nvar1 = N.n;
+    if (K.dynamic)
+        nvar1 = dyn_network<TM>(N, K, dt, xn, v, stem, stem + N.nre, B.dyn_k, B.nnzA, DW, red);
     // the streamed sums store raw totals; the scalings follow in coalesced passes
     const NUPPN_GLB unsigned char *isdiag = gptr(B.csr_isdiag);
     NUPPN_GLB double *Ag = gptr(Aval);
@@ -272,10 +278,28 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B
     int stop = 0;
     const NUPPN_GLB int *rpos = gptr(B.resid_pos);
     NUPPN_GLB double *Ap = gptr(aphys);
-    for (int k = t0; k < B.nnzA; k += TM::NT) {
-        const double val = reduced_value(Ag[k], dt, isdiag[k], stop);
-        Ag[k] = val;
-        Ap[rpos[k]] = val;
+    if (K.dynamic) {
+        // the reduced matrix of the dynamic network embedded in the static one (dynamic.hpp)
+        const NUPPN_GLB int *crow = gptr(B.csr_row), *r2s = gptr(N.red2sp);
+        const NUPPN_GLB unsigned char *eflag = gptr(DW.eflag);
+        for (int k = t0; k < B.nnzA; k += TM::NT) {
+            const bool diag = isdiag[k];
+            double val;
+            if (!dyn_marked(DW, r2s[crow[k]]))
+                val = diag ? 1.0 : 0.0;
+            else if (!diag && !eflag[k])
+                val = 0.0;
+            else
+                val = reduced_value(Ag[k], dt, diag, stop);
+            Ag[k] = val;
+            Ap[rpos[k]] = val;
+        }
+    } else {
+        for (int k = t0; k < B.nnzA; k += TM::NT) {
+            const double val = reduced_value(Ag[k], dt, isdiag[k], stop);
+            Ag[k] = val;
+            Ap[rpos[k]] = val;
+        }
     }
     stop = team_any<TM>(stop != 0, red) ? 1 : 0;
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
@@ -292,7 +316,8 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevBlock &B
         const double e2 = fma(dt, fh, -p) + dt * lo[r];
         const double s2 = s1 + p, b2 = s2 - s1;
         const double e3 = (s1 - (s2 - b2)) + (p - b2);
-        rg[r] = s2 + (e1 + (e2 + e3));
+        // reduce_rhs packs the masked species only: the others solve to an exact zero
+        rg[r] = (!K.dynamic || dyn_marked(DW, sp)) ? s2 + (e1 + (e2 + e3)) : 0.0;
     }
     TM::sync();
     return stop;
@@ -326,10 +351,12 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     const double dt = s.dttry;
     NUPPN_LDS double *red = make_block_lds<double, NB>(lds, b.B.nip).red;
     const int t0 = TM::tid();
-    const int stop = block_assemble<TM>(N, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
+    const DynWork DW{(int *) (slot + b.off[B_FIRST]), (unsigned char *) (slot + b.off[B_RFLAG]),
+                     (unsigned char *) (slot + b.off[B_EFLAG])};
+    const int stop = block_assemble<TM>(N, K, b.B, dt, y_old, y_cur, v, xn, vx, stem, Aval, rhs,
                                         (double *) (slot + b.off[B_LO]),
                                         (double *) (slot + b.off[B_JAC]),
-                                        (double *) (slot + b.off[B_APHYS]), red);
+                                        (double *) (slot + b.off[B_APHYS]), DW, s.st.nvar1, red);
     int rc = stop ? 2 : 1;
     if (!stop && s.pad0 == 0) {
         rc = block_refined_solve<float, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
@@ -378,6 +405,7 @@ struct BlockBDSolver {
     ZoneState &s;
     const double *v;
     const double *ycur;   // abundances of the current stage (the correction test's scale)
+    int nvar1;            // the static system's size (the dynamic network is not served here)
 
     __device__ double *arr(int i) const { return (double *) (slot + b.off[i]); }
 
@@ -465,7 +493,7 @@ __device__ void zone_bd_block(const KArgs &a, const BKArgs &b, char *slot, NUPPN
     if (s.phase != PH_NEWTON) return;
     BDState bd = Z.bd[z];
     auto arr = [&](int i) { return (double *) (slot + b.off[i]); };
-    BlockBDSolver<NB> sv{a, b, slot, lds, s, zrow(Z.v, Z.nre_stride, z), arr(B_YK)};
+    BlockBDSolver<NB> sv{a, b, slot, lds, s, zrow(Z.v, Z.nre_stride, z), arr(B_YK), a.N.n};
     BDWork W{arr(B_YK), arr(B_YEXT), arr(B_YSCAL), arr(B_DELTA), arr(B_RHS), arr(B_X),
              arr(B_QCOL), arr(B_YERR), arr(B_SORT)};
     bd_step(a, z, s, bd, W, sv);

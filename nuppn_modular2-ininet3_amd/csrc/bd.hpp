@@ -158,7 +158,7 @@ __device__ void bd_step(const KArgs &a, int z, ZoneState &s, BDState &bd, const 
             break;
         }
         s.st.iter += ns;
-        s.st.nvar1 = n;
+        s.st.nvar1 = sv.nvar1;
         xcol[k - 1] = xest;
         bd_extrapolate(nsp, k, xest, xcol, W.yk, W.yext, W.qcol, W.yerr);
         copy_vec(W.yk, W.yext, nsp);
@@ -242,32 +242,65 @@ struct PlainBDSolver {
     LUWork w;
     ZoneState &s;
     const double *v;
+    int nvar1;
 
+    // calculate_jacobian (jac_rhs.F90:282-455) once per step: the mask and the entries of the
+    // reduced matrix (every entry of every species on the static network), the 1e-99 seed, and
+    // the stop on an exact zero off the diagonal among the entries kept (444-451)
     __device__ int jacobian(const double *y, double dt) {
         const DevNet &N = a.N;
-        (void) dt;   // min_rate_ov_dt is -1e99 on the static network: no reaction is skipped
+        const DynWork DW{S.first, S.rflag, S.eflag};
         molar(N, y, S.xn);
         wave_sync();
         reaction_terms(N, S.xn, v, S.vx, S.st1, S.st2);
         wave_sync();
-        // calculate_jacobian (jac_rhs.F90:282-455): the 1e-99 seed, then an exact zero off the
-        // diagonal stops it (444-451)
+        if (a.K.dynamic) {
+            nvar1 = dyn_network<Team<1>>(N, a.K, dt, S.xn, v, S.st1, S.st2, N.dyn_q, N.nnzA, DW,
+                                         nullptr);
+        } else {
+            for (int r = lane_id(); r < N.n; r += WAVE) S.first[N.red2sp[r]] = 0;
+            for (int q = lane_id(); q < N.nnzA; q += WAVE) S.eflag[q] = 1;
+            nvar1 = N.n;
+            wave_sync();
+        }
         int bad = 0;
         for (int q = lane_id(); q < N.nnzA; q += WAVE) {
             const double jv = jac_entry(N, q, S.st1, S.st2);
             S.jac_cpy[q] = jv;
-            if (!N.A_isdiag[q] && jv == 0.0) bad = 1;
+            if (!N.A_isdiag[q] && S.eflag[q] && dyn_marked(DW, N.red2sp[N.A_rowind[q]]) &&
+                jv == 0.0)
+                bad = 1;
         }
         wave_sync();
         return ballot(bad) ? E_BAD_STEP : 0;
     }
 
-    // I - h J with reduce_jacobian's drops; E_BAD_STEP where the reference stops
+    // I - h J with reduce_jacobian's drops (jac_rhs.F90:138-161); E_BAD_STEP where the reference
+    // stops.  A dropped entry leaves the reduced matrix for the rest of the step, as the
+    // reference's row(ice) = -1 stays set until the next calculate_jacobian; species outside
+    // the mask keep identity rows.
     __device__ int level(double h) {
         const DevNet &N = a.N;
+        const DynWork DW{S.first, S.rflag, S.eflag};
         int stop = 0;
-        for (int q = lane_id(); q < N.nnzA; q += WAVE)
-            S.Aval[q] = reduced_value(S.jac_cpy[q], h, N.A_isdiag[q], stop);
+        for (int q = lane_id(); q < N.nnzA; q += WAVE) {
+            const bool diag = N.A_isdiag[q];
+            double val;
+            if (!dyn_marked(DW, N.red2sp[N.A_rowind[q]])) {
+                val = diag ? 1.0 : 0.0;
+            } else if (diag) {
+                val = reduced_value(S.jac_cpy[q], h, true, stop);
+            } else if (!S.eflag[q]) {
+                val = 0.0;
+            } else {
+                val = S.jac_cpy[q] * (-h);
+                if (fabs(val) == 0.0 || val == -1.0) {
+                    S.eflag[q] = 0;
+                    val = 0.0;
+                }
+            }
+            S.Aval[q] = val;
+        }
         wave_sync();
         if (ballot(stop)) return E_BAD_STEP;
         const int info = gepp_factor(N.n, N.A_colptr, N.A_rowind, S.Aval, N.iperm_c, w, S.lu);
@@ -287,9 +320,12 @@ struct PlainBDSolver {
         wave_sync();
     }
 
+    // reduce_rhs_bd packs the masked species; the others get x = 0 (bader_deuflhard_step, 420)
     __device__ int solve(const double *rhs, double *x) {
         const DevNet &N = a.N;
-        for (int r = lane_id(); r < N.n; r += WAVE) x[r] = rhs[r];
+        const DynWork DW{S.first, S.rflag, S.eflag};
+        for (int r = lane_id(); r < N.n; r += WAVE)
+            x[r] = dyn_marked(DW, N.red2sp[r]) ? rhs[r] : 0.0;
         wave_sync();
         gepp_solve(N.n, N.perm_c, w, S.lu, x);
         s.st.n_solve++;
@@ -303,7 +339,7 @@ __device__ inline void zone_bd(const KArgs &a, const Slot &S, LUWork w, int z) {
     ZoneState s = Z.state[z];
     if (s.phase != PH_NEWTON) return;
     BDState bd = Z.bd[z];
-    PlainBDSolver sv{a, S, w, s, zrow(Z.v, Z.nre_stride, z)};
+    PlainBDSolver sv{a, S, w, s, zrow(Z.v, Z.nre_stride, z), a.N.n};
     BDWork W{S.y_cur, S.yext, S.yscal, S.delta, S.rhs, S.y_new, S.qcol, S.yerr, S.sortbuf};
     bd_step(a, z, s, bd, W, sv);
     s.phase = PH_AFTER_NEWTON;

@@ -46,6 +46,8 @@ struct DevBlock {
     const int *ent_dst;    // D/C/B entries: [area][block][NB*NB] destination (area 0 D, 1 C, 2 B)
     const double *ent_ys;
     const int *csr_col;    // [nnzA] column of each A entry (A kept in CSR order)
+    const int *csr_row;    // [nnzA] row of each A entry
+    const int *dyn_k;      // [8 nre] the dynamic network's entry map (DevNet::dyn_q) in CSR order
     const unsigned char *csr_isdiag;
     const int *resid_pos;  // [nnzA] physical position of each A entry in the residual stream
     DevSeg jac, dxdt, resid;
@@ -115,57 +117,6 @@ __device__ inline BlockLds<T> make_block_lds(NUPPN_LDS char *base, int nip) {
     L.X2b = L.X1b + NB * C::LD;
     L.vecb = L.X1b;
     return L;
-}
-
-// --- reductions over the threads of a zone (Team): every thread gets the result.  With one
-// wavefront they are cross-lane operations; with several, each wavefront's value goes through
-// the LDS slots red and the wavefronts' values are combined in wavefront order.
-template <class TM>
-__device__ __forceinline__ double team_max(double v, NUPPN_LDS double *red) {
-    v = wave_max(v);
-    if constexpr (TM::NWAVE == 1) {
-        return v;
-    } else {
-        if (lane_id() == 0) red[TM::wave()] = v;
-        TM::sync();
-        double m = red[0];
-#pragma unroll
-        for (int w = 1; w < TM::NWAVE; ++w) m = fmax(m, red[w]);
-        TM::sync();
-        return m;
-    }
-}
-
-template <class TM>
-__device__ __forceinline__ double team_sum(double v, NUPPN_LDS double *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    if constexpr (TM::NWAVE == 1) {
-        return v;
-    } else {
-        if (lane_id() == 0) red[TM::wave()] = v;
-        TM::sync();
-        double m = red[0];
-#pragma unroll
-        for (int w = 1; w < TM::NWAVE; ++w) m = m + red[w];
-        TM::sync();
-        return m;
-    }
-}
-
-template <class TM>
-__device__ __forceinline__ bool team_any(bool p, NUPPN_LDS double *red) {
-    const bool w = ballot(p) != 0;
-    if constexpr (TM::NWAVE == 1) {
-        return w;
-    } else {
-        if (lane_id() == 0) red[TM::wave()] = w ? 1.0 : 0.0;
-        TM::sync();
-        bool any = false;
-#pragma unroll
-        for (int k = 0; k < TM::NWAVE; ++k) any = any || red[k] != 0.0;
-        TM::sync();
-        return any;
-    }
 }
 
 template <class T>

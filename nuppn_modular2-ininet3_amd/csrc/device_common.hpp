@@ -81,6 +81,58 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
+// --- reductions over the threads of a zone (Team): every thread gets the result.  With one
+// wavefront they are cross-lane operations; with several, each wavefront's value goes through
+// the LDS slots red and the wavefronts' values are combined in wavefront order.
+template <class TM>
+__device__ __forceinline__ double team_max(double v, NUPPN_LDS double *red) {
+    v = wave_max(v);
+    if constexpr (TM::NWAVE == 1) {
+        return v;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = v;
+        TM::sync();
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < TM::NWAVE; ++w) m = fmax(m, red[w]);
+        TM::sync();
+        return m;
+    }
+}
+
+template <class TM>
+__device__ __forceinline__ double team_sum(double v, NUPPN_LDS double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    if constexpr (TM::NWAVE == 1) {
+        return v;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = v;
+        TM::sync();
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < TM::NWAVE; ++w) m = m + red[w];
+        TM::sync();
+        return m;
+    }
+}
+
+template <class TM>
+__device__ __forceinline__ bool team_any(bool p, NUPPN_LDS double *red) {
+    const bool w = ballot(p) != 0;
+    if constexpr (TM::NWAVE == 1) {
+        return w;
+    } else {
+        if (lane_id() == 0) red[TM::wave()] = w ? 1.0 : 0.0;
+        TM::sync();
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < TM::NWAVE; ++k) any = any || red[k] != 0.0;
+        TM::sync();
+        return any;
+    }
+}
+
+
 // s = ((0 + a[0]) + a[1]) + ... : the sequential order of Fortran's SUM (every lane gets s)
 template <class P>
 __device__ inline double seq_sum(P a, int n) {
@@ -261,6 +313,9 @@ struct DevNet {
     const unsigned char *A_isdiag;  // [nnzA]
     const int *perm_c;      // [n] SuperLU convention
     const int *iperm_c;     // [n]
+    // the dynamic network: reduced CSC entry of slot m (ice1..ice8) of reaction l at
+    // [m * nre + l], -1 where the slot holds gamma
+    const int *dyn_q;
     // the REACLIB T9 cache (null when the rates are evaluated directly): [kNT9][nre] rates at
     // rho = 1 on the grid rc_t9 [kNT9] of spacing rc_dt9
     const double *rcache;

@@ -10,6 +10,10 @@ namespace nuppn {
 struct DevKnobs {
     int ittd, max_sub_steps, integration_method, interpolate_temp, interpolate_dens;
     double dgrd, grdthreshold, min_t9_for_slv, delta_state_max;
+    // the dynamic network (cyminformal >= 0): the mask and the reduced entries are recomputed
+    // at every Jacobian evaluation (jac_rhs.F90:308-436)
+    int dynamic;
+    double cyminformal, min_rate_ov_dt;
 };
 
 // per-zone diagnostics, layout of nuppn_zone_stats
@@ -30,6 +34,8 @@ struct Slot {
     double *delta;                                              // [nsp]   (BD)
     double *qcol, *yerr;                                        // [8*nsp] (BD)
     double *yscal, *yext;                                       // [nsp]   (BD)
+    int *first;                                                 // [nsp]   (dynamic network)
+    unsigned char *rflag, *eflag;                               // [nre], [nnzA]
     LUStore lu;
 };
 

@@ -281,7 +281,7 @@ struct nuppn_ctx {
     // device copies of the network and symbolic data
     int *d_k = nullptr, *d_rtype = nullptr, *d_red2sp = nullptr, *d_dx_ptr = nullptr;
     int *d_dx_ent = nullptr, *d_jq_ptr = nullptr, *d_jq_ent = nullptr, *d_Acp = nullptr;
-    int *d_Ari = nullptr, *d_perm_c = nullptr, *d_iperm_c = nullptr;
+    int *d_Ari = nullptr, *d_perm_c = nullptr, *d_iperm_c = nullptr, *d_dyn_q = nullptr;
     unsigned char *d_isdiag = nullptr;
     double *d_coef = nullptr, *d_aw = nullptr, *d_zw = nullptr;
     // persistent-kernel slots
@@ -314,6 +314,7 @@ struct nuppn_ctx {
     int bnb = 0;   // block size, 0 when the network's interior is not block tridiagonal
     int *d_b_vpos = nullptr, *d_b_diagk = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_k = nullptr;
     int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr, *d_b_ent_dst = nullptr;
+    int *d_b_csr_row = nullptr, *d_b_dyn_k = nullptr;
     unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
     int *d_seg_i[12] = {};        // jac, dxdt, resid: e_idx, lane_len, lane_s0, round_base
@@ -376,7 +377,8 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_ent_loc, (void *) d_b_ent_col, (void *) d_b_csr_col,
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
                     (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9,
-                    (void *) d_b_resid_pos})
+                    (void *) d_b_resid_pos, (void *) d_dyn_q, (void *) d_b_csr_row,
+                    (void *) d_b_dyn_k})
         if (p) (void) hipFree(p);
     for (void *p : {(void *) d_pf_t9, (void *) d_pf_tab, (void *) d_g_spin, (void *) d_mass_g,
                     (void *) d_q_mev, (void *) d_rev_ch, (void *) d_rev_part, (void *) d_db_fwd,
@@ -430,6 +432,7 @@ static DevNet dev_net(const nuppn_ctx *c) {
     N.A_isdiag = c->d_isdiag;
     N.perm_c = c->d_perm_c;
     N.iperm_c = c->d_iperm_c;
+    N.dyn_q = c->d_dyn_q;
     N.rcache = c->use_cache ? c->d_rcache : nullptr;
     N.rc_t9 = c->d_rc_t9;
     N.rc_dt9 = c->rc_dt9;
@@ -476,6 +479,9 @@ static DevKnobs dev_knobs(const nuppn_knobs &k) {
     K.grdthreshold = k.grdthreshold;
     K.min_t9_for_slv = k.min_t9_for_slv;
     K.delta_state_max = k.delta_state_max;
+    K.dynamic = k.cyminformal >= 0.0;
+    K.cyminformal = k.cyminformal;
+    K.min_rate_ov_dt = k.min_rate_ov_dt;
     return K;
 }
 
@@ -499,6 +505,8 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     B.ent_col = c->d_b_ent_col;
     B.ent_ys = c->d_b_ent_ys;
     B.csr_col = c->d_b_csr_col;
+    B.csr_row = c->d_b_csr_row;
+    B.dyn_k = c->d_b_dyn_k;
     B.csr_isdiag = c->d_b_isdiag;
     B.resid_pos = c->d_b_resid_pos;
     const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
@@ -576,7 +584,8 @@ static void free_batched(nuppn_ctx *c) {
                      (void **) &c->d_b_ent_k, (void **) &c->d_b_ent_loc, (void **) &c->d_b_ent_col,
                      (void **) &c->d_b_csr_col, (void **) &c->d_b_isdiag, (void **) &c->d_b_mass,
                      (void **) &c->d_b_ent_ys, (void **) &c->d_b_ent_dst, (void **) &c->d_jac_lo,
-                     (void **) &c->d_b_resid_pos, (void **) &c->d_bslots}) {
+                     (void **) &c->d_b_resid_pos, (void **) &c->d_bslots,
+                     (void **) &c->d_b_csr_row, (void **) &c->d_b_dyn_k}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -606,7 +615,8 @@ static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
         dev_upload(&c->d_b_ent_loc, b.ent_loc) || dev_upload(&c->d_b_ent_col, b.ent_col) ||
         dev_upload(&c->d_b_csr_col, b.csr_col) || dev_upload(&c->d_b_isdiag, b.csr_isdiag) ||
         dev_upload(&c->d_b_mass, b.mass) || dev_upload(&c->d_b_ent_ys, b.ent_ys) ||
-        dev_upload(&c->d_b_ent_dst, b.ent_dst))
+        dev_upload(&c->d_b_ent_dst, b.ent_dst) || dev_upload(&c->d_b_csr_row, b.csr_row) ||
+        dev_upload(&c->d_b_dyn_k, b.dyn_k))
         return NUPPN_ERR_HIP;
     const SegStream *sc[3] = {&b.jac, &b.dxdt, &b.resid};
     for (int i = 0; i < 3; ++i) {
@@ -828,9 +838,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     nuppn_knobs kn;
     if (knobs) kn = *knobs;
     else nuppn_default_knobs(&kn);
-    if (kn.cyminformal >= 0.0 || kn.irdn != 2) {
-        // the device path uses the static reduced pattern (every species of every reaction);
-        // the reference's dynamic network (cyminformal >= 0) changes it per iteration
+    if (kn.irdn != 2) {   // calculate_jacobian stops on any other choice (jac_rhs.F90:297-304)
         *status = NUPPN_ERR_UNSUPPORTED;
         return nullptr;
     }
@@ -838,7 +846,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         *status = NUPPN_ERR_UNSUPPORTED;
         return nullptr;
     }
-    kn.min_rate_ov_dt = -1.0e99;   // solver_knobs.F90:66-69 for cyminformal < 1e-99
+    if (kn.cyminformal < 1.0e-99) kn.min_rate_ov_dt = -1.0e99;   // solver_knobs.F90:66-69
     auto c = std::make_unique<nuppn_ctx>();
     c->kn = kn;
     bd_tables(kn.dgrd, c->bdt, c->bd);
@@ -866,7 +874,7 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
         dev_upload(&c->d_dx_ptr, s.dx_ptr) || dev_upload(&c->d_dx_ent, s.dx_ent) ||
         dev_upload(&c->d_jq_ptr, s.jq_ptr) || dev_upload(&c->d_jq_ent, s.jq_ent) ||
         dev_upload(&c->d_Acp, s.A_colptr) || dev_upload(&c->d_Ari, s.A_rowind) ||
-        dev_upload(&c->d_isdiag, s.A_isdiag) ||
+        dev_upload(&c->d_isdiag, s.A_isdiag) || dev_upload(&c->d_dyn_q, s.dyn_q) ||
         dev_upload(&c->d_perm_c, (const int *) nullptr, (size_t) s.n) ||
         dev_upload(&c->d_iperm_c, (const int *) nullptr, (size_t) s.n) ||
         dev_upload(&c->d_flag, (const int *) nullptr, 2 * kPassRing) ||

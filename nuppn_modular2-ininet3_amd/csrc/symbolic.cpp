@@ -127,6 +127,11 @@ int build_symbolic(const HostNetwork &net, Symbolic &s) {
     }
     for (int c = 0; c < s.n; ++c) s.A_colptr[c + 1] += s.A_colptr[c];
     s.nnzA = (int) s.A_rowind.size();
+    // the dynamic network's map: the entry each (reaction, slot) contributes to
+    s.dyn_q.assign(8 * (size_t) nre, -1);
+    for (int q = 0; q < s.nnzA; ++q)
+        for (int e = s.jq_ptr[q]; e < s.jq_ptr[q + 1]; ++e)
+            s.dyn_q[(s.jq_ent[e] & 7) * (size_t) nre + (s.jq_ent[e] >> 3)] = q;
 
     // --- dX/dt pull map: contributions of reaction l to species k1, k3, k5, k7 in that order
     //     (jac_rhs.F90:252-255), reactions ascending
@@ -474,6 +479,11 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
     for (int c = 0; c < n; ++c) b.diagk[c] = q2k[b.diagq[c]];
     b.csr_isdiag.resize(nnz);
     for (int k = 0; k < nnz; ++k) b.csr_isdiag[k] = s.A_isdiag[b.csr_q[k]];
+    b.dyn_k.resize(s.dyn_q.size());
+    for (size_t e = 0; e < s.dyn_q.size(); ++e) b.dyn_k[e] = s.dyn_q[e] < 0 ? -1 : q2k[s.dyn_q[e]];
+    b.csr_row.resize(nnz);
+    for (int r = 0; r < n; ++r)
+        for (int k = b.csr_ptr[r]; k < b.csr_ptr[r + 1]; ++k) b.csr_row[k] = r;
     const double *an = net.aw.data();
     // J contributions (statement order of calculate_jacobian, jac_rhs.F90:365-392):
     // value = (+-count) * an[row] / an[col] * stem, stem = st1[l] (m in 0,1,3,4) or st2[l]

@@ -25,6 +25,7 @@ struct Symbolic {
     std::vector<int> A_colptr, A_rowind;       // CSC of the reduced pattern, rows sorted
     std::vector<unsigned char> A_isdiag;
     std::vector<int> jq_ptr, jq_ent;           // per CSC entry: (l<<3 | m) in reaction order
+    std::vector<int> dyn_q;                    // [8*nre] CSC entry of slot m of reaction l, -1: gamma
     std::vector<int> dx_ptr, dx_ent;           // per reduced species: (l<<2 | role)
     std::vector<int> perm_c;                   // column order, SuperLU convention
 };
@@ -89,6 +90,8 @@ struct BlockSym {
     std::vector<int> csr_ptr, csr_q, csr_col;
     // the batched path keeps A in CSR order: ent_k / diagk are ent_q / diagq as CSR positions
     std::vector<int> ent_k, diagk;
+    // the dynamic network's entry map (Symbolic::dyn_q) in CSR order, and each CSR entry's row
+    std::vector<int> dyn_k, csr_row;
     // D/C/B entries: destination in the dense block storage [area][block][NB*NB] (area 0 D_i,
     // 1 C_i, 2 B_i; column-major inside a block)
     std::vector<int> ent_dst;

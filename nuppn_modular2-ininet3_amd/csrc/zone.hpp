@@ -3,6 +3,7 @@
 #pragma once
 #include "device_common.hpp"
 #include "integrate.hpp"
+#include "dynamic.hpp"
 
 namespace nuppn {
 
@@ -12,7 +13,7 @@ namespace nuppn {
 enum SlotArr {
     S_YOLD, S_YCUR, S_YNEW, S_YPS, S_YPSOLD, S_XN, S_DXDT, S_VX, S_ST1, S_ST2, S_VBUF, S_AVAL,
     S_RHS, S_SORT, S_JACCPY, S_DELTA, S_QCOL, S_YERR, S_YSCAL, S_YEXT, S_LI, S_LX, S_UI, S_UX,
-    S_UP, S_COUNT
+    S_UP, S_FIRST, S_RFLAG, S_EFLAG, S_COUNT
 };
 
 struct SlotLayout {
@@ -28,7 +29,8 @@ inline SlotLayout slot_layout(int nsp, int nre, int n, int nnzA) {
         8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * nsp,
         8ull * nre, 8ull * nre, 8ull * nre, 8ull * nre, 8ull * nnzA, 8ull * n, 8ull * pow2,
         8ull * nnzA, 8ull * nsp, 64ull * nsp, 64ull * nsp, 8ull * nsp, 8ull * nsp,
-        4ull * capL, 8ull * capL, 4ull * capU, 8ull * capU, 4ull * (n + 1)};
+        4ull * capL, 8ull * capL, 4ull * capU, 8ull * capU, 4ull * (n + 1), 4ull * nsp,
+        (size_t) nre, (size_t) nnzA};
     SlotLayout L;
     size_t o = 0;
     for (int i = 0; i < S_COUNT; ++i) {
@@ -117,6 +119,9 @@ __device__ inline Slot make_slot(char *base, const size_t *off) {
     S.lu.Ui = (int *) (base + off[S_UI]);
     S.lu.Ux = (double *) (base + off[S_UX]);
     S.lu.Up = (int *) (base + off[S_UP]);
+    S.first = (int *) (base + off[S_FIRST]);
+    S.rflag = (unsigned char *) (base + off[S_RFLAG]);
+    S.eflag = (unsigned char *) (base + off[S_EFLAG]);
     return S;
 }
 
@@ -215,7 +220,7 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
     }
     s.phase = PH_DONE;
     s.ierr = ierr;
-    s.st.nvar1 = N.n;
+    if (!a.K.dynamic) s.st.nvar1 = N.n;
     if (lane_id() == 0) {
         Z.ye[z] = ye;
         Z.ierr[z] = s.lib_err ? s.lib_err : ierr;
@@ -427,12 +432,21 @@ __device__ inline void zone_newton(const KArgs &a, const Slot &S, LUWork w, int 
     wave_sync();
     reaction_terms(N, S.xn, v, S.vx, S.st1, S.st2);
     wave_sync();
-    const int stop = assemble_matrix(N, dt, S.st1, S.st2, S.Aval);
+    const DynWork DW{S.first, S.rflag, S.eflag};
+    int stop;
+    if (K.dynamic) {
+        s.st.nvar1 = dyn_network<Team<1>>(N, K, dt, S.xn, v, S.st1, S.st2, N.dyn_q, N.nnzA, DW,
+                                          nullptr);
+        stop = assemble_matrix_dyn(N, dt, S.st1, S.st2, DW, S.Aval);
+    } else {
+        stop = assemble_matrix(N, dt, S.st1, S.st2, S.Aval);
+    }
     int e = stop ? 1 : 0;
     for (int r = lane_id(); r < N.n; r += WAVE) {
         const int sp = N.red2sp[r];
         const double d = dxdt_species(N, r, S.vx);
-        S.rhs[r] = y_old[sp] - y_cur[sp] + dt * d;
+        // reduce_rhs packs the masked species only: the others solve to an exact zero
+        S.rhs[r] = (!K.dynamic || dyn_marked(DW, sp)) ? y_old[sp] - y_cur[sp] + dt * d : 0.0;
     }
     wave_sync();
     if (!e) {
