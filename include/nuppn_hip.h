@@ -32,7 +32,9 @@ extern "C" {
                                          drops (exactly 0 or -1) are dropped, and a zero where
                                          the reference stops ends the zone with NUPPN_BAD_STEP */
 #define NUPPN_ERR_SINGULAR     (-4)   /* exactly zero pivot (slu_solve returns info > 0 there) */
-#define NUPPN_ERR_UNSUPPORTED  (-5)   /* configuration outside the static-network path */
+#define NUPPN_ERR_UNSUPPORTED  (-5)   /* configuration the device path does not serve (irdn != 2,
+                                         an integration_method other than 0 / 1, a path or
+                                         network shape a call names but the context lacks) */
 #define NUPPN_ERR_MIXING       (-6)   /* the mixing step stops where mixing.F90 does: a zero zone
                                          width or shell mass, or a zero pivot of tridag */
 
@@ -62,8 +64,12 @@ typedef struct {
     int interpolate_dens;
     double dgrd;
     double grdthreshold;
-    double cyminformal;       /* the static-network path needs cyminformal < 0 */
-    double min_rate_ov_dt;
+    double cyminformal;       /* >= 0: the dynamic network (jac_rhs.F90:308-436), the mask and
+                                 the reduced entries recomputed at every Jacobian evaluation;
+                                 < 0: the static network (every species of every reaction) */
+    double min_rate_ov_dt;    /* reactions with |v| / dt at or below it are left out of the
+                                 Jacobian; forced to -1e99 for cyminformal < 1e-99
+                                 (solver_knobs.F90:66-69) */
     double min_t9_for_slv;
     double delta_state_max;
 } nuppn_knobs;
@@ -86,7 +92,8 @@ typedef struct {
 
 typedef struct nuppn_ctx nuppn_ctx;
 
-/* readsolverinput defaults (solver_knobs.F90:43-57) with the static network (cyminformal = -1) */
+/* readsolverinput defaults (solver_knobs.F90:43-57), except the static network (cyminformal = -1;
+ * the namelist's default 1e-30 selects the dynamic one) */
 void nuppn_default_knobs(nuppn_knobs *k);
 
 /* Network set-up: makemap (solver/source/jac_rhs.F90:461-540), the static reduced pattern of
@@ -257,10 +264,17 @@ int nuppn_autotune(nuppn_ctx *ctx, int nz, const double *yps, const double *t9_0
                    const double *t9_1, const double *rho_0, const double *rho_1,
                    const double *dtwant, const char *cache_path, nuppn_tune_result *res);
 
-/* Block structure of the batched path: info[0..6] = block size NB (0: unavailable), blocks,
+/* Block structure of the batched path: info[0..6] = block size NB (0: unavailable; -1: the
+ * band form, for interiors too wide for blocks, whose window is half-bandwidth + 1), blocks,
  * border species, interior species, interior half-bandwidth, resident slots, LDS bytes per
  * slot.  Returns NUPPN_OK or NUPPN_ERR_UNSUPPORTED. */
 int nuppn_batch_block_info(nuppn_ctx *ctx, int *info, int ninfo);
+
+/* Rebuilds the batched path in the given form: nb = 32 or 64 (block tridiagonal with that block
+ * size) or -1 (the band form); the context otherwise takes the first of 32, 64, band that holds.
+ * No counterpart in the reference (a shape choice, like nuppn_autotune's).  Returns NUPPN_OK,
+ * NUPPN_ERR_UNSUPPORTED (the form does not hold for the network) or NUPPN_ERR_ARG. */
+int nuppn_set_block_form(nuppn_ctx *ctx, int nb);
 
 /* Batched mixed-precision path: all zones of an mppnp cycle (replaces the worker loop of
  * frames/mppnp/source/mppnp.F90:713-753).  Same control flow, decisions code and outputs as

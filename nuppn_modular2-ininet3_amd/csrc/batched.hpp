@@ -23,6 +23,7 @@
 #pragma once
 #include "bd.hpp"
 #include "block_lu.hpp"
+#include "band_lu.hpp"
 #include "zone.hpp"
 
 namespace nuppn {
@@ -52,8 +53,10 @@ struct BLayout {
     size_t bytes;   // one slot: arrays, then factors
 };
 
+// band_l / band_u > 0: the band form (nblk = 0), its L and U bands of that many entries in the
+// areas of Sinv and W
 inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk, int nip,
-                            int nbd, int nres_phys) {
+                            int nbd, int nres_phys, size_t band_l = 0, size_t band_u = 0) {
     const size_t bs = (size_t) nblk * NB * NB * 8;   // FP64-sized: FP32 factors use half
     size_t pow2 = 1;
     while (pow2 < (size_t) nsp) pow2 <<= 1;
@@ -62,7 +65,8 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
                                  8ull * nsp, 8ull * nsp, 8ull * nsp, 8ull * n, 64ull * nsp,
                                  64ull * nsp, 8ull * pow2, 8ull * n, 8ull * n,
                                  8ull * nres_phys, 4ull * nsp, (size_t) nre, (size_t) nnzA};
-    const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd), bs, bs, bs,
+    const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd),
+                                  band_l ? 8 * band_l : bs, band_u ? 8 * band_u : bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
                                   4ull * nbd + 4};
     BLayout L;
@@ -81,6 +85,7 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
 
 struct BKArgs {
     DevBlock B;
+    DevBand D;   // the band form's structure (ni = 0 for the block form)
     BatchOpts O;
     char *slot_base;
     size_t slot_bytes;
@@ -96,6 +101,23 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
     F.Bm = (T *) (base + foff[F_BM]);
     F.E = (T *) (base + foff[F_E]);
     F.XF = (T *) (base + foff[F_XF]);
+    F.G = (double *) (base + foff[F_G]);
+    F.gpiv = (int *) (base + foff[F_GPIV]);
+    F.cs = (double *) (base + foff[F_CS]);
+    F.colsc = (double *) (base + foff[F_COLSC]);
+    F.rowinv = (double *) (base + foff[F_ROWINV]);
+    return F;
+}
+
+// the band form's factors in the same factor area (the block form's areas reused: Sinv -> L's
+// band, W -> U's band, E -> L's border rows, XF -> U's border columns)
+template <class T>
+__device__ inline BandFactors<T> band_factors(char *base, const size_t *foff) {
+    BandFactors<T> F;
+    F.Lb = (T *) (base + foff[F_SINV]);
+    F.Uc = (T *) (base + foff[F_W]);
+    F.LE = (T *) (base + foff[F_E]);
+    F.UF = (T *) (base + foff[F_XF]);
     F.G = (double *) (base + foff[F_G]);
     F.gpiv = (int *) (base + foff[F_GPIV]);
     F.cs = (double *) (base + foff[F_CS]);
@@ -125,16 +147,13 @@ __device__ __forceinline__ int block_factor_zone(const BKArgs &b, const double *
 // relative to its own size.  Returns 0 (x, xlo hold the solution), 1 (FP32 factors cannot
 // deliver it: the caller switches to FP64 factors) or -4 (non-finite with FP64 factors: the
 // reference's singular-matrix outcome).
-template <class T, int NB, class TM = Team<1>>
-__device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b,
-                            const double *rhs, const double *y, double *x, double *xlo,
-                            double *res, double *d, char *fac, NUPPN_LDS char *lds,
-                            ZoneStats &st) {
-    constexpr bool fp64 = sizeof(T) == 8;
+template <bool FP64, class TM, class S>
+__device__ __forceinline__ int refine_loop(const DevNet &N, const BKArgs &b, const double *rhs,
+                                           const double *y, double *x, double *xlo, double *res,
+                                           double *d, char *fac, NUPPN_LDS double *red,
+                                           ZoneStats &st, S solve) {
     const DevBlock &B = b.B;
     const int n = B.n;
-    const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
-    const BlockLds<T> L = make_block_lds<T, NB>(lds, B.nip);
     NUPPN_GLB double *xg = gptr(x), *xl = gptr(xlo), *resg = gptr(res);
     const NUPPN_GLB double *dg = gptr((const double *) d), *yg = gptr(y), *rhsg = gptr(rhs);
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
@@ -147,7 +166,7 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b,
     TM::sync();
     double eta_prev = 0.0;
     for (int k = 0; k < b.O.max_refine; ++k) {
-        block_solve<T, NB, TM>(B, F, L, res, d);
+        solve(res, d);
         st.n_solve++;
         double eta = 0.0;
         bool fin = true;
@@ -164,25 +183,39 @@ __device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b,
             eta = fmax(eta, fabs(di) / scale);
             fin = fin && isfinite(hi);
         }
-        eta = team_max<TM>(eta, L.red);
-        if (team_any<TM>(!fin, L.red)) return fp64 ? -4 : 1;
+        eta = team_max<TM>(eta, red);
+        if (team_any<TM>(!fin, red)) return FP64 ? -4 : 1;
         if (eta <= b.O.refine_tol) return 0;   // converged corrections
         // the next residual, extra-precise (d, the correction just applied, serves as its
         // scratch); with accept_tol > 0, x is also accepted once it is backward stable: it
         // solves a system within accept_tol, entry by entry, of A and rhs
         const double omega = block_residual<TM>(B, (const double *) (fac + b.off[B_APHYS]),
                                                 rhs, x, xlo, res, d,
-                                                (double *) (fac + b.off[B_LO]), L.red);
+                                                (double *) (fac + b.off[B_LO]), red);
         st.n_refine++;
         if (b.O.accept_tol > 0.0 && omega <= b.O.accept_tol) return 0;
         // otherwise the corrections must keep contracting (the first correction's eta is
         // about 1 by construction, so contraction is judged from the second one on).  A
         // stall or the step limit with FP32 factors means cond(A) * eps(FP32) is not small
         // and the FP64 factors take over; with FP64 factors the last iterate is the result.
-        if ((k > 1 && eta > 0.5 * eta_prev) || k == b.O.max_refine - 1) return fp64 ? 0 : 1;
+        if ((k > 1 && eta > 0.5 * eta_prev) || k == b.O.max_refine - 1) return FP64 ? 0 : 1;
         eta_prev = eta;
     }
     return 0;
+}
+
+template <class T, int NB, class TM = Team<1>>
+__device__ __forceinline__ int block_refine(const DevNet &N, const BKArgs &b,
+                            const double *rhs, const double *y, double *x, double *xlo,
+                            double *res, double *d, char *fac, NUPPN_LDS char *lds,
+                            ZoneStats &st) {
+    const BlockFactors<T> F = zone_factors<T>(fac, b.foff);
+    const BlockLds<T> L = make_block_lds<T, NB>(lds, b.B.nip);
+    return refine_loop<sizeof(T) == 8, TM>(
+        N, b, rhs, y, x, xlo, res, d, fac, L.red, st,
+        [&](const double *r, double *dd) __attribute__((always_inline)) {
+            block_solve<T, NB, TM>(b.B, F, L, r, dd);
+        });
 }
 
 
@@ -327,9 +360,47 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevKnobs &K
 // the solve with FP32 factors while the zone's factor precision (state pad0) is FP32, and with
 // FP64 factors when that fails (the zone then stays on FP64 for the rest of the call) or when
 // the zone is on FP64 already.  Writes y_new and grd (or lib_err) and sets AFTER_NEWTON.
-template <int NB, class TM>
-__device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
-                                  NUPPN_LDS char *lds, int z) {
+// The factorisation forms of the batched path: the block-tridiagonal one (block size NB) and
+// the band one (band_lu.hpp), each a refined solve with fresh factors and its LDS reduction slots.
+template <int NB>
+struct BlockForm {
+    template <class T, class TM>
+    __device__ __forceinline__ static int refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
+                                        const double *rhs, const double *y, double *x,
+                                        double *xlo, double *res, double *d, char *fac,
+                                        NUPPN_LDS char *lds, ZoneStats &st) {
+        return block_refined_solve<T, NB, TM>(N, b, Aval, rhs, y, x, xlo, res, d, fac, lds, st);
+    }
+    __device__ __forceinline__ static NUPPN_LDS double *red(const BKArgs &b, NUPPN_LDS char *lds) {
+        return make_block_lds<double, NB>(lds, b.B.nip).red;
+    }
+};
+
+struct BandForm {
+    template <class T, class TM>
+    __device__ __forceinline__ static int refined_solve(const DevNet &N, const BKArgs &b, const double *Aval,
+                                        const double *rhs, const double *y, double *x,
+                                        double *xlo, double *res, double *d, char *fac,
+                                        NUPPN_LDS char *lds, ZoneStats &st) {
+        const BandFactors<T> F = band_factors<T>(fac, b.foff);
+        const BandLds<T> L = make_band_lds<T>(lds, b.B.n, b.D.W);
+        st.n_factor++;
+        if (sizeof(T) == 8) st.n_factor_fp64++;
+        if (band_factor<T>(b.B, b.D, Aval, F, L)) return sizeof(T) == 8 ? -4 : 1;
+        return refine_loop<sizeof(T) == 8, TM>(
+            N, b, rhs, y, x, xlo, res, d, fac, L.red, st,
+            [&](const double *r, double *dd) __attribute__((always_inline)) {
+                band_solve<T>(b.B, b.D, F, L, r, dd);
+            });
+    }
+    __device__ __forceinline__ static NUPPN_LDS double *red(const BKArgs &b, NUPPN_LDS char *lds) {
+        return make_band_lds<double>(lds, b.B.n, b.D.W).red;
+    }
+};
+
+template <class FORM, class TM>
+__device__ __forceinline__ void zone_newton_form(const KArgs &a, const BKArgs &b, char *slot,
+                                                 NUPPN_LDS char *lds, int z) {
     const DevNet &N = a.N;
     const DevKnobs &K = a.K;
     const ZoneData &Z = a.Z;
@@ -349,7 +420,7 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     double *res = (double *) (slot + b.off[B_RES]);
     double *d = (double *) (slot + b.off[B_D]);
     const double dt = s.dttry;
-    NUPPN_LDS double *red = make_block_lds<double, NB>(lds, b.B.nip).red;
+    NUPPN_LDS double *red = FORM::red(b, lds);
     const int t0 = TM::tid();
     const DynWork DW{(int *) (slot + b.off[B_FIRST]), (unsigned char *) (slot + b.off[B_RFLAG]),
                      (unsigned char *) (slot + b.off[B_EFLAG])};
@@ -359,13 +430,13 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
                                         (double *) (slot + b.off[B_APHYS]), DW, s.st.nvar1, red);
     int rc = stop ? 2 : 1;
     if (!stop && s.pad0 == 0) {
-        rc = block_refined_solve<float, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
-                                                lds, s.st);
+        rc = FORM::template refined_solve<float, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d,
+                                                     slot, lds, s.st);
         if (rc == 1) s.pad0 = 1;
     }
     if (rc == 1)
-        rc = block_refined_solve<double, NB, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d, slot,
-                                                 lds, s.st);
+        rc = FORM::template refined_solve<double, TM>(N, b, Aval, rhs, y_cur, x, xlo, res, d,
+                                                      slot, lds, s.st);
     if (rc == 0) {
         // y_new = y_cur + (x + xlo), rounded once
         for (int r = t0; r < N.n; r += TM::NT) {
@@ -390,6 +461,12 @@ __device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &
     s.phase = PH_AFTER_NEWTON;
     if (t0 == 0) Z.state[z] = s;
     TM::sync();
+}
+
+template <int NB, class TM>
+__device__ __forceinline__ void zone_newton_block(const KArgs &a, const BKArgs &b, char *slot,
+                                                  NUPPN_LDS char *lds, int z) {
+    zone_newton_form<BlockForm<NB>, TM>(a, b, slot, lds, z);
 }
 
 // The batched path's solver for the Bader-Deuflhard step: J once per step by the chunked sums
@@ -533,6 +610,17 @@ __global__ void __launch_bounds__(64 * NW, 1) zone_newton_team_kernel(KArgs a, B
     const int nact = list_len(a.Z.nact_dev, a.Z.nact);
     for (int i = blockIdx.x; i < nact; i += gridDim.x)
         zone_newton_block<NB, Team<NW>>(a, b, slot, lds, a.Z.active[i]);
+}
+
+// The band form's iteration: one workgroup of four wavefronts per zone (the factorisation's
+// 16 x 16 thread grid), the zones of the list strided over the grid.
+__global__ void __launch_bounds__(256, 1) zone_newton_band_kernel(KArgs a, BKArgs b) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
+    char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;
+    const int nact = list_len(a.Z.nact_dev, a.Z.nact);
+    for (int i = blockIdx.x; i < nact; i += gridDim.x)
+        zone_newton_form<BandForm, BandTeam>(a, b, slot, lds, a.Z.active[i]);
 }
 
 // every zone in PH_NEWTON under integration_method 1: one Bader-Deuflhard step each (a kernel

@@ -315,6 +315,10 @@ struct nuppn_ctx {
     int *d_b_vpos = nullptr, *d_b_diagk = nullptr, *d_b_ent_ptr = nullptr, *d_b_ent_k = nullptr;
     int *d_b_ent_loc = nullptr, *d_b_ent_col = nullptr, *d_b_csr_col = nullptr, *d_b_ent_dst = nullptr;
     int *d_b_csr_row = nullptr, *d_b_dyn_k = nullptr;
+    // the band form's entering lists (band_lu.hpp)
+    int *d_bd_ent_ptr = nullptr, *d_bd_ent_k = nullptr, *d_bd_ent_dst = nullptr;
+    int *d_bd_ent_col = nullptr;
+    double *d_bd_ent_ys = nullptr;
     unsigned char *d_b_isdiag = nullptr;
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
     int *d_seg_i[12] = {};        // jac, dxdt, resid: e_idx, lane_len, lane_s0, round_base
@@ -378,7 +382,8 @@ nuppn_ctx::~nuppn_ctx() {
                     (void *) d_b_isdiag, (void *) d_b_mass, (void *) d_b_ent_ys, (void *) d_bslots,
                     (void *) d_b_ent_dst, (void *) d_jac_lo, (void *) d_rcache, (void *) d_rc_t9,
                     (void *) d_b_resid_pos, (void *) d_dyn_q, (void *) d_b_csr_row,
-                    (void *) d_b_dyn_k})
+                    (void *) d_b_dyn_k, (void *) d_bd_ent_ptr, (void *) d_bd_ent_k,
+                    (void *) d_bd_ent_dst, (void *) d_bd_ent_col, (void *) d_bd_ent_ys})
         if (p) (void) hipFree(p);
     for (void *p : {(void *) d_pf_t9, (void *) d_pf_tab, (void *) d_g_spin, (void *) d_mass_g,
                     (void *) d_q_mev, (void *) d_rev_ch, (void *) d_rev_part, (void *) d_db_fwd,
@@ -524,6 +529,24 @@ static DevBlock dev_block(const nuppn_ctx *c) {
     return B;
 }
 
+// the band form's structure (ni = 0 for the block form)
+static DevBand dev_band(const nuppn_ctx *c) {
+    DevBand D{};
+    const BlockSym &b = c->bsym;
+    if (c->bnb != kBandForm) return D;
+    D.ni = b.ni;
+    D.nbd = b.nbd;
+    D.bw = b.bandwidth;
+    D.W = b.band_w;
+    D.WT = b.band_w + b.nbd;
+    D.ent_ptr = c->d_bd_ent_ptr;
+    D.ent_k = c->d_bd_ent_k;
+    D.ent_dst = c->d_bd_ent_dst;
+    D.ent_col = c->d_bd_ent_col;
+    D.ent_ys = c->d_bd_ent_ys;
+    return D;
+}
+
 // The batched path's Bader-Deuflhard step is switched off: its kernel took a memory aperture
 // violation on the GPU after the segmented sums moved to the streamed form, with no cause found
 // (the same sums run in the backward-Euler kernel, and a CPU emulation of the BD kernel under
@@ -585,7 +608,10 @@ static void free_batched(nuppn_ctx *c) {
                      (void **) &c->d_b_csr_col, (void **) &c->d_b_isdiag, (void **) &c->d_b_mass,
                      (void **) &c->d_b_ent_ys, (void **) &c->d_b_ent_dst, (void **) &c->d_jac_lo,
                      (void **) &c->d_b_resid_pos, (void **) &c->d_bslots,
-                     (void **) &c->d_b_csr_row, (void **) &c->d_b_dyn_k}) {
+                     (void **) &c->d_b_csr_row, (void **) &c->d_b_dyn_k,
+                     (void **) &c->d_bd_ent_ptr, (void **) &c->d_bd_ent_k,
+                     (void **) &c->d_bd_ent_dst, (void **) &c->d_bd_ent_col,
+                     (void **) &c->d_bd_ent_ys}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -607,9 +633,16 @@ static void free_batched(nuppn_ctx *c) {
 static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
     const int nb = force_nb ? force_nb : choose_block_size(c->net, c->sym);
     if (nb == 0) return 0;   // the batched path stays unavailable; the plain path is unaffected
+    const bool band = nb == kBandForm;
     c->bsym = BlockSym();
-    if (build_block_symbolic(c->net, c->sym, nb, c->bsym) != 0 || c->bsym.nbd > 8) return 0;
+    if (build_block_symbolic(c->net, c->sym, band ? 0 : nb, c->bsym) != 0 || c->bsym.nbd > 8)
+        return 0;
     const BlockSym &b = c->bsym;
+    if (band && (dev_upload(&c->d_bd_ent_ptr, b.bent_ptr) || dev_upload(&c->d_bd_ent_k, b.bent_k) ||
+                 dev_upload(&c->d_bd_ent_dst, b.bent_dst) ||
+                 dev_upload(&c->d_bd_ent_col, b.bent_col) ||
+                 dev_upload(&c->d_bd_ent_ys, b.bent_ys)))
+        return NUPPN_ERR_HIP;
     if (dev_upload(&c->d_b_vpos, b.vpos) || dev_upload(&c->d_b_diagk, b.diagk) ||
         dev_upload(&c->d_b_ent_ptr, b.ent_ptr) || dev_upload(&c->d_b_ent_k, b.ent_k) ||
         dev_upload(&c->d_b_ent_loc, b.ent_loc) || dev_upload(&c->d_b_ent_col, b.ent_col) ||
@@ -629,6 +662,38 @@ static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
     }
     if (dev_upload(&c->d_jac_lo, b.jac.e_coef_lo) || dev_upload(&c->d_b_resid_pos, b.resid_pos))
         return NUPPN_ERR_HIP;
+    if (band) {
+        // one workgroup of four wavefronts per zone, the factorisation's window in its registers
+        static_assert(kBandWindowMax == kBandMaxWT, "band window limits differ");
+        c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, 1, 0, b.ni,
+                                  b.nbd, b.resid.round_base[kSegRounds],
+                                  (size_t) b.ni * b.bandwidth, (size_t) b.ni * b.band_w);
+        c->blds64 = band_lds_bytes(c->sym.n, b.band_w, b.nbd, 8);
+        c->blds32 = band_lds_bytes(c->sym.n, b.band_w, b.nbd, 4);
+        if (c->blds64 > 160 * 1024) return 0;
+        const void *k = (const void *) zone_newton_band_kernel;
+        if (ensure_stack(k)) return NUPPN_ERR_HIP;
+        if (c->blds64 > 64 * 1024 &&
+            hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int) c->blds64) != hipSuccess)
+            return 0;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, c->blds64) !=
+                hipSuccess || per_cu <= 0)
+            return 0;
+        int dev = 0, ncu = 0;
+        (void) hipGetDevice(&dev);
+        (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        size_t free_b = 0, total_b = 0;
+        (void) hipMemGetInfo(&free_b, &total_b);
+        const int by_mem = (int) std::max<size_t>(1, (free_b / 4) / c->blayout.bytes);
+        c->bmax_slots = std::max(1, std::min(ncu * per_cu, by_mem));
+        c->bgrid[1] = c->bgrid[2] = c->bmax_slots;
+        c->ncu = std::max(1, ncu);
+        c->team_ok[2] = c->team_ok[4] = 0;
+        c->bnb = kBandForm;
+        return 0;
+    }
     c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, nb, b.nblk, b.nip,
                               b.nbd, b.resid.round_base[kSegRounds]);
     auto rnd = [](size_t x) { return (x + 255) & ~(size_t) 255; };
@@ -1269,6 +1334,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     memset(&b, 0, sizeof(b));
     if (block) {
         b.B = dev_block(c);
+        b.D = dev_band(c);
         b.O.fp64_factors = bo->factor_precision == 64;
         b.O.max_refine = bo->max_refine;
         b.O.refine_tol = bo->refine_tol;
@@ -1373,6 +1439,10 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
             void *args[] = {&ka};
             e = kernel(bd ? (const void *) zone_bd_kernel : (const void *) zone_newton_kernel,
                        dim3(nl), dim3(64), args, c->lds);
+        } else if (c->bnb == kBandForm) {
+            void *args[] = {&ka, &b};
+            e = kernel((const void *) zone_newton_band_kernel,
+                       dim3(std::max(1, std::min(nl, c->bgrid[1]))), dim3(256), args, c->blds64);
         } else {
             void *args[] = {&ka, &b};
             // few zones: several wavefronts per zone shorten each one's iteration (four when the
@@ -1759,7 +1829,8 @@ int nuppn_autotune(nuppn_ctx *c, int nz, const double *yps, const double *t9_0,
     if (!c || nz <= 0 || !yps || !t9_0 || !t9_1 || !rho_0 || !rho_1 || !dtwant || !res)
         return NUPPN_ERR_ARG;
     std::lock_guard<std::mutex> g(c->mu);
-    if (!c->bnb || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
+    // (the band form has one shape: nothing to choose)
+    if (!c->bnb || c->bnb == kBandForm || !batched_method_enabled(c)) return NUPPN_ERR_UNSUPPORTED;
     memset(res, 0, sizeof(*res));
     const TuneKey key{c->net.nsp, c->net.nre, c->sym.nnzA, c->ncu};
     int nb = 0, tz = 0, tw = 0;
@@ -1857,6 +1928,24 @@ int nuppn_autotune(nuppn_ctx *c, int nz, const double *yps, const double *t9_0,
             fclose(f);
         }
     }
+    return NUPPN_OK;
+}
+
+int nuppn_set_block_form(nuppn_ctx *c, int nb) {
+    if (!c || (nb != 32 && nb != 64 && nb != kBandForm)) return NUPPN_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    BlockSym probe;
+    if (build_block_symbolic(c->net, c->sym, nb == kBandForm ? 0 : nb, probe) != 0)
+        return NUPPN_ERR_UNSUPPORTED;
+    if (nb == c->bnb) return NUPPN_OK;
+    const int nb0 = c->bnb;
+    free_batched(c);
+    if (setup_batched(c, nb) != 0 || c->bnb != nb) {
+        free_batched(c);
+        if (nb0) (void) setup_batched(c, nb0);
+        return NUPPN_ERR_HIP;
+    }
+    c->tuned = 0;
     return NUPPN_OK;
 }
 

@@ -313,9 +313,18 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
     b.pos.assign(n, 0);
     for (int p = 0; p < n; ++p) b.pos[b.ord[p]] = p;
     b.bandwidth = interior_bandwidth(s, b.pos, b.ni);
-    if (b.bandwidth > NB || b.ni == 0) return -1;
-    b.nblk = (b.ni + NB - 1) / NB;
-    b.nip = b.nblk * NB;
+    if (b.ni == 0) return -1;
+    if (NB == 0) {
+        // the band form: window and border within the factorisation's thread grid
+        b.band_w = b.bandwidth + 1;
+        if (b.band_w + b.nbd > kBandWindowMax) return -1;
+        b.nblk = 0;
+        b.nip = b.ni;
+    } else {
+        if (b.bandwidth > NB) return -1;
+        b.nblk = (b.ni + NB - 1) / NB;
+        b.nip = b.nblk * NB;
+    }
     b.mass.resize(n);
     for (int i = 0; i < n; ++i) b.mass[i] = net.aw[s.red2sp[i]];
     b.diagq.assign(n, -1);
@@ -325,10 +334,50 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
     for (int c = 0; c < n; ++c)
         if (b.diagq[c] < 0 || b.mass[c] <= 0.0) return -2;
 
-    const int nblk = b.nblk, nlists = 3 * nblk - 2 + 3;
+    if (NB == 0) {
+        // entering lists of the band window (band_lu.hpp): slot of interior position i is
+        // i mod W, of border b W + b
+        const int W = b.band_w, WT = W + b.nbd, ni = b.ni;
+        std::vector<std::vector<int>> lq(ni + 1);
+        for (int c = 0; c < n; ++c)
+            for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
+                const int pr = b.pos[s.A_rowind[p]], pc = b.pos[c];
+                int li;
+                if (pr < ni && pc < ni) li = std::max(pr, pc) + 1;
+                else if (pr < ni) li = pr + 1;
+                else if (pc < ni) li = pc + 1;
+                else li = 0;
+                lq[li].push_back(p);
+            }
+        auto slot = [&](int ps) { return ps < ni ? ps % W : W + (ps - ni); };
+        b.bent_ptr.assign(1, 0);
+        b.bent_q.clear();
+        b.bent_dst.clear();
+        b.bent_col.clear();
+        b.bent_ys.clear();
+        std::vector<int> colof(s.nnzA);
+        for (int c = 0; c < n; ++c)
+            for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) colof[p] = c;
+        for (int li = 0; li <= ni; ++li) {
+            for (int p : lq[li]) {
+                const int r = s.A_rowind[p], c = colof[p];
+                b.bent_q.push_back(p);
+                b.bent_dst.push_back(slot(b.pos[r]) * WT + slot(b.pos[c]));
+                b.bent_col.push_back(c);
+                b.bent_ys.push_back(b.mass[c] / b.mass[r]);
+            }
+            b.bent_ptr.push_back((int) b.bent_q.size());
+        }
+        b.ent_ptr.assign(1, 0);
+        b.ent_q.clear();
+        b.ent_loc.clear();
+        b.ent_col.clear();
+        b.ent_ys.clear();
+    }
+    const int nblk = b.nblk, nlists = NB == 0 ? 0 : 3 * nblk - 2 + 3;
     const int LE = 3 * nblk - 2, LF = LE + 1, LG = LE + 2;
     std::vector<std::vector<int>> q(nlists), loc(nlists);
-    for (int c = 0; c < n; ++c)
+    for (int c = 0; c < n && NB > 0; ++c)
         for (int p = s.A_colptr[c]; p < s.A_colptr[c + 1]; ++p) {
             const int r = s.A_rowind[p];
             const int pr = b.pos[r], pc = b.pos[c];
@@ -353,6 +402,7 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
             q[list].push_back(p);
             loc[list].push_back(l);
         }
+    if (NB > 0) {
     b.ent_ptr.assign(1, 0);
     b.ent_q.clear();
     b.ent_loc.clear();
@@ -370,6 +420,7 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
             b.ent_ys.push_back(b.mass[c] / b.mass[r]);
         }
         b.ent_ptr.push_back((int) b.ent_q.size());
+    }
     }
     b.vpos.assign(n, 0);
     for (int i = 0; i < n; ++i) b.vpos[i] = b.pos[i] < b.ni ? b.pos[i] : b.nip + b.pos[i] - b.ni;
@@ -467,8 +518,10 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
     for (int k = 0; k < nnz; ++k) q2k[b.csr_q[k]] = k;
     b.ent_k.resize(b.ent_q.size());
     for (size_t e = 0; e < b.ent_q.size(); ++e) b.ent_k[e] = q2k[b.ent_q[e]];
+    b.bent_k.resize(b.bent_q.size());
+    for (size_t e = 0; e < b.bent_q.size(); ++e) b.bent_k[e] = q2k[b.bent_q[e]];
     const int nblk = b.nblk, BS = b.NB * b.NB;
-    b.ent_dst.assign(b.ent_ptr[3 * nblk - 2], 0);
+    b.ent_dst.assign(nblk > 0 ? b.ent_ptr[3 * nblk - 2] : 0, 0);
     for (int L = 0; L < 3 * nblk - 2; ++L) {
         const int area = L < nblk ? 0 : (L < 2 * nblk - 1 ? 1 : 2);
         const int blk = L < nblk ? L : (L < 2 * nblk - 1 ? L - nblk : L - (2 * nblk - 1));
@@ -538,9 +591,9 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
 }
 
 int choose_block_size(const HostNetwork &net, const Symbolic &s) {
-    for (int NB : {32, 64}) {
+    for (int NB : {32, 64, 0}) {
         BlockSym b;
-        if (build_block_symbolic(net, s, NB, b) == 0) return NB;
+        if (build_block_symbolic(net, s, NB, b) == 0) return NB == 0 ? kBandForm : NB;
     }
     return 0;
 }

@@ -92,6 +92,17 @@ struct BlockSym {
     std::vector<int> ent_k, diagk;
     // the dynamic network's entry map (Symbolic::dyn_q) in CSR order, and each CSR entry's row
     std::vector<int> dyn_k, csr_row;
+    // the band form (NB = 0: an interior too wide for blocks, band_lu.hpp): window W =
+    // bandwidth + 1; the entries entering the factorisation window, list 0 the border-border
+    // block, list r + 1 those of interior index r (the larger of the entry's interior positions,
+    // or its interior position when the other index is on the border)
+    int band_w = 0;
+    std::vector<int> bent_ptr;    // [ni + 2]
+    std::vector<int> bent_q;      // CSC entry index
+    std::vector<int> bent_k;      // the same as a CSR position
+    std::vector<int> bent_dst;    // window slot, row * (W + nbd) + col
+    std::vector<int> bent_col;    // reduced column
+    std::vector<double> bent_ys;  // mass(col) / mass(row)
     // D/C/B entries: destination in the dense block storage [area][block][NB*NB] (area 0 D_i,
     // 1 C_i, 2 B_i; column-major inside a block)
     std::vector<int> ent_dst;
@@ -104,12 +115,20 @@ struct BlockSym {
     std::vector<int> resid_pos;   // [nnzA] physical position of each CSR entry in the residual stream
 };
 
-// Builds the block structure for block size NB (32 or 64); returns 0, or -1 when the interior
-// half-bandwidth exceeds NB (the block-tridiagonal form does not hold).
+// Builds the block structure for block size NB (32 or 64), or the band form for NB = 0;
+// returns 0, or -1 when the interior half-bandwidth exceeds NB (the block-tridiagonal form does
+// not hold) or, for the band form, the window and border exceed kBandWindowMax.
 int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, BlockSym &b);
 
 // smallest supported block size (32, 64) for which the network's interior is block tridiagonal;
 // 0 if none
+// the band form's limit on window + border (its factorisation's 16 x 16 thread grid holds
+// 8 x 8 slots per thread) and choose_block_size's answer for that form
+constexpr int kBandWindowMax = 128;
+constexpr int kBandForm = -1;
+
+// the block size of the batched path: 32 or 64 where the interior is block tridiagonal with
+// that size, else kBandForm where the band form fits, else 0 (no batched path)
 int choose_block_size(const HostNetwork &net, const Symbolic &s);
 
 // COLAMD (v2.3 semantics, SuperLU_5.0/SRC/colamd.c) on an m x n CSC pattern, default knobs;

@@ -90,7 +90,8 @@ module nuppn_hip
        nuppn_rates_eval, nuppn_set_rate_cache, nuppn_rate_cache_table, nuppn_dxdt, &
        nuppn_jacobian_reduced, nuppn_lu_solve_reduced, nuppn_integrate_zone, &
        nuppn_integrate_batch, nuppn_integrate_batch_dev, nuppn_default_batch_opts, &
-       nuppn_batch_block_info, nuppn_integrate_batch_mixed, nuppn_integrate_batch_mixed_dev, &
+       nuppn_batch_block_info, nuppn_set_block_form, nuppn_integrate_batch_mixed, &
+       nuppn_integrate_batch_mixed_dev, &
        nuppn_last_timing, slu_solve, slu_solve_noalloc, nuppn_default_mppnp_opts, &
        nuppn_mppnp_cycle, nuppn_mppnp_cycle_dev, nuppn_set_rate_physics, nuppn_reverse_pairs, &
        nuppn_evaluate_all_rates, nuppn_autotune
@@ -241,6 +242,12 @@ module nuppn_hip
        type(c_ptr), value          :: ctx
        integer(c_int), intent(out) :: info(*)
        integer(c_int), value       :: ninfo
+     end function
+
+     integer(c_int) function nuppn_set_block_form(ctx, nb) bind(c, name="nuppn_set_block_form")
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: nb
      end function
 
      integer(c_int) function nuppn_integrate_batch_mixed(ctx, nz, yps, t9_0, t9_1, rho_0, &

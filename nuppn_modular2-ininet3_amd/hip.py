@@ -140,6 +140,7 @@ def load():
                                          C.c_void_p, C.c_void_p, C.c_void_p])
     L.nuppn_default_batch_opts.argtypes = [C.POINTER(BatchOpts)]
     L.nuppn_batch_block_info.argtypes = [C.c_void_p, _ip, C.c_int]
+    L.nuppn_set_block_form.argtypes = [C.c_void_p, C.c_int]
     L.nuppn_integrate_batch_mixed.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                               _dp, _ip, C.c_void_p, C.POINTER(BatchOpts)]
     L.nuppn_integrate_batch_mixed_dev.argtypes = ([C.c_void_p, C.c_int] + [C.c_void_p] * 9 +
@@ -381,6 +382,11 @@ class Context:
                                               dtwant, ierr, stats, stream)
         if rc != OK:
             raise RuntimeError(f"nuppn_integrate_batch_dev failed ({rc})")
+
+    def set_block_form(self, nb):
+        """rebuilds the batched path as block size 32 / 64 or the band form (-1)
+        (nuppn_set_block_form); returns the status code."""
+        return load().nuppn_set_block_form(self.h, int(nb))
 
     def block_info(self):
         """dict of the batched path's block structure (nuppn_batch_block_info)."""

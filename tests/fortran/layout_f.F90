@@ -12,7 +12,7 @@ program layout_f
   type(nuppn_mppnp_opts), target :: q
   type(nuppn_rate_physics), target :: rp
   type(nuppn_tune_result), target :: tr
-  type(c_funptr) :: f(31)
+  type(c_funptr) :: f(32)
   integer(c_intptr_t) :: b
 
   b = transfer(c_loc(n), b)
@@ -81,7 +81,8 @@ program layout_f
   f(29) = c_funloc(nuppn_reverse_pairs)
   f(30) = c_funloc(nuppn_evaluate_all_rates)
   f(31) = c_funloc(nuppn_autotune)
-  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 31)])
+  f(32) = c_funloc(nuppn_set_block_form)
+  write(*, '(a, 1x, i0)') 'bound_procedures', count([(c_associated(f(b)), b = 1, 32)])
 contains
   integer(c_intptr_t) function off(p)
     type(c_ptr), intent(in) :: p

@@ -280,18 +280,3 @@ def test_c_fortran_dgssv_dropin(hip, oracle, c1):
         assert np.all(b[n:ldb] == 0)
         info, h3 = hip.c_fortran_dgssv(3, n, int(cp[n]), 2, vals, rif, cpf, b, ldb, h)
         assert info == 0 and h3 == 0
-
-
-def test_c3_batched_path_unavailable(c3, hip):
-    """The 5,000-species network's interior coupling (alpha captures two Z rows of up to 51
-    species apart: band 102) exceeds the 64-species blocks, so the batched path reports itself
-    unavailable and nuppn_integrate_batch_mixed refuses with NUPPN_ERR_UNSUPPORTED, leaving the
-    abundances untouched; the plain path serves the network."""
-    w, ctx, o = c3
-    assert not ctx.block_info()["available"]
-    sel = np.array([0])
-    y0 = w.x0[sel].copy()
-    with pytest.raises(RuntimeError, match=str(hip.ERR_UNSUPPORTED)):
-        ctx.integrate_batch_mixed(y0, *[a[sel] for a in (w.t9[:, 0], w.t9[:, 1], w.rho[:, 0],
-                                                            w.rho[:, 1], w.dt)])
-    np.testing.assert_array_equal(y0, w.x0[sel])
