@@ -48,6 +48,9 @@ def _args():
                         "abundances carry over from cycle to cycle); the FP64-factor "
                         "comparison runs on the first cycle only")
     p.add_argument("--save", default="", help="write the step's abundances and counters (npz)")
+    p.add_argument("--large-only", type=int, default=0, metavar="ZONES",
+                   help="run only the 5,000-species network's batched cycle on its first ZONES "
+                        "zones and print its figures (a profiling workload)")
     return p.parse_args()
 
 
@@ -148,6 +151,47 @@ def _cpu_zone(z):
     return ierr, time.perf_counter() - t
 
 
+def large_network(pkg, hip, plain=True, zones=None):
+    """The large network: 256 zones x 5,000 species (34,055 reactions) on the batched path's band
+    form, one cycle, and the plain FP64 path on four of its cool zones."""
+    w3 = pkg.network.config(3)
+    ctx3 = hip.Context(w3.net)
+    info3 = ctx3.block_info()
+    n3 = w3.x0.shape[0] if zones is None else zones
+    a3 = [v[:n3] for v in (w3.t9[:, 0], w3.t9[:, 1], w3.rho[:, 0], w3.rho[:, 1], w3.dt)]
+    ctx3.integrate_batch_mixed(w3.x0[:8], *[v[:8] for v in a3])   # warm-up
+    t0 = time.perf_counter()
+    ie3, y3, _, st3 = ctx3.integrate_batch_mixed(w3.x0[:n3], *a3)
+    el3 = time.perf_counter() - t0
+    tm3 = ctx3.last_timing()
+    out = {
+        "zones": n3, "species": w3.net.nsp, "reactions": w3.net.nre,
+        "form": "band (half-bandwidth %d, border %d)" % (info3["bandwidth"], info3["nbd"]),
+        "zone_timesteps_per_s_batched": n3 / el3, "cycle_ms": 1e3 * el3,
+        "newton_ms": tm3["newton_ms"], "passes": tm3["passes"],
+        "newton_iterations": int(st3["n_newton"].sum()),
+        "max_newton_per_zone": int(st3["n_newton"].max()),
+        "fp64_factorisations": int(st3["n_factor_fp64"].sum()),
+        "factorisations": int(st3["n_factor"].sum()),
+        "solves": int(st3["n_solve"].sum()),
+        "ierr_nonzero": int((ie3 != 0).sum()),
+        "max_abs_sumX_minus_1": float(np.abs(y3.sum(axis=1) - 1.0).max()),
+        "what": "256 zones x 5,000 species, T9 0.1..2, rho 1e2..1e6, dt 1e4 s"}
+    log(f"large network: {el3 * 1e3:.1f} ms for {n3} zones, {tm3['passes']:.0f} passes")
+    if plain:
+        sel3 = np.array([5, 6, 7, 10])
+        t0 = time.perf_counter()
+        ie3p, y3p, _, st3p = ctx3.integrate_batch(w3.x0[sel3], *[v[sel3] for v in a3])
+        el3p = time.perf_counter() - t0
+        out["plain_sample"] = {"zones": sel3.tolist(), "seconds": el3p,
+                               "zone_timesteps_per_s": len(sel3) / el3p,
+                               "newton_iterations": int(st3p["n_newton"].sum()),
+                               "ierr_nonzero": int((ie3p != 0).sum())}
+        log(f"large network plain sample: {el3p:.1f} s")
+    ctx3.close()
+    return out
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -155,6 +199,11 @@ def main():
               f"zones across GPUs (one contiguous block of zones per GPU), which this one-GPU "
               f"build does not run", file=sys.stderr)
         sys.exit(2)
+    if a.large_only:
+        from __graft_entry__ import _load_package
+        pkg = _load_package()
+        print(json.dumps(large_network(pkg, pkg.hip, plain=False, zones=a.large_only)))
+        return
     # the CPU baseline first: no GPU state exists yet in this process or its workers
     cpu = None
     cpu_one = None
@@ -382,6 +431,26 @@ def main():
                                           "reference's defaults); the screened rates make the "
                                           "zones stiffer"}}
         log(f"mppnp cycle: {el_m * 1e3:.1f} ms; rate physics cycle {el_p * 1e3:.1f} ms")
+        # the same 2,048-zone cycle under the dynamic network (cyminformal 1e-30, the
+        # reference's default): the mask and reduced entries recomputed every iteration
+        ctx_d = hip.Context(net, hip.default_knobs(cyminformal=1.0e-30, min_rate_ov_dt=1.0e-99))
+        yd = x0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx_d.integrate_batch_mixed_dev(nz, yd.data_ptr(), *(t.data_ptr() for t in (t9_0, t9_1,
+                                        r0, r1)), ye.data_ptr(), dt.data_ptr(), ierr.data_ptr(),
+                                        stats.data_ptr())
+        torch.cuda.synchronize()
+        el_d = time.perf_counter() - t0
+        std = stats.cpu().numpy().view(hip.STATS_DTYPE).reshape(-1)
+        extra["dynamic_network"] = {
+            "zone_timesteps_per_s": nz / el_d, "ierr_nonzero": int((ierr.cpu().numpy() != 0).sum()),
+            "nvar1_min_median_max": [int(std["nvar1"].min()), int(np.median(std["nvar1"])),
+                                     int(std["nvar1"].max())],
+            "what": "the 2,048-zone cycle with cyminformal = 1e-30, min_rate_ov_dt = 1e-99"}
+        ctx_d.close()
+        log(f"dynamic network cycle {el_d * 1e3:.1f} ms")
+        extra["large_network"] = large_network(pkg, hip, plain=True)
 
     if a.save:
         np.savez(a.save, y_mixed=y_mixed, y_fp64=y64, nsubt_mixed=st_mixed["nsubt"],

@@ -32,6 +32,8 @@ constexpr int kBandTG = 16;                        // thread grid side: 16 x 16 
 constexpr int kBandR = 8;                          // window rows (and columns) per thread
 constexpr int kBandMaxWT = kBandTG * kBandR;       // 128: window + border the grid holds
 constexpr int kBandChunk = 32;                     // sweep columns staged per LDS buffer
+constexpr int kBandEnterChunk = 16;                // entering lists prefetched per LDS bank
+constexpr int kBandEnterCap = 512;                 // entries one bank holds
 using BandTeam = Team<4>;
 
 struct DevBand {
@@ -68,6 +70,9 @@ struct BandLds {
     NUPPN_LDS T *v;          // [n]
     NUPPN_LDS T *stage;      // [2][kBandChunk * W]
     NUPPN_LDS double *tb;    // [nbd]
+    NUPPN_LDS T *eval;       // [2][kBandEnterCap] prefetched entering values
+    NUPPN_LDS int *edst;     // [2][kBandEnterCap] their window slots
+    NUPPN_LDS int *eoff;     // [2][kBandEnterChunk + 1] list starts within the bank
 };
 
 __host__ __device__ inline size_t band_lds_bytes(int n, int W, int nbd, size_t tsz) {
@@ -77,7 +82,10 @@ __host__ __device__ inline size_t band_lds_bytes(int n, int W, int nbd, size_t t
     o += (size_t) n * tsz;
     o = (o + 15) & ~(size_t) 15;
     o += 2 * (size_t) kBandChunk * W * tsz;
-    o += 8 * (size_t) (nbd + 1);
+    o += 8 * (size_t) (kBandMaxWT / kBandTG);   // nbd <= 8
+    (void) nbd;
+    o = (o + 15) & ~(size_t) 15;
+    o += 2 * (size_t) kBandEnterCap * (tsz + 4) + 8 * (kBandEnterChunk + 1);
     return (o + 255) & ~(size_t) 255;
 }
 
@@ -99,8 +107,20 @@ __device__ inline BandLds<T> make_band_lds(NUPPN_LDS char *base, int n, int W) {
     L.stage = (NUPPN_LDS T *) (base + o);
     o += 2 * (size_t) kBandChunk * W * sizeof(T);
     L.tb = (NUPPN_LDS double *) (base + o);
+    o += 8 * (size_t) (kBandMaxWT / kBandTG);   // nbd <= 8
+    o = (o + 15) & ~(size_t) 15;
+    L.eval = (NUPPN_LDS T *) (base + o);
+    o += 2 * (size_t) kBandEnterCap * sizeof(T);
+    L.edst = (NUPPN_LDS int *) (base + o);
+    o += 2 * (size_t) kBandEnterCap * 4;
+    L.eoff = (NUPPN_LDS int *) (base + o);
     return L;
 }
+
+__device__ __forceinline__ float readlane_t(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_t(double v, int l) { return readlane_d(v, l); }
 
 // Factors of the assembled matrix (A in CSR order) in precision T.  Returns 0, or 1 when a pivot
 // or the border Schur complement is not finite or zero.  All 256 threads of the zone call it.
@@ -129,13 +149,33 @@ __device__ __forceinline__ int band_factor(const DevBlock &B, const DevBand &D, 
     const NUPPN_GLB int *eptr = gptr(D.ent_ptr), *ek = gptr(D.ent_k), *edst = gptr(D.ent_dst);
     const NUPPN_GLB int *ecolx = gptr(D.ent_col);
     const NUPPN_GLB double *eys = gptr(D.ent_ys);
-    // the entries of list li into the entering buffers of parity p (row part: slot row sr)
-    auto scatter = [&](int li, int sr, int p) __attribute__((always_inline)) {
-        for (int e = eptr[li] + t; e < eptr[li + 1]; e += TM::NT) {
-            const T val = (T) (A[ek[e]] * eys[e] * cs[ecolx[e]]);
-            const int dst = edst[e], r = dst / WT, c = dst - r * WT;
-            if (r == sr) L.erow[p * kBandMaxWT + c] = val;
-            else L.ecol[p * kBandMaxWT + r] = val;
+    // The entering entries, index after index (lists 1..ni), are prefetched into LDS by groups
+    // of kBandEnterChunk lists, with their values already scaled, one group ahead: a step then
+    // reads its entries from LDS instead of waiting for three dependent global loads.
+    auto prefetch = [&](int grp) __attribute__((always_inline)) {
+        const int l0 = 1 + grp * kBandEnterChunk, l1 = min(l0 + kBandEnterChunk, ni + 1);
+        if (l0 > ni) return;
+        const int bank = grp & 1, e0 = eptr[l0];
+        NUPPN_LDS T *bv = L.eval + bank * kBandEnterCap;
+        NUPPN_LDS int *bd = L.edst + bank * kBandEnterCap;
+        for (int e = e0 + t; e < eptr[l1]; e += TM::NT) {
+            bv[e - e0] = (T) (A[ek[e]] * eys[e] * cs[ecolx[e]]);
+            bd[e - e0] = edst[e];
+        }
+        for (int q = t; q <= l1 - l0; q += TM::NT)
+            L.eoff[bank * (kBandEnterChunk + 1) + q] = eptr[l0 + q] - e0;
+    };
+    // the entries of interior index rin into the entering buffers of parity p (row part: the
+    // slot row sr)
+    auto scatter = [&](int rin, int sr, int p) __attribute__((always_inline)) {
+        const int grp = rin / kBandEnterChunk, q = rin - grp * kBandEnterChunk, bank = grp & 1;
+        const NUPPN_LDS int *off = L.eoff + bank * (kBandEnterChunk + 1);
+        const NUPPN_LDS T *bv = L.eval + bank * kBandEnterCap;
+        const NUPPN_LDS int *bd = L.edst + bank * kBandEnterCap;
+        for (int e = off[q] + t; e < off[q + 1]; e += TM::NT) {
+            const int dst = bd[e], r = dst / WT, c = dst - r * WT;
+            if (r == sr) L.erow[p * kBandMaxWT + c] = bv[e];
+            else L.ecol[p * kBandMaxWT + r] = bv[e];
         }
     };
     T w[kBandR][kBandR];
@@ -164,31 +204,59 @@ __device__ __forceinline__ int band_factor(const DevBlock &B, const DevBand &D, 
         TM::sync();
     }
     // the first W interior indices enter one at a time, then every step eliminates index k and
-    // index k + W enters in its slot
+    // index k + W enters in its slot.  L's column and U's row of every step go to LDS first and
+    // reach global memory in batches of CHF steps (a barrier waits for the stores before it, so
+    // storing every step would make every step wait for its stores), from two banks in turn.
     const int nfill = min(W, ni);
     NUPPN_GLB T *Lb = gptr(F.Lb), *Uc = gptr(F.Uc), *LE = gptr(F.LE), *UF = gptr(F.UF);
+    const int rowlen = bw + W + 2 * nbd;
+    const int CHF = kBandChunk * W / rowlen;
+    auto flush = [&](int k0, int cnt) __attribute__((always_inline)) {
+        const NUPPN_LDS T *o = L.stage + (size_t) ((k0 / CHF) & 1) * CHF * rowlen;
+        for (int rr = 0; rr < cnt; ++rr) {
+            const int k = k0 + rr;
+            const NUPPN_LDS T *orow = o + (size_t) rr * rowlen;
+            for (int q = t; q < bw; q += TM::NT) Lb[(size_t) k * bw + q] = orow[q];
+            for (int q = t; q <= bw; q += TM::NT)
+                if (k + q < ni) Uc[(size_t) (k + q) * W + q] = orow[bw + q];
+            for (int b = t; b < nbd; b += TM::NT) {
+                LE[(size_t) b * ni + k] = orow[bw + W + b];
+                UF[(size_t) k * nbd + b] = orow[bw + W + nbd + b];
+            }
+        }
+    };
+    prefetch(0);
+    TM::sync();
     int fail = 0;
     for (int k = -nfill; k < ni; ++k) {
         const int p = k & 1;
         const int s = k < 0 ? k + nfill : k % W;         // slot of the pivot (or of the entry)
         const int rin = k < 0 ? k + nfill : k + W;      // index entering at this step
         const bool enter = rin < ni;
+        const int si = s / kBandTG, sm = s % kBandTG;   // register index and owner of slot s
         NUPPN_LDS T *ur = L.urow + p * kBandMaxWT, *lc = L.lcol + p * kBandMaxWT;
         NUPPN_LDS T *er = L.erow + p * kBandMaxWT, *ec = L.ecol + p * kBandMaxWT;
         if (k >= 0) {
+            if (tr == sm) {
 #pragma unroll
-            for (int i = 0; i < kBandR; ++i)
-                if (tr + kBandTG * i == s)
+                for (int i = 0; i < kBandR; ++i)
+                    if (i == si)
 #pragma unroll
-                    for (int j = 0; j < kBandR; ++j) ur[tc + kBandTG * j] = w[i][j];
+                        for (int j = 0; j < kBandR; ++j) ur[tc + kBandTG * j] = w[i][j];
+            }
+            if (tc == sm) {
 #pragma unroll
-            for (int j = 0; j < kBandR; ++j)
-                if (tc + kBandTG * j == s)
+                for (int j = 0; j < kBandR; ++j)
+                    if (j == si)
 #pragma unroll
-                    for (int i = 0; i < kBandR; ++i) lc[tr + kBandTG * i] = w[i][j];
+                        for (int i = 0; i < kBandR; ++i) lc[tr + kBandTG * i] = w[i][j];
+            }
         }
-        if (enter) scatter(rin + 1, rin % W, p);
+        if (enter) scatter(rin, rin % W, p);
         TM::sync();
+        // (group g's bank was last read before this barrier, at index g * chunk - 1)
+        if (enter && rin % kBandEnterChunk == 0) prefetch(rin / kBandEnterChunk + 1);
+        if (k > 0 && k % CHF == 0) flush(k - CHF, CHF);
         if (k >= 0) {
             const T piv = ur[s];
             if (!(isfinite((double) piv) && piv != T(0))) {   // uniform: every thread reads it
@@ -196,67 +264,68 @@ __device__ __forceinline__ int band_factor(const DevBlock &B, const DevBand &D, 
                 break;
             }
             const T inv = T(1) / piv;
+            // the rank-1 update of every slot (the pivot row and column are recycled below)
             T l[kBandR], u[kBandR];
 #pragma unroll
-            for (int i = 0; i < kBandR; ++i) {
-                const int r = tr + kBandTG * i;
-                l[i] = r == s ? T(0) : lc[r] * inv;
-            }
+            for (int i = 0; i < kBandR; ++i) l[i] = lc[tr + kBandTG * i] * inv;
 #pragma unroll
-            for (int j = 0; j < kBandR; ++j) {
-                const int c = tc + kBandTG * j;
-                u[j] = c == s ? T(0) : ur[c];
-            }
+            for (int j = 0; j < kBandR; ++j) u[j] = ur[tc + kBandTG * j];
 #pragma unroll
             for (int i = 0; i < kBandR; ++i)
 #pragma unroll
                 for (int j = 0; j < kBandR; ++j) w[i][j] = fma_t(-l[i], u[j], w[i][j]);
-            // L's column k, U's row k (column-band storage), the border parts
+            // L's column k, U's row k and the border parts to the output bank
+            NUPPN_LDS T *orow = L.stage + (size_t) ((k / CHF) & 1) * CHF * rowlen +
+                                (size_t) (k % CHF) * rowlen;
             for (int q = t; q < bw; q += TM::NT) {
-                const int r = k + 1 + q;
-                Lb[(size_t) k * bw + q] = r < ni ? lc[r % W] * inv : T(0);
+                int sl = s + 1 + q;
+                if (sl >= W) sl -= W;
+                orow[q] = k + 1 + q < ni ? lc[sl] * inv : T(0);
             }
             for (int q = t; q <= bw; q += TM::NT) {
-                const int j = k + q;
-                if (j < ni) Uc[(size_t) j * W + q] = ur[j % W];
+                int sl = s + q;
+                if (sl >= W) sl -= W;
+                orow[bw + q] = ur[sl];
             }
             for (int b = t; b < nbd; b += TM::NT) {
-                LE[(size_t) b * ni + k] = lc[W + b] * inv;
-                UF[(size_t) k * nbd + b] = ur[W + b];
+                orow[bw + W + b] = lc[W + b] * inv;
+                orow[bw + W + nbd + b] = ur[W + b];
             }
         }
-        // the recycled slot: zero, or the entering index's entries
-#pragma unroll
-        for (int i = 0; i < kBandR; ++i) {
-            const int r = tr + kBandTG * i;
-#pragma unroll
-            for (int j = 0; j < kBandR; ++j) {
-                const int c = tc + kBandTG * j;
-                if (r == s) {
-                    w[i][j] = enter ? er[c] : T(0);
-                } else if (c == s) {
-                    w[i][j] = enter ? ec[r] : T(0);
-                }
-            }
-        }
-        // each buffer entry has one reader (the slot's owner): it clears what it read
-        if (enter) {
+        // the recycled slot: zero, or the entering index's entries; each buffer entry has one
+        // reader (the slot's owner), which clears it
+        if (tr == sm) {
 #pragma unroll
             for (int i = 0; i < kBandR; ++i)
-                if (tr + kBandTG * i == s)
+                if (i == si)
 #pragma unroll
-                    for (int j = 0; j < kBandR; ++j) er[tc + kBandTG * j] = T(0);
+                    for (int j = 0; j < kBandR; ++j) {
+                        NUPPN_LDS T *e = er + tc + kBandTG * j;
+                        w[i][j] = enter ? *e : T(0);
+                        *e = T(0);
+                    }
+        }
+        if (tc == sm) {
 #pragma unroll
             for (int j = 0; j < kBandR; ++j)
-                if (tc + kBandTG * j == s)
+                if (j == si)
 #pragma unroll
                     for (int i = 0; i < kBandR; ++i)
-                        if (tr + kBandTG * i != s) ec[tr + kBandTG * i] = T(0);
+                        if (!(tr == sm && i == si)) {
+                            NUPPN_LDS T *e = ec + tr + kBandTG * i;
+                            w[i][j] = enter ? *e : T(0);
+                            *e = T(0);
+                        }
         }
     }
     if (fail) {
         TM::sync();
         return 1;
+    }
+    TM::sync();
+    if (ni > 0) {
+        const int k0 = (ni - 1) / CHF * CHF;
+        flush(k0, ni - k0);
     }
     // the border Schur complement to FP64, LU with partial pivoting (thread 0)
     NUPPN_GLB double *G = gptr(F.G);
@@ -335,19 +404,39 @@ __device__ __forceinline__ void band_solve(const DevBlock &B, const DevBand &D,
     };
     if (ng > 0) stage_l(0, 0, t0, NT);
     TM::sync();
+    // The sweeping wavefront keeps the W active entries in registers, lane l the window slots l
+    // and l + 64 (interior index i in slot i mod W): each step reads y_k from its owner lane,
+    // updates the other slots and recycles y_k's slot for index k + W (from LDS, still the
+    // right-hand side there), so the step-to-step chain runs through registers, not LDS.
+    const int l = lane_id();
+    T r0 = T(0), r1 = T(0);
+    if (TM::wave() == 0) {
+        if (l < W && l < ni) r0 = v[l];
+        if (l + WAVE < W && l + WAVE < ni) r1 = v[l + WAVE];
+    }
+    int s = 0;   // slot of index k
     for (int g = 0; g < ng; ++g) {
         if (g + 1 < ng && TM::wave() != 0) stage_l(g + 1, (g + 1) & 1, t0 - WAVE, NT - WAVE);
         if (TM::wave() == 0) {
             const NUPPN_LDS T *st = L.stage + (size_t) (g & 1) * CH * W;
             const int k0 = g * CH, kn = min(CH, ni - k0);
-            const int l = lane_id();
             for (int kk = 0; kk < kn; ++kk) {
                 const int k = k0 + kk;
-                const T y = v[k];
-                lds_fence();
-                for (int q = l; q < bw; q += WAVE)
-                    if (k + 1 + q < ni) v[k + 1 + q] = fma_t(-st[kk * bw + q], y, v[k + 1 + q]);
-                lds_fence();
+                int d0 = l - s, d1 = l + WAVE - s;   // slot distance ahead of the pivot
+                if (d0 < 0) d0 += W;
+                if (d1 < 0) d1 += W;
+                const T a0 = (d0 > 0 && k + d0 < ni) ? st[kk * bw + d0 - 1] : T(0);
+                const T a1 = (l + WAVE < W && d1 > 0 && k + d1 < ni) ? st[kk * bw + d1 - 1] : T(0);
+                const T y = readlane_t(s < WAVE ? r0 : r1, s & (WAVE - 1));
+                r0 = fma_t(-a0, y, r0);
+                r1 = fma_t(-a1, y, r1);
+                const T nv = k + W < ni ? v[k + W] : T(0);
+                if (l == 0) v[k] = y;
+                if (l == (s & (WAVE - 1))) {
+                    if (s < WAVE) r0 = nv;
+                    else r1 = nv;
+                }
+                if (++s == W) s = 0;
             }
         }
         TM::sync();
@@ -392,20 +481,40 @@ __device__ __forceinline__ void band_solve(const DevBlock &B, const DevBand &D,
     };
     if (ng > 0) stage_u(0, 0, t0, NT);
     TM::sync();
+    // the same register window, descending: slot i mod W holds index i of j - W + 1 .. j
+    r0 = T(0);
+    r1 = T(0);
+    if (TM::wave() == 0 && ni > 0) {
+        const int top = ni - 1, ts = top % W;
+        int dd0 = ts - l, dd1 = ts - l - WAVE;   // distance below the top index
+        if (dd0 < 0) dd0 += W;
+        if (dd1 < 0) dd1 += W;
+        if (l < W && top - dd0 >= 0) r0 = v[top - dd0];
+        if (l + WAVE < W && top - dd1 >= 0) r1 = v[top - dd1];
+    }
+    s = ni > 0 ? (ni - 1) % W : 0;   // slot of index j
     for (int g = 0; g < ng; ++g) {
         if (g + 1 < ng && TM::wave() != 0) stage_u(g + 1, (g + 1) & 1, t0 - WAVE, NT - WAVE);
         if (TM::wave() == 0) {
             const NUPPN_LDS T *st = L.stage + (size_t) (g & 1) * CH * W;
             const int j1 = ni - g * CH, jn = min(CH, j1), j0 = j1 - jn;
-            const int l = lane_id();
             for (int jj = jn - 1; jj >= 0; --jj) {
                 const int j = j0 + jj;
-                const T y = v[j] / st[jj * W];
-                lds_fence();
-                for (int d = 1 + l; d <= bw; d += WAVE)
-                    if (j - d >= 0) v[j - d] = fma_t(-st[jj * W + d], y, v[j - d]);
+                int d0 = s - l, d1 = s - l - WAVE;   // slot distance below the pivot
+                if (d0 < 0) d0 += W;
+                if (d1 < 0) d1 += W;
+                const T a0 = (d0 > 0 && j - d0 >= 0) ? st[jj * W + d0] : T(0);
+                const T a1 = (l + WAVE < W && d1 > 0 && j - d1 >= 0) ? st[jj * W + d1] : T(0);
+                const T y = readlane_t(s < WAVE ? r0 : r1, s & (WAVE - 1)) / st[jj * W];
+                r0 = fma_t(-a0, y, r0);
+                r1 = fma_t(-a1, y, r1);
+                const T nv = j - W >= 0 ? v[j - W] : T(0);
                 if (l == 0) v[j] = y;
-                lds_fence();
+                if (l == (s & (WAVE - 1))) {
+                    if (s < WAVE) r0 = nv;
+                    else r1 = nv;
+                }
+                if (--s < 0) s = W - 1;
             }
         }
         TM::sync();

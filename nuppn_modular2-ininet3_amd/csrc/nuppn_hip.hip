@@ -665,6 +665,8 @@ static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
     if (band) {
         // one workgroup of four wavefronts per zone, the factorisation's window in its registers
         static_assert(kBandWindowMax == kBandMaxWT, "band window limits differ");
+        static_assert(kBandListGroup == kBandEnterChunk && kBandListCap == kBandEnterCap,
+                      "band entering-list groups differ");
         c->blayout = block_layout(c->net.nsp, c->net.nre, c->sym.n, c->sym.nnzA, 1, 0, b.ni,
                                   b.nbd, b.resid.round_base[kSegRounds],
                                   (size_t) b.ni * b.bandwidth, (size_t) b.ni * b.band_w);

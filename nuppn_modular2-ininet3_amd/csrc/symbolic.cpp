@@ -368,6 +368,9 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
             }
             b.bent_ptr.push_back((int) b.bent_q.size());
         }
+        for (int l0 = 1; l0 <= ni; l0 += kBandListGroup)
+            if (b.bent_ptr[std::min(l0 + kBandListGroup, ni + 1)] - b.bent_ptr[l0] > kBandListCap)
+                return -1;
         b.ent_ptr.assign(1, 0);
         b.ent_q.clear();
         b.ent_loc.clear();

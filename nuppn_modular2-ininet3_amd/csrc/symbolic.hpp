@@ -126,6 +126,10 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
 // 8 x 8 slots per thread) and choose_block_size's answer for that form
 constexpr int kBandWindowMax = 128;
 constexpr int kBandForm = -1;
+// the factorisation prefetches the entering lists by groups of kBandListGroup consecutive
+// interior indices into LDS banks of kBandListCap entries (band_lu.hpp)
+constexpr int kBandListGroup = 16;
+constexpr int kBandListCap = 512;
 
 // the block size of the batched path: 32 or 64 where the interior is block tridiagonal with
 // that size, else kBandForm where the band form fits, else 0 (no batched path)
