@@ -165,6 +165,54 @@ __device__ __forceinline__ int band_factor(const DevBlock &B, const DevBand &D, 
         for (int q = t; q <= l1 - l0; q += TM::NT)
             L.eoff[bank * (kBandEnterChunk + 1) + q] = eptr[l0 + q] - e0;
     };
+    // Every later group goes through a pipeline of four steps, one dependent load level per
+    // step (the list bounds, the entries' fields, the matrix values, the LDS writes), so that no
+    // step waits for a global load: group g + 1 starts at the first index of group g and is in
+    // its bank three steps later, long before its first index.
+    static_assert(kBandEnterCap <= 2 * TM::NT, "two entries per thread");
+    int q_e0 = 0, q_e1 = 0, q_off = 0, q_k[2] = {0, 0}, q_col[2] = {0, 0}, q_dst[2] = {0, 0};
+    double q_ys[2] = {0.0, 0.0}, q_a[2] = {0.0, 0.0}, q_c[2] = {0.0, 0.0};
+    auto pipeline = [&](int rin) __attribute__((always_inline)) {
+        const int ph = rin % kBandEnterChunk, grp = rin / kBandEnterChunk + 1;
+        const int l0 = 1 + grp * kBandEnterChunk, l1 = min(l0 + kBandEnterChunk, ni + 1);
+        if (ph > 3 || l0 > ni) return;
+        if (ph == 0) {
+            q_e0 = eptr[l0];
+            q_e1 = eptr[l1];
+            if (t <= l1 - l0) q_off = eptr[l0 + t];
+        } else if (ph == 1) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int e = q_e0 + t + TM::NT * i;
+                if (e < q_e1) {
+                    q_k[i] = ek[e];
+                    q_ys[i] = eys[e];
+                    q_col[i] = ecolx[e];
+                    q_dst[i] = edst[e];
+                }
+            }
+        } else if (ph == 2) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                if (q_e0 + t + TM::NT * i < q_e1) {
+                    q_a[i] = A[q_k[i]];
+                    q_c[i] = cs[q_col[i]];
+                }
+        } else {
+            const int bank = grp & 1;
+            NUPPN_LDS T *bv = L.eval + bank * kBandEnterCap;
+            NUPPN_LDS int *bd = L.edst + bank * kBandEnterCap;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int e = q_e0 + t + TM::NT * i;
+                if (e < q_e1) {
+                    bv[e - q_e0] = (T) (q_a[i] * q_ys[i] * q_c[i]);
+                    bd[e - q_e0] = q_dst[i];
+                }
+            }
+            if (t <= l1 - l0) L.eoff[bank * (kBandEnterChunk + 1) + t] = q_off - q_e0;
+        }
+    };
     // the entries of interior index rin into the entering buffers of parity p (row part: the
     // slot row sr)
     auto scatter = [&](int rin, int sr, int p) __attribute__((always_inline)) {
@@ -254,8 +302,8 @@ __device__ __forceinline__ int band_factor(const DevBlock &B, const DevBand &D, 
         }
         if (enter) scatter(rin, rin % W, p);
         TM::sync();
-        // (group g's bank was last read before this barrier, at index g * chunk - 1)
-        if (enter && rin % kBandEnterChunk == 0) prefetch(rin / kBandEnterChunk + 1);
+        // (group g + 1's bank was last read before this barrier, at index g * chunk - 1)
+        if (enter) pipeline(rin);
         if (k > 0 && k % CHF == 0) flush(k - CHF, CHF);
         if (k >= 0) {
             const T piv = ur[s];
