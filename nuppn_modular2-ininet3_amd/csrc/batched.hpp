@@ -256,31 +256,42 @@ __device__ __forceinline__ void block_dxdt_dd(const DevBlock &B, const double *v
         });
 }
 
-// The Jacobian entries (CSR order) from the per-reaction stems: every entry the sum of its
-// contributions cnt stem a_row / a_col as real values - the coefficient a double-double from the
-// host, its product with the stem split exactly, the sum in double-double (Dot2) - plus the
-// 1e-99 seed of calculate_jacobian (jac_rhs.F90:309), rounded once.  jlo: [nnzA] scratch.
+// The Jacobian entries (CSR order) from the per-reaction stems: every entry is the sum of its
+// contributions cnt stem a_row / a_col (jac_rhs.F90:365-392) plus the 1e-99 seed of
+// calculate_jacobian (jac_rhs.F90:309).  All contributions of an entry share the ratio
+// a_row / a_col, so the stream carries the counts (packed in the operand words: bits 0..19 the
+// stem, bits 20..23 the count + 8), the sum of cnt stem is carried in double-double (each product
+// split exactly by an FMA, Dot2 sums), and the entry is that sum times the ratio (a double-double
+// from the host) plus the seed, rounded once.  jlo: [nnzA] scratch.
 template <class TM = Team<1>>
 __device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const double *stemp,
                                                   double *jacp, double *jlop) {
     const DevSeg &J = B.jac;
-    const NUPPN_GLB double *coef = gptr(J.e_coef), *coefl = gptr(J.e_coef_lo), *st = gptr(stemp);
+    const NUPPN_GLB double *st = gptr(stemp);
+    const NUPPN_GLB double *rhi = gptr(B.jac_rhi), *rlo = gptr(B.jac_rlo);
     NUPPN_GLB double *jac = gptr(jacp), *jlo = gptr(jlop);
     seg_stream<SumDD2, TM>(
         J,
         [&](int e, int idx) {
-            const double s = st[idx], ch = coef[e];
-            const double t = s * ch;
-            return SumDD2{t, fma(s, ch, -t) + s * coefl[e]};
+            (void) e;
+            const double s = st[idx & 0xFFFFF];
+            const double c = (double) (((idx >> 20) & 0xF) - 8);
+            const double t = s * c;
+            return SumDD2{t, fma(s, c, -t)};
         },
         [&](int k, const SumDD2 &t) {
             jac[k] = t.hi;
             jlo[k] = t.lo;
         });
     for (int k = TM::tid(); k < B.nnzA; k += TM::NT) {
-        const double hi = jac[k], seed = 1.0e-99;
-        const double s = hi + seed, bb = s - hi;
-        jac[k] = s + (((hi - (s - bb)) + (seed - bb)) + jlo[k]);
+        // (rh + rl) (sh + sl) + seed: the leading product split exactly, the cross terms and
+        // the seed's TwoSum error added to it, rounded once
+        const double sh = jac[k], sl = jlo[k], rh = rhi[k], rl = rlo[k];
+        const double p = rh * sh;
+        const double pe = fma(rh, sh, -p) + (rh * sl + rl * sh);
+        const double seed = 1.0e-99;
+        const double s = p + seed, bb = s - p;
+        jac[k] = s + (((p - (s - bb)) + (seed - bb)) + pe);
     }
     TM::sync();
 }

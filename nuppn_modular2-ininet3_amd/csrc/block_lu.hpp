@@ -29,9 +29,8 @@ namespace nuppn {
 // a SegStream on the device (the host builder's comment has the layout), physical order
 struct DevSeg {
     int nseg;
-    const int *e_idx;       // operand index, bit 31 on a segment's last term
-    const double *e_coef;
-    const double *e_coef_lo;   // J: low parts of the coefficients (null for the other streams)
+    const int *e_idx;       // operand index, bit 31 on a segment's last term (J: with the count)
+    const double *e_coef;   // coefficients (null for J)
     const int *round_base;  // [kSegRounds + 1]
     const int *lane_len;    // [64 kSegRounds]
     const int *lane_s0;     // [64 kSegRounds]
@@ -50,6 +49,7 @@ struct DevBlock {
     const int *dyn_k;      // [8 nre] the dynamic network's entry map (DevNet::dyn_q) in CSR order
     const unsigned char *csr_isdiag;
     const int *resid_pos;  // [nnzA] physical position of each A entry in the residual stream
+    const double *jac_rhi, *jac_rlo;   // [nnzA] the mass-number ratio of each J entry, hi + lo
     DevSeg jac, dxdt, resid;
 };
 

@@ -323,8 +323,8 @@ struct nuppn_ctx {
     double *d_b_mass = nullptr, *d_b_ent_ys = nullptr;
     int *d_seg_i[12] = {};        // jac, dxdt, resid: e_idx, lane_len, lane_s0, round_base
     int *d_b_resid_pos = nullptr; // A entry -> physical position in the residual stream
-    double *d_seg_c[3] = {};      // jac, dxdt, resid: e_coef
-    double *d_jac_lo = nullptr;   // jac: e_coef_lo
+    double *d_seg_c[3] = {};      // dxdt, resid: e_coef; [0]: J's per-entry ratio, high parts
+    double *d_jac_lo = nullptr;   // J's per-entry ratio, low parts
     double *d_rcache = nullptr, *d_rc_t9 = nullptr;   // REACLIB T9 cache, grid
     // rate physics (nuppn_set_rate_physics)
     int pf_n = 0, ndb = 0, screen = 0;
@@ -523,9 +523,10 @@ static DevBlock dev_block(const nuppn_ctx *c) {
         ds[i]->lane_s0 = c->d_seg_i[4 * i + 2];
         ds[i]->round_base = c->d_seg_i[4 * i + 3];
         ds[i]->e_coef = c->d_seg_c[i];
-        ds[i]->e_coef_lo = nullptr;
     }
-    B.jac.e_coef_lo = c->d_jac_lo;
+    B.jac.e_coef = nullptr;   // the J stream's counts are packed in its operand words
+    B.jac_rhi = c->d_seg_c[0];
+    B.jac_rlo = c->d_jac_lo;
     return B;
 }
 
@@ -657,10 +658,10 @@ static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
             dev_upload(&c->d_seg_i[4 * i + 1], sc[i]->lane_len) ||
             dev_upload(&c->d_seg_i[4 * i + 2], sc[i]->lane_s0) ||
             dev_upload(&c->d_seg_i[4 * i + 3], sc[i]->round_base) ||
-            dev_upload(&c->d_seg_c[i], sc[i]->e_coef))
+            dev_upload(&c->d_seg_c[i], i == 0 ? b.jac_rhi : sc[i]->e_coef))
             return NUPPN_ERR_HIP;
     }
-    if (dev_upload(&c->d_jac_lo, b.jac.e_coef_lo) || dev_upload(&c->d_b_resid_pos, b.resid_pos))
+    if (dev_upload(&c->d_jac_lo, b.jac_rlo) || dev_upload(&c->d_b_resid_pos, b.resid_pos))
         return NUPPN_ERR_HIP;
     if (band) {
         // one workgroup of four wavefronts per zone, the factorisation's window in its registers

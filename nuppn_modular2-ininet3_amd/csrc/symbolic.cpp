@@ -445,26 +445,22 @@ int build_block_symbolic(const HostNetwork &net, const Symbolic &s, int NB, Bloc
 
 namespace {
 
-// segments given as per-segment element lists (and, for J, the coefficients' low parts) -> the
-// streamed layout (SegStream)
-void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, SegStream &out,
-                  const std::vector<std::vector<double>> *seg_lo = nullptr) {
+// segments given as per-segment element lists -> the streamed layout (SegStream)
+void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, SegStream &out) {
     out.nseg = (int) segs.size();
     std::vector<int> idx;       // logical order
-    std::vector<double> cf, lo;
+    std::vector<double> cf;
     std::vector<int> seg_of;    // segment of every term
     for (int sg = 0; sg < out.nseg; ++sg) {
         const auto &seg = segs[sg];
         if (seg.empty()) {
             idx.push_back(0);
             cf.push_back(0.0);
-            lo.push_back(0.0);
             seg_of.push_back(sg);
         }
         for (size_t t = 0; t < seg.size(); ++t) {
             idx.push_back(seg[t].first);
             cf.push_back(seg[t].second);
-            lo.push_back(seg_lo ? (*seg_lo)[sg][t] : 0.0);
             seg_of.push_back(sg);
         }
         idx.back() |= (int) (1u << 31);
@@ -499,7 +495,6 @@ void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, 
     const int nphys = out.round_base[kSegRounds];
     out.e_idx.assign(nphys, 0);
     out.e_coef.assign(nphys, 0.0);
-    out.e_coef_lo.assign(seg_lo ? nphys : 0, 0.0);
     for (int r = 0; r < kSegRounds; ++r)
         for (int l = 0; l < 64; ++l)
             for (int j = 0; j < out.lane_len[64 * r + l]; ++j) {
@@ -508,7 +503,6 @@ void build_stream(const std::vector<std::vector<std::pair<int, double>>> &segs, 
                 out.phys[t] = P;
                 out.e_idx[P] = idx[t];
                 out.e_coef[P] = cf[t];
-                if (seg_lo) out.e_coef_lo[P] = lo[t];
             }
 }
 
@@ -542,14 +536,22 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
         for (int k = b.csr_ptr[r]; k < b.csr_ptr[r + 1]; ++k) b.csr_row[k] = r;
     const double *an = net.aw.data();
     // J contributions (statement order of calculate_jacobian, jac_rhs.F90:365-392):
-    // value = (+-count) * an[row] / an[col] * stem, stem = st1[l] (m in 0,1,3,4) or st2[l]
-    // The coefficient cnt * an[row] / an[col] is kept as a double-double, hi + lo, with hi the
-    // rounded quotient and lo = (num - hi an[col]) / an[col] (the remainder of the division is
-    // exact with an FMA), so that the device can sum each entry's contributions as real values.
+    // value = (+-count) * an[row] / an[col] * stem, stem = st1[l] (m in 0,1,3,4) or st2[l].
+    // Every contribution of one entry shares the ratio an[row] / an[col], so the stream carries
+    // only the count, packed into the term's operand word (bits 20..23: count + 8; bits 0..19:
+    // the stem index), and the ratio is applied once per entry: a double-double hi + lo, with hi
+    // the rounded quotient and lo = (an[row] - hi an[col]) / an[col] (the division's remainder is
+    // exact with an FMA).  The device sums count * stem in double-double (each product split
+    // exactly) and multiplies the sum by the ratio, rounding once.
+    b.jac_rhi.assign(nnz, 0.0);
+    b.jac_rlo.assign(nnz, 0.0);
     std::vector<std::vector<std::pair<int, double>>> segs(nnz);
-    std::vector<std::vector<double>> seg_lo(nnz);
     for (int k = 0; k < nnz; ++k) {
         const int q = b.csr_q[k];
+        const int srow = s.red2sp[b.csr_row[k]], scol = s.red2sp[b.csr_col[k]];
+        const double ar = an[srow], ac = an[scol];
+        b.jac_rhi[k] = ar / ac;
+        b.jac_rlo[k] = std::fma(-b.jac_rhi[k], ac, ar) / ac;
         for (int e = s.jq_ptr[q]; e < s.jq_ptr[q + 1]; ++e) {
             const int l = s.jq_ent[e] >> 3, m = s.jq_ent[e] & 7;
             const int in1 = net.k[l], n1 = net.k[nre + l], in2 = net.k[2 * nre + l];
@@ -559,13 +561,20 @@ static int build_batched_sums(const HostNetwork &net, const Symbolic &s, BlockSy
             const int cols[8] = {in1, in1, in2, in1, in1, in2, in2, in2};
             const int cnt[8] = {-n1, -n2, -n1, no1, no2, -n2, no1, no2};
             const bool stem2[8] = {false, false, true, false, false, true, true, true};
-            const double num = cnt[m] * an[rows[m]], ac = an[cols[m]];
-            const double hi = num / ac;
-            segs[k].push_back({stem2[m] ? nre + l : l, hi});
-            seg_lo[k].push_back(std::fma(-hi, ac, num) / ac);
+            const int op = stem2[m] ? nre + l : l;
+            if (rows[m] != srow || cols[m] != scol || cnt[m] < -8 || cnt[m] > 7 || op >= (1 << 20))
+                return -1;
+            segs[k].push_back({op, (double) cnt[m]});
         }
     }
-    build_stream(segs, b.jac, &seg_lo);
+    build_stream(segs, b.jac);
+    // pack the counts into the operand words (padding and empty-segment terms: count 0)
+    for (size_t P = 0; P < b.jac.e_idx.size(); ++P) {
+        const int raw = b.jac.e_idx[P];
+        const int cnt = (int) b.jac.e_coef[P];
+        b.jac.e_idx[P] = (raw & (int) (1u << 31)) | (raw & 0xFFFFF) | ((cnt + 8) << 20);
+    }
+    b.jac.e_coef.clear();
     // dX/dt (calculate_dxdt, jac_rhs.F90:250-272): roles k1 -k2, k3 -1, k5 +k6, k7 +k8, times A
     std::vector<std::vector<std::pair<int, double>>> dsegs(n);
     for (int r = 0; r < n; ++r) {

@@ -50,9 +50,9 @@ int build_symbolic(const HostNetwork &net, Symbolic &s);
 constexpr int kSegRounds = 4;
 struct SegStream {
     int nseg = 0;
-    std::vector<int> e_idx;         // operand index, | (1 << 31) on a segment's last term
-    std::vector<double> e_coef;     // coefficient
-    std::vector<double> e_coef_lo;  // J only: the coefficient's low part (hi + lo its exact value)
+    std::vector<int> e_idx;         // operand index, | (1 << 31) on a segment's last term (J: the
+                                    // count packed in bits 20..23, BlockSym::jac_rhi)
+    std::vector<double> e_coef;     // coefficient (emptied for J)
     std::vector<int> round_base;    // [kSegRounds + 1]
     std::vector<int> lane_len;      // [64 kSegRounds]
     std::vector<int> lane_s0;       // [64 kSegRounds]
@@ -112,6 +112,9 @@ struct BlockSym {
     // the residual rows from the CSR entries (operand = column; the coefficient of the term at
     // physical position P is A's entry stored there, see resid_pos)
     SegStream jac, dxdt, resid;
+    // J: per CSR entry the ratio of mass numbers an[row] / an[col] as a double-double hi + lo,
+    // the factor every contribution of the entry shares
+    std::vector<double> jac_rhi, jac_rlo;
     std::vector<int> resid_pos;   // [nnzA] physical position of each CSR entry in the residual stream
 };
 
