@@ -45,7 +45,11 @@ enum BSlotArr { B_XN, B_VX, B_ST, B_AVAL, B_RHS, B_X, B_RES, B_D, B_JAC, B_YK, B
                 B_COUNT };
 
 // the factors, in the same slot after the arrays above
-enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_COUNT };
+// (F_DIN .. F_BIN: the dense blocks D_i, C_i, B_i of the scaled matrix, the factorisation's
+// inputs, one set per factor precision (FP64, then FP32); only their pattern positions are ever
+// written, so the zeros elsewhere persist from the slots' allocation)
+enum BFacArr { F_CS, F_COLSC, F_ROWINV, F_SINV, F_W, F_BM, F_E, F_XF, F_G, F_GPIV, F_DIN, F_CIN,
+               F_BIN, F_DIN32, F_CIN32, F_BIN32, F_COUNT };
 
 struct BLayout {
     size_t off[B_COUNT];
@@ -68,7 +72,7 @@ inline BLayout block_layout(int nsp, int nre, int n, int nnzA, int NB, int nblk,
     const size_t flen[F_COUNT] = {8ull * n, 8ull * (nip + nbd), 8ull * (nip + nbd),
                                   band_l ? 8 * band_l : bs, band_u ? 8 * band_u : bs, bs,
                                   8ull * nbd * nip + 8, 8ull * nbd * nip + 8, 8ull * nbd * nbd + 8,
-                                  4ull * nbd + 4};
+                                  4ull * nbd + 4, bs, bs, bs, bs / 2, bs / 2, bs / 2};
     BLayout L;
     size_t o = 0;
     for (int i = 0; i < B_COUNT; ++i) {
@@ -106,6 +110,10 @@ __device__ inline BlockFactors<T> zone_factors(char *base, const size_t *foff) {
     F.cs = (double *) (base + foff[F_CS]);
     F.colsc = (double *) (base + foff[F_COLSC]);
     F.rowinv = (double *) (base + foff[F_ROWINV]);
+    const bool f64 = sizeof(T) == 8;
+    F.Din = (T *) (base + foff[f64 ? F_DIN : F_DIN32]);
+    F.Cin = (T *) (base + foff[f64 ? F_CIN : F_CIN32]);
+    F.Bin = (T *) (base + foff[f64 ? F_BIN : F_BIN32]);
     return F;
 }
 

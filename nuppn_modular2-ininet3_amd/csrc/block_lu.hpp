@@ -63,6 +63,10 @@ struct BlockFactors {
     double *cs;         // [n] 1 / diagonal of A: the column scaling
     double *colsc;      // [nip + nbd] by padded position: mass * column scaling
     double *rowinv;     // [nip + nbd] by padded position: 1 / mass
+    // the dense blocks of the scaled matrix (D_i, C_i, B_i: [nblk][NB*NB] column-major, one set
+    // per precision): written at the pattern's positions before every factorisation and read by
+    // it and by the solves (B_i, C_i); the other positions hold zeros from the allocation
+    T *Din, *Cin, *Bin;
 };
 
 template <class T, int NB>
@@ -417,7 +421,7 @@ __device__ __forceinline__ void thomas_solve(const DevBlock &B, const BlockFacto
     for (int j = 0; j < CPL; ++j) mB[j] = T(0);
     for (int i = 0; i < nblk; ++i) {
         if (i + 1 < nblk) {
-            load_mv<T, NB>(F.Bm + (size_t) i * BS, nB);
+            load_mv<T, NB>(F.Bin + (size_t) i * BS, nB);
             load_mv<T, NB>(F.Sinv + (size_t) (i + 1) * BS, nS);
         }
         if (i > 0) {
@@ -513,7 +517,7 @@ __device__ __forceinline__ void twisted_solve(const DevBlock &B, const BlockFact
         load_mv<T, NB>(F.Sinv, mS);
         for (int i = 0; i < m; ++i) {
             if (i + 1 < m) load_mv<T, NB>(F.Sinv + (size_t) (i + 1) * BS, nS);
-            load_mv<T, NB>(F.Bm + (size_t) i * BS, nO);                       // B_i, next step
+            load_mv<T, NB>(F.Bin + (size_t) i * BS, nO);                    // B_i, next step
             if (i > 0) sub(mO, i, i - 1);
             mul(mS, i);
             copy(mS, nS);
@@ -523,7 +527,7 @@ __device__ __forceinline__ void twisted_solve(const DevBlock &B, const BlockFact
         load_mv<T, NB>(F.Sinv + (size_t) (N - 1) * BS, mS);
         for (int i = N - 1; i > m; --i) {
             if (i - 1 > m) load_mv<T, NB>(F.Sinv + (size_t) (i - 1) * BS, nS);
-            load_mv<T, NB>(F.W + (size_t) (i - 1) * BS, nO);                  // C_{i-1}, next
+            load_mv<T, NB>(F.Cin + (size_t) (i - 1) * BS, nO);              // C_{i-1}, next
             if (i < N - 1) sub(mO, i, i + 1);
             mul(mS, i);
             copy(mS, nS);
@@ -534,11 +538,11 @@ __device__ __forceinline__ void twisted_solve(const DevBlock &B, const BlockFact
     TM::sync();
     if (active && role == 0) {
         if (m > 0) {
-            load_mv<T, NB>(F.Bm + (size_t) (m - 1) * BS, mO);
+            load_mv<T, NB>(F.Bin + (size_t) (m - 1) * BS, mO);
             sub(mO, m, m - 1);
         }
         if (m < N - 1) {
-            load_mv<T, NB>(F.W + (size_t) m * BS, mO);
+            load_mv<T, NB>(F.Cin + (size_t) m * BS, mO);
             sub(mO, m, m + 1);
         }
         load_mv<T, NB>(F.Sinv + (size_t) m * BS, mS);
@@ -580,8 +584,9 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// nsteps elimination steps of a block sweep on one wavefront, the dense blocks in place (D_i at
-// Sinv[i], C_i at W[i], B_i at Bm[i]).  Step on block i, with a = the block's updated diagonal:
+// nsteps elimination steps of a block sweep on one wavefront, the dense blocks read from the
+// input areas (D_i at Din[i], C_i at Cin[i], B_i at Bin[i]).  Step on block i, with a = the
+// block's updated diagonal:
 //   top-down (UP false):  a <- S_i^-1 -> Sinv[i];  W_i = S_i^-1 C_i -> W[i];
 //                         a <- D_{i+1} - B_i W_i, next block i + 1;
 //   bottom-up (UP true):  a <- R_i^-1 -> Sinv[i];  V_i = R_i^-1 B_{i-1} -> Bm[i-1];
@@ -602,24 +607,25 @@ __device__ __forceinline__ int block_sweep(const BlockFactors<T> &F, NUPPN_LDS T
     for (int k = 0; k < nsteps; ++k) {
         const int i = UP ? i0 - k : i0 + k;
         const int nx = UP ? i - 1 : i + 1;                    // the next block
-        T *m1 = UP ? F.Bm + (size_t) (i - 1) * BS : F.W + (size_t) i * BS;    // C_i / B_{i-1}
-        T *m2 = UP ? F.W + (size_t) (i - 1) * BS : F.Bm + (size_t) i * BS;    // B_i / C_{i-1}
+        T *m1 = UP ? F.Bm + (size_t) (i - 1) * BS : F.W + (size_t) i * BS;    // V_i / W_i out
+        const T *i1 = UP ? F.Bin + (size_t) (i - 1) * BS : F.Cin + (size_t) i * BS;   // B_{i-1} / C_i
+        const T *i2 = UP ? F.Cin + (size_t) (i - 1) * BS : F.Bin + (size_t) i * BS;   // C_{i-1} / B_i
         const bool zero = zero_last && k == nsteps - 1;
-        glb_load_cm<T, NB>(m1, cb);
-        glb_load_cm<T, NB>(m2, bb);
+        glb_load_cm<T, NB>(i1, cb);
+        glb_load_cm<T, NB>(i2, bb);
         if constexpr (NB == 32) {
             if (zero) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) dnt.t[q] = typename MVec<T>::type{0, 0, 0, 0};
             } else {
-                mtile_load_glb<T>(F.Sinv + (size_t) nx * BS, dnt);
+                mtile_load_glb<T>(F.Din + (size_t) nx * BS, dnt);
             }
         } else {
             if (zero) {
 #pragma unroll
                 for (int t = 0; t < C::R; ++t) dn.v[t] = T(0);
             } else {
-                glb_load_cm<T, NB>(F.Sinv + (size_t) nx * BS, dn);
+                glb_load_cm<T, NB>(F.Din + (size_t) nx * BS, dn);
             }
         }
         gj_inverse<T, NB>(a, X2);   // X2 is free between the products
@@ -714,33 +720,28 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         gptr(F.rowinv)[p] = 1.0 / ms;
     }
     TM::sync();
-    // dense blocks of the scaled matrix for all block steps at once: D_i where S_i^-1 goes, C_i
-    // where W_i goes, B_i in place (the loop then only streams them, prefetched)
+    // dense blocks of the scaled matrix for all block steps at once (the loop then only streams
+    // them, prefetched): the pattern's entries into the FP64 input areas, whose other positions
+    // stay zero (nothing else writes these areas)
     {
         const size_t area = (size_t) nblk * BS;
-        NUPPN_GLB T *S0 = gptr(F.Sinv), *W0 = gptr(F.W), *B0 = gptr(F.Bm);
-        for (size_t e = t0; e < area; e += NT) {
-            S0[e] = T(0);
-            W0[e] = T(0);
-            B0[e] = T(0);
-        }
-        TM::sync();
+        NUPPN_GLB T *D0 = gptr(F.Din), *C0 = gptr(F.Cin), *B0 = gptr(F.Bin);
         const int nent = gptr(B.ent_ptr)[3 * nblk - 2];
         for (int e = t0; e < nent; e += NT) {
             const int dst = gptr(B.ent_dst)[e];
             const int ar = (int) (dst / area);
-            NUPPN_GLB T *base = ar == 0 ? S0 : (ar == 1 ? W0 : B0);
+            NUPPN_GLB T *base = ar == 0 ? D0 : (ar == 1 ? C0 : B0);
             base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
         }
         const int pr = t0, last = nblk - 1;   // identity on the padded diagonal positions
-        if (pr < NB && last * NB + pr >= B.ni) S0[(size_t) last * BS + pr * NB + pr] = T(1);
+        if (pr < NB && last * NB + pr >= B.ni) D0[(size_t) last * BS + pr * NB + pr] = T(1);
         TM::sync();
     }
     int fail = 0;
     if constexpr (TM::NWAVE == 1) {
         // block Thomas: S_0 = D_0, W_i = S_i^-1 C_i, S_{i+1} = D_{i+1} - B_i W_i
         Regs<T, NB> a;
-        glb_load_cm<T, NB>(F.Sinv, a);
+        glb_load_cm<T, NB>(F.Din, a);
         fail = block_sweep<T, NB, false>(F, L.X1, L.X2, 0, nblk - 1, false, a);
         if (!fail) fail = block_last_inverse<T, NB>(F, L.X2, nblk - 1, a);
     } else {
@@ -751,10 +752,10 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         const int m = twist_block(nblk);
         Regs<T, NB> a;
         if (TM::wave() == 0) {
-            glb_load_cm<T, NB>(F.Sinv, a);
+            glb_load_cm<T, NB>(F.Din, a);
             fail = block_sweep<T, NB, false>(F, L.X1, L.X2, 0, m, false, a);
         } else if (TM::wave() == 1 && nblk - 1 > m) {
-            glb_load_cm<T, NB>(F.Sinv + (size_t) (nblk - 1) * BS, a);
+            glb_load_cm<T, NB>(F.Din + (size_t) (nblk - 1) * BS, a);
             fail = block_sweep<T, NB, true>(F, L.X1b, L.X2b, nblk - 1, nblk - 1 - m, true, a);
             lds_store_cm<T, NB>(L.X1b, a);   // -C_m V_{m+1}
             wave_fence();

@@ -752,6 +752,9 @@ static int ensure_bslots(nuppn_ctx *c, int want) {
     c->d_bslots = nullptr;
     c->bslots = 0;
     HIP_OK(hipMalloc((void **) &c->d_bslots, c->blayout.bytes * (size_t) want));
+    // zero once: the factorisation's dense input areas are only ever written at the pattern's
+    // positions (block_factor), so their other positions must start at zero
+    HIP_OK(hipMemset(c->d_bslots, 0, c->blayout.bytes * (size_t) want));
     c->bslots = want;
     return 0;
 }
