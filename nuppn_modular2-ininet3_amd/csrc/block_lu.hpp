@@ -223,6 +223,14 @@ __device__ __forceinline__ void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *buf) {
             row1[c] = r1;
         }
         lds_fence();
+        // the pivot columns' entries of this lane's rows, all loads issued at once and before the
+        // pivot block's inverse, whose latency they overlap (one LDS round trip per step)
+        T x0[C::R], x1[C::R];
+#pragma unroll
+        for (int t = 0; t < C::R; ++t) {
+            x0[t] = col0[h * C::R + t];
+            x1[t] = col1[h * C::R + t];
+        }
         // P^-1 = [[p11, -p01], [-p10, p00]] / det
         const T p00 = col0[k], p10 = col0[k + 1], p01 = col1[k], p11 = col1[k + 1];
         const T dinv = pivot_rcp(p00 * p11 - p01 * p10);
@@ -242,10 +250,8 @@ __device__ __forceinline__ void gj_inverse(Regs<T, NB> &a, NUPPN_LDS T *buf) {
             f1 = fma_t(q10, u0, q11 * u1);
         }
 #pragma unroll
-        for (int t = 0; t < C::R; ++t) {
-            const T x0 = col0[h * C::R + t], x1 = col1[h * C::R + t];
-            a.v[t] = fma_t(-x0, f0, fma_t(-x1, f1, inK ? T(0) : a.v[t]));
-        }
+        for (int t = 0; t < C::R; ++t)
+            a.v[t] = fma_t(-x0[t], f0, fma_t(-x1[t], f1, inK ? T(0) : a.v[t]));
         if (h == kh) {
 #pragma unroll
             for (int t = 0; t < C::R; t += 2)
