@@ -337,6 +337,9 @@ int nuppn_last_timing(nuppn_ctx *ctx, double *out, int nout);
 /* Link-level drop-ins with the reference signatures and semantics.
  * slu_solve: solver/source/slu_f90/slu_f90.c:3-54 (Fortran interface slu_f90/slu.f90:6-11);
  *   colind/rowptr are shifted to 0-based in place when shiftindex != 0, as the reference does.
+ *   Its device buffers (pattern, values, dense-capacity factors) are kept per GPU from call to
+ *   call and reallocated only for a larger system (held until the process ends); calls are
+ *   serialised by a process-wide lock.
  * c_fortran_dgssv_: solver/source/SuperLU_5.0/FORTRAN/c_fortran_dgssv.c:35-190
  *   (iopt 1 factor -> handle, 2 solve, 3 free; 1-based CSC). */
 void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int colind[],

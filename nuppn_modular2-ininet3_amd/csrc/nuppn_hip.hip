@@ -2014,6 +2014,7 @@ struct DropinFactors {
     LUStore st{};
     size_t lds = 0;
     int nb_cap = 0;
+    int cap_n = 0, cap_nnz = 0;   // the largest system the buffers hold
 };
 
 void dropin_free(DropinFactors *f) {
@@ -2027,58 +2028,78 @@ void dropin_free(DropinFactors *f) {
 }
 
 // factor a 0-based CSC matrix (rows sorted or not; COLAMD handles both); returns SuperLU info,
-// or a negative library error
+// or a negative library error.  reuse: buffers of an earlier call, taken when they hold this
+// system (n and nnz within their capacities; *out is then reuse), otherwise new ones are
+// allocated (and reuse is left to the caller).  On a negative error *out is not set and only
+// new buffers are freed.
 int dropin_factor(int n, const int *colptr, const int *rowind, const double *val,
-                  DropinFactors **out) {
+                  DropinFactors **out, DropinFactors *reuse = nullptr) {
     std::vector<int> perm_c;
     if (superlu_column_order(n, colptr, rowind, perm_c) != 0) return NUPPN_ERR_ARG;
     std::vector<int> iperm(n);
     for (int i = 0; i < n; ++i) iperm[perm_c[i]] = i;
-    auto *f = new DropinFactors;
-    f->n = n;
     const int nnz = colptr[n];
-    const size_t capL = (size_t) n * (n - 1) / 2 + 1, capU = (size_t) n * (n + 1) / 2 + 1;
-    if (dev_upload(&f->d_Acp, colptr, n + 1) || dev_upload(&f->d_Ari, rowind, nnz) ||
-        dev_upload(&f->d_val, val, nnz) || dev_upload(&f->d_perm_c, perm_c) ||
-        dev_upload(&f->d_iperm_c, iperm) || dev_upload(&f->d_perm_r, (const int *) nullptr, n) ||
-        dev_upload(&f->d_Lp, (const int *) nullptr, n + 1) ||
-        dev_upload(&f->d_info, (const int *) nullptr, 1) ||
-        dev_upload(&f->st.Li, (const int *) nullptr, capL) ||
-        dev_upload(&f->st.Lx, (const double *) nullptr, capL) ||
-        dev_upload(&f->st.Ui, (const int *) nullptr, capU) ||
-        dev_upload(&f->st.Ux, (const double *) nullptr, capU) ||
-        dev_upload(&f->st.Up, (const int *) nullptr, n + 1)) {
-        dropin_free(f);
-        return NUPPN_ERR_HIP;
+    const bool take = reuse && reuse->cap_n >= n && reuse->cap_nnz >= nnz;
+    DropinFactors *f = take ? reuse : new DropinFactors;
+    auto fail = [&](int code) {
+        if (f != reuse) dropin_free(f);
+        return code;
+    };
+    f->n = n;
+    if (take) {
+        const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+        if (hipMemcpy(f->d_Acp, colptr, sizeof(int) * (n + 1), h2d) != hipSuccess ||
+            hipMemcpy(f->d_Ari, rowind, sizeof(int) * nnz, h2d) != hipSuccess ||
+            hipMemcpy(f->d_val, val, sizeof(double) * nnz, h2d) != hipSuccess ||
+            hipMemcpy(f->d_perm_c, perm_c.data(), sizeof(int) * n, h2d) != hipSuccess ||
+            hipMemcpy(f->d_iperm_c, iperm.data(), sizeof(int) * n, h2d) != hipSuccess)
+            return fail(NUPPN_ERR_HIP);
+    } else {
+        const size_t capL = (size_t) n * (n - 1) / 2 + 1, capU = (size_t) n * (n + 1) / 2 + 1;
+        if (dev_upload(&f->d_Acp, colptr, n + 1) || dev_upload(&f->d_Ari, rowind, nnz) ||
+            dev_upload(&f->d_val, val, nnz) || dev_upload(&f->d_perm_c, perm_c) ||
+            dev_upload(&f->d_iperm_c, iperm) ||
+            dev_upload(&f->d_perm_r, (const int *) nullptr, n) ||
+            dev_upload(&f->d_Lp, (const int *) nullptr, n + 1) ||
+            dev_upload(&f->d_info, (const int *) nullptr, 1) ||
+            dev_upload(&f->st.Li, (const int *) nullptr, capL) ||
+            dev_upload(&f->st.Lx, (const double *) nullptr, capL) ||
+            dev_upload(&f->st.Ui, (const int *) nullptr, capU) ||
+            dev_upload(&f->st.Ux, (const double *) nullptr, capU) ||
+            dev_upload(&f->st.Up, (const int *) nullptr, n + 1))
+            return fail(NUPPN_ERR_HIP);
+        f->cap_n = n;
+        f->cap_nnz = nnz;
     }
     f->lds = lds_bytes(n);
     if (f->lds > 64 * 1024)
         (void) hipFuncSetAttribute((const void *) lu_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) f->lds);
-    if (ensure_stack((const void *) lu_kernel)) {
-        dropin_free(f);
-        return NUPPN_ERR_HIP;
-    }
+    if (ensure_stack((const void *) lu_kernel)) return fail(NUPPN_ERR_HIP);
     hipLaunchKernelGGL(lu_kernel, dim3(1), dim3(64), f->lds, 0, n, f->d_Acp, f->d_Ari, f->d_val,
                        f->d_perm_c, f->d_iperm_c, 0, (double *) nullptr, f->st, f->d_perm_r,
                        f->d_Lp, f->d_info, 1);
     int info = 0;
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpy(&info, f->d_info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-        dropin_free(f);
-        return NUPPN_ERR_HIP;
-    }
+        hipMemcpy(&info, f->d_info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NUPPN_ERR_HIP);
     *out = f;
     return info;
 }
 
+// slu_solve's buffers, per device, kept from call to call (until the process ends)
+std::mutex g_slu_mu;
+std::vector<DropinFactors *> g_slu_ws;
+
 int dropin_solve(DropinFactors *f, int nrhs, double *b, int ldb) {
     const int n = f->n;
     if (nrhs <= 0) return 0;
-    if (nrhs > f->nb_cap) {
+    if (nrhs > f->nb_cap) {   // room for nrhs vectors of the largest system the buffers hold
         dev_free(f->d_b);
         f->d_b = nullptr;
-        if (dev_upload(&f->d_b, (const double *) nullptr, (size_t) n * nrhs)) return NUPPN_ERR_HIP;
+        f->nb_cap = 0;
+        if (dev_upload(&f->d_b, (const double *) nullptr, (size_t) f->cap_n * nrhs))
+            return NUPPN_ERR_HIP;
         f->nb_cap = nrhs;
     }
     for (int r = 0; r < nrhs; ++r)
@@ -2128,15 +2149,26 @@ void slu_solve(int n, int nnz, int isrow, int shiftindex, double val[], int coli
         std::copy(colind, colind + nnz, ri.begin());
         std::copy(val, val + nnz, cv.begin());
     }
-    DropinFactors *f = nullptr;
-    const int info = dropin_factor(n, cp.data(), ri.data(), cv.data(), &f);
+    // the device buffers of the last call on this device are reused while the system fits them
+    // (a larger one replaces them); calls are serialised
+    std::lock_guard<std::mutex> lock(g_slu_mu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        *ierr = NUPPN_ERR_HIP;
+        return;
+    }
+    if ((int) g_slu_ws.size() <= dev) g_slu_ws.resize(dev + 1, nullptr);
+    DropinFactors *ws = g_slu_ws[dev], *f = nullptr;
+    const int info = dropin_factor(n, cp.data(), ri.data(), cv.data(), &f, ws);
+    if (f && f != ws) {
+        dropin_free(ws);
+        g_slu_ws[dev] = f;
+    }
     if (info != 0) {
         *ierr = info;
-        if (f) dropin_free(f);
         return;
     }
     *ierr = dropin_solve(f, 1, rhs, n);
-    dropin_free(f);
 }
 
 }  // extern "C"

@@ -251,6 +251,33 @@ def test_slu_solve_dropin(hip, oracle):
     assert hip.slu_solve(val2, ci2, rp2, b2) > 0
 
 
+def test_slu_solve_reuses_its_buffers_across_sizes(hip, oracle, c1):
+    """slu_solve keeps its device buffers from call to call and grows them for a larger system:
+    g20 (via CSR), a 1,099-species network matrix, then g20 again and the network matrix again.
+    Every solution is bit-equal to the oracle GEPP in SuperLU's column order, whatever buffers
+    the call found."""
+    g = np.load(os.path.join(GOLD, "g20.npz"))
+    w, ctx, o = c1
+    v = o.rates(w.t9[2, 0], w.rho[2, 0])
+    n1, rp, ci, va, mask = o.reduced_matrix(w.dt[2], w.x0[2], v)
+    cp1, ri1, cv1 = oracle.csr_to_csc(n1, rp, ci, va)
+    cases = [(int(g["n"]), g["colptr"], g["rowind"], g["values"]), (n1, cp1, ri1, cv1)]
+    for n, cp, ri, cv in cases + cases:
+        cp = np.asarray(cp)
+        ri = np.asarray(ri)
+        cv = np.ascontiguousarray(cv, dtype=np.float64)
+        rhs = np.random.default_rng(n).standard_normal(n)
+        perm_c = hip.column_order(n, cp, ri)
+        solve = oracle.gepp(n, cp, ri, cv, perm_c)[4]
+        # the CSC matrix handed over as the CSR pattern of its transpose: isrow = 0 takes
+        # (rowptr, colind) as the column pointers and row indices
+        b = rhs.copy()
+        ierr = hip.slu_solve(cv.copy(), (ri + 1).astype(np.int32), (cp + 1).astype(np.int32), b,
+                             isrow=0)
+        assert ierr == 0, (n, ierr)
+        np.testing.assert_array_equal(b, solve(rhs), err_msg=f"n = {n}")
+
+
 def test_c_fortran_dgssv_dropin(hip, oracle, c1):
     """iopt=1 factor / iopt=2 solve (two right-hand sides, ldb > n) / iopt=3 free on g20 and on
     a network matrix; solutions bit-equal to the oracle GEPP in SuperLU's column order."""
