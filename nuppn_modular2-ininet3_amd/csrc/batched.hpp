@@ -291,16 +291,20 @@ __device__ __forceinline__ void block_jacobian_dd(const DevBlock &B, const doubl
             jac[k] = t.hi;
             jlo[k] = t.lo;
         });
-    for (int k = TM::tid(); k < B.nnzA; k += TM::NT) {
-        // (rh + rl) (sh + sl) + seed: the leading product split exactly, the cross terms and
-        // the seed's TwoSum error added to it, rounded once
-        const double sh = jac[k], sl = jlo[k], rh = rhi[k], rl = rlo[k];
-        const double p = rh * sh;
-        const double pe = fma(rh, sh, -p) + (rh * sl + rl * sh);
-        const double seed = 1.0e-99;
-        const double s = p + seed, bb = s - p;
-        jac[k] = s + (((p - (s - bb)) + (seed - bb)) + pe);
-    }
+    struct Jq {
+        double sh, sl, rh, rl;
+    };
+    batched_for<8>(
+        TM::tid(), B.nnzA, TM::NT, [&](int k) { return Jq{jac[k], jlo[k], rhi[k], rlo[k]}; },
+        [&](int k, const Jq &q) {
+            // (rh + rl) (sh + sl) + seed: the leading product split exactly, the cross terms
+            // and the seed's TwoSum error added to it, rounded once
+            const double p = q.rh * q.sh;
+            const double pe = fma(q.rh, q.sh, -p) + (q.rh * q.sl + q.rl * q.sh);
+            const double seed = 1.0e-99;
+            const double s = p + seed, bb = s - p;
+            jac[k] = s + (((p - (s - bb)) + (seed - bb)) + pe);
+        });
     TM::sync();
 }
 
@@ -347,11 +351,18 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevKnobs &K
             Ap[rpos[k]] = val;
         }
     } else {
-        for (int k = t0; k < B.nnzA; k += TM::NT) {
-            const double val = reduced_value(Ag[k], dt, isdiag[k], stop);
-            Ag[k] = val;
-            Ap[rpos[k]] = val;
-        }
+        struct Rq {
+            double a;
+            int pos;
+            bool diag;
+        };
+        batched_for<8>(
+            t0, B.nnzA, TM::NT, [&](int k) { return Rq{Ag[k], rpos[k], isdiag[k] != 0}; },
+            [&](int k, const Rq &q) {
+                const double val = reduced_value(q.a, dt, q.diag, stop);
+                Ag[k] = val;
+                Ap[q.pos] = val;
+            });
     }
     stop = team_any<TM>(stop != 0, red) ? 1 : 0;
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);

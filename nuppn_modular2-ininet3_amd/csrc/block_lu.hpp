@@ -708,23 +708,32 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
     const int t0 = TM::tid();
     const int nblk = B.nblk;
     bool bad = false;
-    for (int i = t0; i < B.n; i += NT) {
-        const double d = gptr(Aval)[gptr(B.diag_a)[i]];
-        if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
-        gptr(F.cs)[i] = 1.0 / d;
-    }
+    // (the loops below issue the loads of several of a thread's elements before their stores:
+    // batched_for)
+    batched_for<8>(
+        t0, B.n, NT, [&](int i) { return gptr(Aval)[gptr(B.diag_a)[i]]; },
+        [&](int i, double d) {
+            if (!(fabs(d) > 0.0) || !isfinite(d)) bad = true;
+            gptr(F.cs)[i] = 1.0 / d;
+        });
     if (team_any<TM>(bad, L.red)) return 1;
     // scalings by padded position (interior blocks, then border): padding rows/columns stay 1
     for (int p = B.ni + t0; p < B.nip; p += NT) {
         gptr(F.colsc)[p] = 1.0;
         gptr(F.rowinv)[p] = 1.0;
     }
-    for (int i = t0; i < B.n; i += NT) {
-        const int p = gptr(B.vpos)[i];
-        const double ms = gptr(B.mass)[i];
-        gptr(F.colsc)[p] = ms * gptr(F.cs)[i];
-        gptr(F.rowinv)[p] = 1.0 / ms;
-    }
+    struct Sc {
+        double ms, c;
+        int p;
+    };
+    batched_for<8>(
+        t0, B.n, NT,
+        [&](int i) { return Sc{gptr(B.mass)[i], gptr(F.cs)[i], gptr(B.vpos)[i]}; },
+        [&](int i, const Sc &q) {
+            (void) i;
+            gptr(F.colsc)[q.p] = q.ms * q.c;
+            gptr(F.rowinv)[q.p] = 1.0 / q.ms;
+        });
     TM::sync();
     // dense blocks of the scaled matrix for all block steps at once (the loop then only streams
     // them, prefetched): the pattern's entries into the FP64 input areas, whose other positions
@@ -733,12 +742,18 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         const size_t area = (size_t) nblk * BS;
         NUPPN_GLB T *D0 = gptr(F.Din), *C0 = gptr(F.Cin), *B0 = gptr(F.Bin);
         const int nent = gptr(B.ent_ptr)[3 * nblk - 2];
-        for (int e = t0; e < nent; e += NT) {
-            const int dst = gptr(B.ent_dst)[e];
-            const int ar = (int) (dst / area);
-            NUPPN_GLB T *base = ar == 0 ? D0 : (ar == 1 ? C0 : B0);
-            base[dst - ar * area] = (T) block_entry(B, Aval, F.cs, e);
-        }
+        struct En {
+            double v;
+            int dst;
+        };
+        batched_for<8>(
+            t0, nent, NT, [&](int e) { return En{block_entry(B, Aval, F.cs, e), gptr(B.ent_dst)[e]}; },
+            [&](int e, const En &q) {
+                (void) e;
+                const int ar = (int) (q.dst / area);
+                NUPPN_GLB T *base = ar == 0 ? D0 : (ar == 1 ? C0 : B0);
+                base[q.dst - ar * area] = (T) q.v;
+            });
         const int pr = t0, last = nblk - 1;   // identity on the padded diagonal positions
         if (pr < NB && last * NB + pr >= B.ni) D0[(size_t) last * BS + pr * NB + pr] = T(1);
         TM::sync();
@@ -791,12 +806,23 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
         }
         for (int i = t0; i < nbd * nbd; i += NT) Gg[i] = 0.0;
         TM::sync();
-        for (int e = eptr[LE] + t0; e < eptr[LE + 1]; e += NT)
-            Eg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = eptr[LF] + t0; e < eptr[LF + 1]; e += NT)
-            XFg[eloc[e]] = (T) block_entry(B, Aval, F.cs, e);
-        for (int e = eptr[LG] + t0; e < eptr[LG + 1]; e += NT)
-            Gg[eloc[e]] = block_entry(B, Aval, F.cs, e);
+        struct Ev {
+            double v;
+            int loc;
+        };
+        auto entries = [&](int list, auto put) {
+            const int e0 = eptr[list];
+            batched_for<8>(
+                t0, eptr[list + 1] - e0, NT,
+                [&](int j) { return Ev{block_entry(B, Aval, F.cs, e0 + j), eloc[e0 + j]}; },
+                [&](int j, const Ev &q) {
+                    (void) j;
+                    put(q.loc, q.v);
+                });
+        };
+        entries(LE, [&](int loc, double v) { Eg[loc] = (T) v; });
+        entries(LF, [&](int loc, double v) { XFg[loc] = (T) v; });
+        entries(LG, [&](int loc, double v) { Gg[loc] = v; });
         TM::sync();
         const int lane = lane_id();
         const int vs = nip + 8;

@@ -368,6 +368,27 @@ template <class T>
 __device__ __forceinline__ const NUPPN_GLB T *gptr(const T *p) {
     return (const NUPPN_GLB T *) p;
 }
+// for (i = t0; i < n; i += nt) store(i, load(i)), with the loads of U of a thread's iterations
+// issued before any of their stores.  A plain loop whose stores might alias its loads waits
+// out a full memory round trip per iteration; here one round trip covers U iterations.  Each
+// element's arithmetic is unchanged.
+template <int U, class LD, class ST>
+__device__ __forceinline__ void batched_for(int t0, int n, int nt, LD load, ST store) {
+    for (int i0 = t0; i0 < n; i0 += U * nt) {
+        decltype(load(0)) r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * nt;
+            if (i < n) r[u] = load(i);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * nt;
+            if (i < n) store(i, r[u]);
+        }
+    }
+}
+
 // xn = X/A for considered isotopes (A > 0), xn(gamma) = 1 (jac_rhs.F90:235-240, 312-317)
 // (thread t0 of the nt that share the zone)
 __device__ inline void molar(const DevNet &N, const double *xp, double *xnp, int t0 = lane_id(),
@@ -387,13 +408,23 @@ __device__ inline void reaction_terms(const DevNet &N, const double *xnp, const 
     const NUPPN_GLB int *k = gptr(N.k);
     const NUPPN_GLB double *xn = gptr(xnp), *v = gptr(vp);
     NUPPN_GLB double *vx = gptr(vxp), *st1 = gptr(st1p), *st2 = gptr(st2p);
-    for (int l = t0; l < nre; l += nt) {
-        const int in1 = k[l], n1 = k[nre + l], in2 = k[2 * nre + l], n2 = k[3 * nre + l];
-        const double y1 = xn[in1], y2 = xn[in2], vl = v[l];
-        vx[l] = kFact[n1] * vl * powi(y1, n1) * y2;
-        st1[l] = (double) n1 * kFact[n1] * kFact[n2] * powi(y2, n2) * powi(y1, n1 - 1) * vl;
-        st2[l] = (double) n2 * kFact[n1] * kFact[n2] * powi(y1, n1) * powi(y2, n2 - 1) * vl;
-    }
+    struct Rt {
+        double y1, y2, vl;
+        int n1, n2;
+    };
+    batched_for<4>(
+        t0, nre, nt,
+        [&](int l) {
+            const int in1 = k[l], n1 = k[nre + l], in2 = k[2 * nre + l], n2 = k[3 * nre + l];
+            return Rt{xn[in1], xn[in2], v[l], n1, n2};
+        },
+        [&](int l, const Rt &q) {
+            const double y1 = q.y1, y2 = q.y2, vl = q.vl;
+            const int n1 = q.n1, n2 = q.n2;
+            vx[l] = kFact[n1] * vl * powi(y1, n1) * y2;
+            st1[l] = (double) n1 * kFact[n1] * kFact[n2] * powi(y2, n2) * powi(y1, n1 - 1) * vl;
+            st2[l] = (double) n2 * kFact[n1] * kFact[n2] * powi(y1, n1) * powi(y2, n2 - 1) * vl;
+        });
 }
 
 // dX/dt of reduced species r from the per-reaction fluxes, accumulated in reaction order
