@@ -166,11 +166,13 @@ __device__ __forceinline__ int refine_loop(const DevNet &N, const BKArgs &b, con
     const NUPPN_GLB double *dg = gptr((const double *) d), *yg = gptr(y), *rhsg = gptr(rhs);
     const NUPPN_GLB int *red2sp = gptr(N.red2sp);
     const int t0 = TM::tid();
-    for (int i = t0; i < n; i += TM::NT) {
-        xg[i] = 0.0;
-        xl[i] = 0.0;
-        resg[i] = rhsg[i];
-    }
+    batched_for<4>(
+        t0, n, TM::NT, [&](int i) { return rhsg[i]; },
+        [&](int i, double r) {
+            xg[i] = 0.0;
+            xl[i] = 0.0;
+            resg[i] = r;
+        });
     TM::sync();
     double eta_prev = 0.0;
     for (int k = 0; k < b.O.max_refine; ++k) {
@@ -178,19 +180,24 @@ __device__ __forceinline__ int refine_loop(const DevNet &N, const BKArgs &b, con
         st.n_solve++;
         double eta = 0.0;
         bool fin = true;
-        for (int i = t0; i < n; i += TM::NT) {
-            const double di = dg[i], xh = xg[i];
-            // (xh + xlo) + d: TwoSum, the rounding error into the low part
-            const double sh = xh + di;
-            const double bb = sh - xh;
-            const double lo = xl[i] + ((xh - (sh - bb)) + (di - bb));
-            const double hi = sh + lo;
-            xg[i] = hi;
-            xl[i] = lo - (hi - sh);
-            const double scale = fabs(yg[red2sp[i]] + hi) + kRefineFloor;
-            eta = fmax(eta, fabs(di) / scale);
-            fin = fin && isfinite(hi);
-        }
+        struct Uq {
+            double di, xh, xlo, y;
+        };
+        batched_for<4>(
+            t0, n, TM::NT, [&](int i) { return Uq{dg[i], xg[i], xl[i], yg[red2sp[i]]}; },
+            [&](int i, const Uq &q) {
+                const double di = q.di, xh = q.xh;
+                // (xh + xlo) + d: TwoSum, the rounding error into the low part
+                const double sh = xh + di;
+                const double bb = sh - xh;
+                const double lo = q.xlo + ((xh - (sh - bb)) + (di - bb));
+                const double hi = sh + lo;
+                xg[i] = hi;
+                xl[i] = lo - (hi - sh);
+                const double scale = fabs(q.y + hi) + kRefineFloor;
+                eta = fmax(eta, fabs(di) / scale);
+                fin = fin && isfinite(hi);
+            });
         eta = team_max<TM>(eta, red);
         if (team_any<TM>(!fin, red)) return FP64 ? -4 : 1;
         if (eta <= b.O.refine_tol) return 0;   // converged corrections
@@ -369,19 +376,28 @@ __device__ __forceinline__ int block_assemble(const DevNet &N, const DevKnobs &K
     const NUPPN_GLB double *yo = gptr(y_old), *yc = gptr(y_cur), *lo = gptr((const double *) lop);
     NUPPN_GLB double *rg = gptr(rhs);
     block_dxdt_dd<TM>(B, vx, rhs, lop);
-    for (int r = t0; r < N.n; r += TM::NT) {
-        // y_old - y_cur + dt (hi + lo): each step split exactly (TwoSum, FMA), rounded once
-        const int sp = red2sp[r];
-        const double a = yo[sp], b = -yc[sp], fh = rg[r];
-        const double s1 = a + b, b1 = s1 - a;
-        const double e1 = (a - (s1 - b1)) + (b - b1);
-        const double p = dt * fh;
-        const double e2 = fma(dt, fh, -p) + dt * lo[r];
-        const double s2 = s1 + p, b2 = s2 - s1;
-        const double e3 = (s1 - (s2 - b2)) + (p - b2);
-        // reduce_rhs packs the masked species only: the others solve to an exact zero
-        rg[r] = (!K.dynamic || dyn_marked(DW, sp)) ? s2 + (e1 + (e2 + e3)) : 0.0;
-    }
+    struct Hq {
+        double a, b, fh, fl;
+        int sp;
+    };
+    batched_for<4>(
+        t0, N.n, TM::NT,
+        [&](int r) {
+            const int sp = red2sp[r];
+            return Hq{yo[sp], -yc[sp], rg[r], lo[r], sp};
+        },
+        [&](int r, const Hq &q) {
+            // y_old - y_cur + dt (hi + lo): each step split exactly (TwoSum, FMA), rounded once
+            const double a = q.a, b = q.b, fh = q.fh;
+            const double s1 = a + b, b1 = s1 - a;
+            const double e1 = (a - (s1 - b1)) + (b - b1);
+            const double p = dt * fh;
+            const double e2 = fma(dt, fh, -p) + dt * q.fl;
+            const double s2 = s1 + p, b2 = s2 - s1;
+            const double e3 = (s1 - (s2 - b2)) + (p - b2);
+            // reduce_rhs packs the masked species only: the others solve to an exact zero
+            rg[r] = (!K.dynamic || dyn_marked(DW, q.sp)) ? s2 + (e1 + (e2 + e3)) : 0.0;
+        });
     TM::sync();
     return stop;
 }
@@ -469,13 +485,22 @@ __device__ __forceinline__ void zone_newton_form(const KArgs &a, const BKArgs &b
                                                       slot, lds, s.st);
     if (rc == 0) {
         // y_new = y_cur + (x + xlo), rounded once
-        for (int r = t0; r < N.n; r += TM::NT) {
-            const int sp = N.red2sp[r];
-            const double yc = y_cur[sp], xh = x[r];
-            const double sh = yc + xh;
-            const double bb = sh - yc;
-            y_new[sp] = sh + (((yc - (sh - bb)) + (xh - bb)) + xlo[r]);
-        }
+        struct Yq {
+            double yc, xh, xl;
+            int sp;
+        };
+        batched_for<4>(
+            t0, N.n, TM::NT,
+            [&](int r) {
+                const int sp = N.red2sp[r];
+                return Yq{y_cur[sp], x[r], xlo[r], sp};
+            },
+            [&](int r, const Yq &q) {
+                (void) r;
+                const double sh = q.yc + q.xh;
+                const double bb = sh - q.yc;
+                y_new[q.sp] = sh + (((q.yc - (sh - bb)) + (q.xh - bb)) + q.xl);
+            });
         TM::sync();
         double g = 0.0;
         for (int i = t0; i < N.nsp; i += TM::NT) {

@@ -968,7 +968,13 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
         }
         TM::sync();
     }
-    for (int c = t0; c < B.n; c += NT) x[c] = (double) v[vpos[c]] * colsc[vpos[c]] * unsc;
+    batched_for<4>(
+        t0, B.n, NT,
+        [&](int c) {
+            const int p = vpos[c];
+            return (double) v[p] * colsc[p];
+        },
+        [&](int c, double t) { x[c] = t * unsc; });
     TM::sync();
 }
 
@@ -1140,16 +1146,21 @@ __device__ __forceinline__ double block_residual(const DevBlock &B, const double
             absum[i] = t.a;
         });
     double om = 0.0;
-    for (int i = TM::tid(); i < B.n; i += TM::NT) {
-        // rhs - (hi + lo): the difference rhs - hi split exactly, then the low parts
-        const double b = rhs[i], hi = r[i];
-        const double s = b - hi;
-        const double bb = s - b;
-        const double err = (b - (s - bb)) + (-hi - bb);
-        const double ri = s + (err - lo[i]);
-        r[i] = ri;
-        om = fmax(om, fabs(ri) / (absum[i] + fabs(b) + kAbundanceFloor));
-    }
+    struct Rr {
+        double b, hi, lo, a;
+    };
+    batched_for<4>(
+        TM::tid(), B.n, TM::NT, [&](int i) { return Rr{rhs[i], r[i], lo[i], absum[i]}; },
+        [&](int i, const Rr &q) {
+            // rhs - (hi + lo): the difference rhs - hi split exactly, then the low parts
+            const double b = q.b, hi = q.hi;
+            const double s = b - hi;
+            const double bb = s - b;
+            const double err = (b - (s - bb)) + (-hi - bb);
+            const double ri = s + (err - q.lo);
+            r[i] = ri;
+            om = fmax(om, fabs(ri) / (q.a + fabs(b) + kAbundanceFloor));
+        });
     om = team_max<TM>(om, red);
     TM::sync();
     return om;

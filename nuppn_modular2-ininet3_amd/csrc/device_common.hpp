@@ -395,8 +395,12 @@ __device__ inline void molar(const DevNet &N, const double *xp, double *xnp, int
                              int nt = WAVE) {
     const NUPPN_GLB double *x = gptr(xp), *aw = gptr(N.aw);
     NUPPN_GLB double *xn = gptr(xnp);
-    for (int i = t0; i < N.nsp; i += nt)
-        xn[i] = i == N.igamma ? 1.0 : (aw[i] > 0.0 ? x[i] / aw[i] : 0.0);
+    struct Mq {
+        double x, a;
+    };
+    batched_for<4>(
+        t0, N.nsp, nt, [&](int i) { return Mq{x[i], aw[i]}; },
+        [&](int i, const Mq &q) { xn[i] = i == N.igamma ? 1.0 : (q.a > 0.0 ? q.x / q.a : 0.0); });
 }
 
 // per-reaction quantities: vx (calculate_dxdt flux, jac_rhs.F90:245) and the two Jacobian stems
