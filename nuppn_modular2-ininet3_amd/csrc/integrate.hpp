@@ -85,14 +85,21 @@ __device__ inline double sorted_sum(const double *vals, int n, P buf) {
 // calculate_ye (utils.F90:537-553)
 template <class P>
 __device__ inline double calculate_ye(const DevNet &N, const double *x, double *eta, P buf) {
-    for (int i = lane_id(); i < N.nsp; i += WAVE)
-        eta[i] = (i != N.igamma) ? (N.aw[i] - 2.0 * N.zw[i]) * x[i] / N.aw[i] : 0.0;
+    struct Eq {
+        double a, zz, xx;
+    };
+    batched_for<4>(
+        lane_id(), N.nsp, WAVE, [&](int i) { return Eq{N.aw[i], N.zw[i], x[i]}; },
+        [&](int i, const Eq &q) {
+            eta[i] = (i != N.igamma) ? (q.a - 2.0 * q.zz) * q.xx / q.a : 0.0;
+        });
     wave_sync();
     return (1.0 - sorted_sum(eta, N.nsp, buf)) * 0.5;
 }
 
 __device__ inline void copy_vec(double *dst, const double *src, int n) {
-    for (int i = lane_id(); i < n; i += WAVE) dst[i] = src[i];
+    batched_for<4>(
+        lane_id(), n, WAVE, [&](int i) { return src[i]; }, [&](int i, double v) { dst[i] = v; });
     wave_sync();
 }
 

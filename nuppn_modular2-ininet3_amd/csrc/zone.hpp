@@ -319,11 +319,12 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
             if (ierr == 0) {
                 s.dtdone = s.dtdone + s.dttry;
                 // yps = max(yps, 1e-99), then yps_old = yps (one read of yps)
-                for (int j = lane_id(); j < nsp; j += WAVE) {
-                    const double y = fmax(yps[j], 1.0e-99);
-                    yps[j] = y;
-                    yps_old[j] = y;
-                }
+                batched_for<4>(
+                    lane_id(), nsp, WAVE, [&](int j) { return fmax(yps[j], 1.0e-99); },
+                    [&](int j, double y) {
+                        yps[j] = y;
+                        yps_old[j] = y;
+                    });
                 wave_sync();
                 s.st.nsucc++;
                 if (s.dtdone >= dtwant) { s.st.nsubt = s.i; zone_finish(a, z, s, 0, true, sb); break; }
@@ -381,12 +382,13 @@ __device__ inline void zone_control(const KArgs &a, int z, NUPPN_LDS double *sb)
         }
         if (my_t9 >= K.min_t9_for_slv) {
             // backward_euler_newton set-up (backward_euler.F90:49-53)
-            for (int j = lane_id(); j < nsp; j += WAVE) {
-                const double y = yps[j];
-                y_old[j] = y;
-                y_cur[j] = y;
-                y_new[j] = y;
-            }
+            batched_for<4>(
+                lane_id(), nsp, WAVE, [&](int j) { return yps[j]; },
+                [&](int j, double y) {
+                    y_old[j] = y;
+                    y_cur[j] = y;
+                    y_new[j] = y;
+                });
             wave_sync();
             s.iter = 0;
             s.grd = 0.0;
