@@ -365,11 +365,58 @@ def main():
                      "ierr_nonzero_last": int((i2.cpu().numpy() != 0).sum()),
                      "max_abs_sumX_minus_1_end": float(np.abs(ym.sum(axis=1) - 1.0).max()),
                      "finite": bool(np.isfinite(ym).all())}
+        # the plain FP64 path on a sample of the same workload: zones stratified by the batched
+        # run's Newton count up to 120 (the plain path's one-wavefront GEPP takes ~0.1 s per
+        # Newton iteration, so the workload's hottest zones would take minutes), and the batched
+        # path on the same zones
+        uplain = None
+        if not a.no_plain:
+            nn = st2["n_newton"].astype(np.int64)
+            cand = np.flatnonzero(nn <= 120)
+            cand = cand[np.argsort(nn[cand], kind="stable")]
+            sel2 = np.sort(cand[np.linspace(0, cand.size - 1, min(a.plain_sample, cand.size))
+                                .round().astype(int)])
+            out = {}
+            for path in ("plain", "mixed"):
+                idx = torch.tensor(sel2, device=dev)
+                ys = tens(w2.x0)[idx].contiguous()
+                av = [t[idx].contiguous() for t in a2]
+                es = torch.full((len(sel2),), 0.5, dtype=torch.float64, device=dev)
+                is_ = torch.zeros(len(sel2), dtype=torch.int32, device=dev)
+                ss = torch.zeros((len(sel2), nstat), dtype=torch.int64, device=dev)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ptrs = [t.data_ptr() for t in (ys, av[0], av[1], av[2], av[3], es, av[4], is_, ss)]
+                if path == "plain":
+                    ctx.integrate_batch_dev(len(sel2), *ptrs, 0)
+                else:
+                    ctx.integrate_batch_mixed_dev(len(sel2), *ptrs)
+                torch.cuda.synchronize()
+                out[path] = (time.perf_counter() - t0, ys.cpu().numpy(), is_.cpu().numpy(),
+                             np.frombuffer(ss.cpu().numpy().tobytes(), dtype=hip.STATS_DTYPE))
+                log(f"uneven plain sample ({path}): {out[path][0] * 1e3:.1f} ms on {len(sel2)} zones")
+            (elp2, yp2, iep2, stp2), (elb2, yb2, ieb2, stb2) = out["plain"], out["mixed"]
+            mp = (yp2 > 1e-20) | (yb2 > 1e-20)
+            uplain = {"zones": int(len(sel2)),
+                      "selection": "stratified in the batched run's Newton count over the zones "
+                                   f"with at most 120 ({int(nn[sel2].min())} .. "
+                                   f"{int(nn[sel2].max())} Newton iterations, T9 "
+                                   f"{w2.t9[sel2, 0].min():.3f} .. {w2.t9[sel2, 0].max():.3f})",
+                      "plain_zone_timesteps_per_s": len(sel2) / elp2,
+                      "mixed_zone_timesteps_per_s": len(sel2) / elb2,
+                      "mixed_over_plain": elp2 / elb2,
+                      "ierr_nonzero_plain": int((iep2 != 0).sum()),
+                      "ierr_nonzero_mixed": int((ieb2 != 0).sum()),
+                      "max_rel_err_mixed_vs_plain": float(np.max(np.abs(yp2[mp] - yb2[mp]) /
+                                                                 np.maximum(yp2[mp], yb2[mp]))),
+                      "zones_same_decisions": int(((stp2["nsubt"] == stb2["nsubt"]) &
+                                                   (stp2["n_newton"] == stb2["n_newton"])).sum())}
         y64u, st64u = res["fp64"][1], res["fp64"][3]
         mu = (yy > 1e-20) | (y64u > 1e-20)
         relu = np.where(mu, np.abs(yy - y64u) / np.maximum(np.maximum(yy, y64u), 1e-300), 0.0)
         sameu = (st2["nsubt"] == st64u["nsubt"]) & (st2["n_newton"] == st64u["n_newton"])
         uneven = {"zones": n2, "zone_timesteps_per_s_mixed": n2 / el2,
+                  "plain_sample": uplain,
                   "zone_timesteps_per_s_fp64_factors": n2 / res["fp64"][0],
                   "mixed_equals_fp64_bitwise": bool(np.array_equal(yy, y64u)),
                   "zones_not_bitwise_equal": int((yy != y64u).any(axis=1).sum()),
