@@ -48,6 +48,10 @@ def _args():
                         "abundances carry over from cycle to cycle); the FP64-factor "
                         "comparison runs on the first cycle only")
     p.add_argument("--save", default="", help="write the step's abundances and counters (npz)")
+    p.add_argument("--bd-only", type=int, default=0, metavar="ZONES",
+                   help="run only the Bader-Deuflhard workload (4,096 zones x 1,100 species, "
+                        "T9 1..4) on the plain FP64 path for its first ZONES zones and print its "
+                        "figures (the batched path's BD step is switched off)")
     p.add_argument("--large-only", type=int, default=0, metavar="ZONES",
                    help="run only the 5,000-species network's batched cycle on its first ZONES "
                         "zones and print its figures (a profiling workload)")
@@ -151,6 +155,36 @@ def _cpu_zone(z):
     return ierr, time.perf_counter() - t
 
 
+def bd_workload(pkg, hip, zones):
+    """The Bader-Deuflhard workload (semi-implicit extrapolation, integration_method = 1): its
+    first `zones` zones of 4,096 (1,100 species, T9 log-uniform 1..4, rho 1e5..1e9, dt
+    1e-3..1e2 s) in one call on the plain FP64 path (device GEPP in the reference's pivot order,
+    one zone per wavefront, all zones concurrently)."""
+    w = pkg.network.config(4)
+    ctx = hip.Context(w.net, hip.default_knobs(integration_method=1))
+    try:
+        sel = np.arange(zones)
+        args = [v[sel] for v in (w.t9[:, 0], w.t9[:, 1], w.rho[:, 0], w.rho[:, 1], w.dt)]
+        ctx.integrate_batch(w.x0[:1], *[v[:1] for v in args])   # warm-up: one zone
+        t0 = time.perf_counter()
+        ie, y, _, st = ctx.integrate_batch(w.x0[sel], *args)
+        el = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    out = {"zones": zones, "path": "plain FP64 (device GEPP), Bader-Deuflhard",
+           "seconds": el, "zone_timesteps_per_s": zones / el,
+           "ierr_nonzero": int((ie != 0).sum()),
+           "sub_steps_max": int(st["nsubt"].max()),
+           "factorisations": int(st["n_factor"].sum()),
+           "factorisations_max_per_zone": int(st["n_factor"].max()),
+           "stage_solves": int(st["n_solve"].sum()),
+           "max_abs_sumX_minus_1": float(np.abs(y.sum(axis=1) - 1.0).max()),
+           "what": "4,096 zones x 1,100 species, T9 1..4, rho 1e5..1e9, dt 1e-3..1e2 s: the "
+                   "first zones of the workload"}
+    log(f"BD workload: {el:.1f} s for {zones} zones")
+    return out
+
+
 def large_network(pkg, hip, plain=True, zones=None):
     """The large network: 256 zones x 5,000 species (34,055 reactions) on the batched path's band
     form, one cycle, and the plain FP64 path on four of its cool zones."""
@@ -199,10 +233,13 @@ def main():
               f"zones across GPUs (one contiguous block of zones per GPU), which this one-GPU "
               f"build does not run", file=sys.stderr)
         sys.exit(2)
-    if a.large_only:
+    if a.large_only or a.bd_only:
         from __graft_entry__ import _load_package
         pkg = _load_package()
-        print(json.dumps(large_network(pkg, pkg.hip, plain=False, zones=a.large_only)))
+        if a.large_only:
+            print(json.dumps(large_network(pkg, pkg.hip, plain=False, zones=a.large_only)))
+        else:
+            print(json.dumps(bd_workload(pkg, pkg.hip, a.bd_only)))
         return
     # the CPU baseline first: no GPU state exists yet in this process or its workers
     cpu = None
