@@ -1014,16 +1014,26 @@ __device__ __forceinline__ void seg_round(const DevSeg &S, int r, V val, E emit)
     int s = s0;
     A acc{}, first{};
     bool has_first = false, open = false;
+    // the operand words of the next batch are loaded while this batch's operands are gathered
+    // (one memory round trip per batch instead of two)
+    int rawn[BATCH];
+    auto load_raw = [&](int j0) {
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j)
+            rawn[j] = eidx[base + WAVE * min(j0 + j, len - 1)];   // past the range: a repeat
+    };
+    if (len > 0) load_raw(0);
     for (int j0 = 0; j0 < len; j0 += BATCH) {
         A v[BATCH];
         bool end[BATCH];
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
-            const int P = base + WAVE * min(j0 + j, len - 1);   // past the range: a repeat
-            const int raw = eidx[P];
+            const int P = base + WAVE * min(j0 + j, len - 1);
+            const int raw = rawn[j];
             end[j] = raw < 0;
             v[j] = val(P, raw & 0x7fffffff);
         }
+        if (j0 + BATCH < len) load_raw(j0 + BATCH);
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
             if (j0 + j < len) {
