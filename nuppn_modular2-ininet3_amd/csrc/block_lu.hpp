@@ -872,14 +872,38 @@ __device__ __forceinline__ int block_factor(const DevBlock &B, const double *Ava
             }
         }
         TM::sync();
-        for (int b2 = 0; b2 < nbd; ++b2)
-            for (int b1 = 0; b1 < nbd; ++b1) {
-                double sacc = 0.0;
-                for (int j = t0; j < nip; j += NT)
-                    sacc += (double) Eg[(size_t) b1 * nip + j] * (double) XFg[(size_t) b2 * nip + j];
-                sacc = team_sum<TM>(sacc, L.red);
-                if (t0 == 0) Gg[b1 + b2 * nbd] -= sacc;
+        if (nbd == 3) {
+            // the nine sums of E X_F in one reduction (LDS scratch: the second operand area,
+            // free once the border columns are solved; a single wavefront needs none)
+            double s9[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s9[q] = 0.0;
+            for (int j = t0; j < nip; j += NT) {
+                double e[3], x[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    e[q] = (double) Eg[(size_t) q * nip + j];
+                    x[q] = (double) XFg[(size_t) q * nip + j];
+                }
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2)
+#pragma unroll
+                    for (int b1 = 0; b1 < 3; ++b1) s9[3 * b2 + b1] += e[b1] * x[b2];
             }
+            team_sum_k<TM, 9>(s9, (NUPPN_LDS double *) L.X1b);
+            if (t0 == 0)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int b1 = 0; b1 < 3; ++b1) Gg[b1 + b2 * 3] -= s9[3 * b2 + b1];
+        } else {
+            for (int b2 = 0; b2 < nbd; ++b2)
+                for (int b1 = 0; b1 < nbd; ++b1) {
+                    double sacc = 0.0;
+                    for (int j = t0; j < nip; j += NT)
+                        sacc += (double) Eg[(size_t) b1 * nip + j] * (double) XFg[(size_t) b2 * nip + j];
+                    sacc = team_sum<TM>(sacc, L.red);
+                    if (t0 == 0) Gg[b1 + b2 * nbd] -= sacc;
+                }
+        }
         TM::sync();
         int gbad = 0;
         if (t0 == 0) {
@@ -939,11 +963,23 @@ __device__ __forceinline__ void block_solve(const DevBlock &B, const BlockFactor
     TM::sync();
     if (nbd > 0) {
         NUPPN_LDS double *tb = L.bscr;
-        for (int bb = 0; bb < nbd; ++bb) {
-            double s = 0.0;
-            for (int j = t0; j < nip; j += NT) s += (double) Eg[(size_t) bb * nip + j] * (double) v[j];
-            s = team_sum<TM>(s, L.red);
-            if (t0 == 0) tb[bb] = (double) v[nip + bb] - s;
+        if (nbd == 3) {   // the usual border (n, p, alpha): the three sums in one reduction
+            double s3[3] = {0.0, 0.0, 0.0};
+            for (int j = t0; j < nip; j += NT) {
+                const double vj = (double) v[j];
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) s3[bb] += (double) Eg[(size_t) bb * nip + j] * vj;
+            }
+            team_sum_k<TM, 3>(s3, L.red);
+            if (t0 == 0)
+                for (int bb = 0; bb < 3; ++bb) tb[bb] = (double) v[nip + bb] - s3[bb];
+        } else {
+            for (int bb = 0; bb < nbd; ++bb) {
+                double s = 0.0;
+                for (int j = t0; j < nip; j += NT) s += (double) Eg[(size_t) bb * nip + j] * (double) v[j];
+                s = team_sum<TM>(s, L.red);
+                if (t0 == 0) tb[bb] = (double) v[nip + bb] - s;
+            }
         }
         TM::sync();
         // G' u = t with the row interchanges, unit-lower and upper triangular solves (thread 0)

@@ -116,6 +116,29 @@ __device__ __forceinline__ double team_sum(double v, NUPPN_LDS double *red) {
     }
 }
 
+// K independent team_sums with one pair of barriers (the same order of additions as K calls of
+// team_sum, so the same bits): scr holds K * NWAVE doubles of LDS
+template <class TM, int K>
+__device__ __forceinline__ void team_sum_k(double (&v)[K], NUPPN_LDS double *scr) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, WAVE);
+    if constexpr (TM::NWAVE > 1) {
+        if (lane_id() == 0)
+#pragma unroll
+            for (int k = 0; k < K; ++k) scr[k * TM::NWAVE + TM::wave()] = v[k];
+        TM::sync();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double m = scr[k * TM::NWAVE];
+#pragma unroll
+            for (int w = 1; w < TM::NWAVE; ++w) m = m + scr[k * TM::NWAVE + w];
+            v[k] = m;
+        }
+        TM::sync();
+    }
+}
+
 template <class TM>
 __device__ __forceinline__ bool team_any(bool p, NUPPN_LDS double *red) {
     const bool w = ballot(p) != 0;
