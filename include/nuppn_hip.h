@@ -221,9 +221,8 @@ int nuppn_integrate_batch_dev(nuppn_ctx *ctx, int nz, double *yps, const double 
  * team_zones: a pass whose work list holds at most this many zones runs every zone on a
  * workgroup of several wavefronts (shared assembly and residuals, a twisted two-ended
  * factorisation: a shorter iteration), a longer list on one wavefront per zone.  -1 (default):
- * twice the device's compute-unit count; 0: never.
- * team_waves: the wavefronts of such a workgroup, 2 or 4; 0 (default): 4 when the list has no
- * more zones than the device has compute units, else 2.
+ * every pass; 0: never.
+ * team_waves: the wavefronts of such a workgroup, 2 or 4; 0 (default): 4.
  * queue_passes: controller/worker passes the host queues before it reads the work-list counts
  * back (1..16, default 16; the counts stay on the device, so queued passes need no host round
  * trip).  While the list holds more zones than team_zones the host still reads back every pass.
@@ -246,7 +245,7 @@ void nuppn_default_batch_opts(nuppn_batch_opts *o);
 /* Autotuner of the batched path (no counterpart in the reference): times one cycle of the
  * sample batch (host arrays, as nuppn_integrate_batch_mixed) under each candidate shape - block
  * size 32 and 64 where the network's band allows it, and the multi-wavefront kernels for short
- * work lists (team_zones default / 0 / 4 x CUs with two wavefronts) - and keeps the fastest: the
+ * work lists (team_zones default / 0 / 2 x CUs with two wavefronts) - and keeps the fastest: the
  * context is rebuilt with that block size and its options become the defaults of later calls
  * with opts NULL.  cache_path (may be NULL): a text file of earlier choices keyed by network
  * (nsp, nre, nnz) and CU count; a hit applies it without timing, a new choice is appended.

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <climits>
 #include <mutex>
 #include <vector>
 
@@ -1363,7 +1364,7 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
     const long long max_pass = (long long) c->kn.max_sub_steps * (c->kn.ittd + 2) + 16;
     const bool bd = c->kn.integration_method == 1;
     const int queue = block ? std::max(1, std::min(bo->queue_passes, kPassRing)) : kPassRing;
-    const int team_max = block ? (bo->team_zones < 0 ? 2 * c->ncu : bo->team_zones) : 0;
+    const int team_max = block ? (bo->team_zones < 0 ? INT_MAX : bo->team_zones) : 0;
     c->newton_ms = 0.0;
     c->control_ms = 0.0;
     c->newton_launches = 0;
@@ -1451,9 +1452,10 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
                        dim3(std::max(1, std::min(nl, c->bgrid[1]))), dim3(256), args, c->blds64);
         } else {
             void *args[] = {&ka, &b};
-            // few zones: several wavefronts per zone shorten each one's iteration (four when the
-            // list fits one zone per CU, else two)
-            int nw = bo->team_waves ? bo->team_waves : (bnd <= c->ncu ? 4 : 2);
+            // several wavefronts per zone shorten each one's iteration; with four a zone takes
+            // a whole CU, and the zones of a longer list follow one another as workgroups finish
+            // (measured faster than one wavefront per zone even with 2,048 zones in the list)
+            int nw = bo->team_waves ? bo->team_waves : 4;
             if (!c->team_ok[nw]) nw = c->team_ok[2] ? 2 : 0;
             if (!bd && nw && bnd <= team_max) {
                 e = kernel(team_kernel_ptr(c->bnb, nw), dim3(nl), dim3(64 * nw), args,
@@ -1873,7 +1875,7 @@ int nuppn_autotune(nuppn_ctx *c, int nz, const double *yps, const double *t9_0,
     HIP_OK(hipMemcpy(c->d_rho_1, rho_1, sizeof(double) * nz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_dt, dtwant, sizeof(double) * nz, hipMemcpyHostToDevice));
     std::vector<double> first(nsp * nz), cur(nsp * nz);
-    const int team_choices[3][2] = {{-1, 0}, {0, 0}, {4 * c->ncu, 2}};
+    const int team_choices[3][2] = {{-1, 0}, {0, 0}, {2 * c->ncu, 2}};
     const int nb0 = c->bnb;
     double best = 1e300;
     res->consistent = 1;
