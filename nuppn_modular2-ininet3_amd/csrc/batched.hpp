@@ -656,9 +656,10 @@ __global__ void __launch_bounds__(64, W) zone_newton_block_kernel(KArgs a, BKArg
 // residuals, the dense-block set-up, the border columns and every element-wise pass are shared,
 // and the block factorisation and solves run twisted, top-down on wavefront 0 and bottom-up on
 // wavefront 1.  For passes whose list is short, where one zone's iteration latency sets the pass
-// time.
-template <int NB, int NW>
-__global__ void __launch_bounds__(64 * NW, 1) zone_newton_team_kernel(KArgs a, BKArgs b) {
+// time.  MINB = 2 holds the registers to two such workgroups per CU (long lists: more zones in
+// flight, each iteration a little slower).
+template <int NB, int NW, int MINB = 1>
+__global__ void __launch_bounds__(64 * NW, MINB) zone_newton_team_kernel(KArgs a, BKArgs b) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     NUPPN_LDS char *lds = (NUPPN_LDS char *) smem;
     char *slot = b.slot_base + (size_t) blockIdx.x * b.slot_bytes;

@@ -337,6 +337,7 @@ struct nuppn_ctx {
     BLayout blayout{};
     size_t blds32 = 0, blds64 = 0;   // LDS per wavefront with FP32 / FP64 factors
     size_t blds_team = 0;            // LDS of a multi-wavefront kernel's workgroup
+    int team_ok2 = 0;                // the two-per-CU four-wavefront variant fits
     int team_ok[5] = {};             // [2], [4]: that multi-wavefront kernel fits the device
     int ncu = 0;                     // compute units of the device
     int bmax_slots = 0, bslots = 0;
@@ -741,6 +742,19 @@ static int setup_batched(nuppn_ctx *c, int force_nb = 0) {
                 hipSuccess &&
             per_cu >= 1)
             c->team_ok[nw] = 1;
+    }
+    c->team_ok2 = 0;
+    if (nb == 32 && method == 0 && c->team_ok[4]) {
+        const void *tk2 = (const void *) zone_newton_team_kernel<32, 4, 2>;
+        int per_cu = 0;
+        if (ensure_stack(tk2)) return NUPPN_ERR_HIP;
+        if ((c->blds_team <= 64 * 1024 ||
+             hipFuncSetAttribute(tk2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) c->blds_team) == hipSuccess) &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tk2, 256, c->blds_team) ==
+                hipSuccess &&
+            per_cu >= 2)
+            c->team_ok2 = 1;
     }
     c->bnb = nb;
     return 0;
@@ -1458,8 +1472,11 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
             int nw = bo->team_waves ? bo->team_waves : 4;
             if (!c->team_ok[nw]) nw = c->team_ok[2] ? 2 : 0;
             if (!bd && nw && bnd <= team_max) {
-                e = kernel(team_kernel_ptr(c->bnb, nw), dim3(nl), dim3(64 * nw), args,
-                           c->blds_team);
+                // a list longer than one zone per CU: two four-wavefront workgroups per CU
+                const bool two = nw == 4 && c->bnb == 32 && bnd > c->ncu && c->team_ok2;
+                e = kernel(two ? (const void *) zone_newton_team_kernel<32, 4, 2>
+                               : team_kernel_ptr(c->bnb, nw),
+                           dim3(nl), dim3(64 * nw), args, c->blds_team);
             } else {
                 // the register-lean two-per-SIMD variant only when the single-wavefront one
                 // cannot give every zone of the list its own wavefront
