@@ -562,8 +562,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
                      "kernel": "the Newton worker, one launch per pass: "
-                               "zone_newton_block_kernel<32, 1|2> (many zones) or "
-                               "zone_newton_team_kernel<32, 2|4> (short work lists)",
+                               "zone_newton_team_kernel<32, 4> (four wavefronts per zone; two "
+                               "such workgroups per CU while the list holds more zones than "
+                               "the device has CUs)",
                      "launches": launches, "avg_launch_ms": 1e3 * avg_launch_s,
                      # time-weighted: the same algorithmic bytes over the whole timed step
                      "pipeline_frac": bytes_step / (elapsed / a.steps) / 1e9 / HBM_PEAK_GBS},
