@@ -248,9 +248,11 @@ __device__ inline void zone_screening(const DevNet &N, double t9, double rho, co
     const double zeta = pow(fourov3pi * tp * tp / tk / tk, third);
     const double c_a1 = 2.7822, c_a2 = 98.34, c_a3 = sqrt(3.0) - c_a1 / sqrt(c_a2);
     const double c_b1 = -1.7476, c_b2 = 66.07, c_b3 = 1.12, c_b4 = 65.0, alfa = 0.022;
-    for (int i = lane_id(); i < nre; i += WAVE) {
-        const double vi = v[i];
-        if (vi <= 1.0e-99) continue;
+    // in place, each rate from itself: 8 of a lane's rates are read before any is written
+    NUPPN_GLB double *vg = gptr(v);
+    batched_for<8>(lane_id(), nre, WAVE, [&](int i) {
+        const double vi = vg[i];
+        if (vi <= 1.0e-99) return vi;
         const int r1 = N.k[i], r2 = N.k[nre + i] == 2 ? N.k[i] : N.k[2 * nre + i];
         const double z1 = N.zw[r1], z2 = N.zw[r2];
         double screen = 1.0;
@@ -274,8 +276,8 @@ __device__ inline void zone_screening(const DevNet &N, double t9, double rho, co
             screen = fmin(exp(t1 + t2 + t3), 1.0e30);
         }
         screen = fmin(screen, 1.0e30);
-        v[i] = vi * screen;
-    }
+        return vi * screen;
+    }, [&](int i, double r) { vg[i] = r; });
     wave_sync();
 }
 
@@ -286,28 +288,39 @@ __device__ inline void zone_screening(const DevNet &N, double t9, double rho, co
 __device__ inline void zone_rates(const DevNet &N, double t9, double rho, double *v,
                                   const double *x, NUPPN_LDS double *buf) {
     const DevPhys &P = N.P;
+    // the rates of 8 of a lane's reactions are formed before any is stored (the stores to v
+    // could alias the coefficient loads, which would cost a memory round trip per reaction)
+    NUPPN_GLB double *vg = gptr(v);
+    const auto st = [&](int l, double r) { vg[l] = r; };
     if (N.rcache) {
         const CachePos c = cache_pos(N, t9);
-        for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = cached_rate(N, l, N.rtype[l], c, rho);
+        batched_for<8>(
+            lane_id(), N.nre, WAVE, [&](int l) { return cached_rate(N, l, N.rtype[l], c, rho); },
+            st);
     } else {
         const T9Pow p = t9_powers(t9);
         if (P.pf_n > 0) {
             const PfPos q = pf_pos(P, t9);
-            for (int l = lane_id(); l < N.nre; l += WAVE) {
-                const bool rev = P.rev_ch[l] > 0;
-                v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho,
-                                    rev ? reverse_gfac(P, N.nre, l, q) : 1.0, rev);
-            }
+            batched_for<8>(
+                lane_id(), N.nre, WAVE,
+                [&](int l) {
+                    const bool rev = P.rev_ch[l] > 0;
+                    return reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho,
+                                        rev ? reverse_gfac(P, N.nre, l, q) : 1.0, rev);
+                },
+                st);
         } else {
-            for (int l = lane_id(); l < N.nre; l += WAVE)
-                v[l] = reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho);
+            batched_for<8>(
+                lane_id(), N.nre, WAVE,
+                [&](int l) { return reaclib_rate(N.coef, N.nre, l, N.rtype[l], p, rho); }, st);
         }
     }
     wave_sync();
     if (P.pf_n > 0 && P.ndb > 0) zone_detailed_balance(N, t9, rho, v);
     if (P.screen) zone_screening(N, t9, rho, x, v, buf);
     if (P.ndb > 0 || P.screen) {
-        for (int l = lane_id(); l < N.nre; l += WAVE) v[l] = fmax(v[l], 1.0e-99);
+        batched_for<8>(
+            lane_id(), N.nre, WAVE, [&](int l) { return fmax(vg[l], 1.0e-99); }, st);
         wave_sync();
     }
 }

@@ -137,6 +137,12 @@ __global__ void __launch_bounds__(64) zone_init_kernel(KArgs a) {
     zone_init(a, blockIdx.x, (NUPPN_LDS double *) smem);
 }
 
+// the final Ye of every zone of a batch (after the last pass)
+__global__ void __launch_bounds__(64) zone_ye_kernel(KArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    zone_final_ye(a, blockIdx.x, (NUPPN_LDS double *) smem);
+}
+
 // one controller pass over the zones of the previous pass's work list (every zone at first: the
 // zones not on it are done)
 __global__ void __launch_bounds__(64) zone_control_kernel(KArgs a) {
@@ -982,6 +988,9 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
                                  (int) c->sort_lds) != hipSuccess ||
              hipFuncSetAttribute((const void *) zone_init_kernel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) c->sort_lds) != hipSuccess ||
+             hipFuncSetAttribute((const void *) zone_ye_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int) c->sort_lds) != hipSuccess))
             return nullptr;
     }
@@ -993,7 +1002,8 @@ nuppn_ctx *nuppn_ctx_create(const nuppn_network *net, const nuppn_knobs *knobs, 
     const void *plain_kernel = kn.integration_method == 1 ? (const void *) zone_bd_kernel
                                                           : (const void *) zone_newton_kernel;
     if (ensure_stack(plain_kernel) || ensure_stack((const void *) zone_control_kernel) ||
-        ensure_stack((const void *) zone_init_kernel) || ensure_stack((const void *) lu_kernel) ||
+        ensure_stack((const void *) zone_init_kernel) ||
+        ensure_stack((const void *) zone_ye_kernel) || ensure_stack((const void *) lu_kernel) ||
         ensure_stack((const void *) assemble_kernel))
         return nullptr;
     if (c->lds > 64 * 1024 &&
@@ -1565,6 +1575,10 @@ static int run_batch(nuppn_ctx *c, int nz, double *yps, const double *t9_0, cons
         pass += ch;
     }
     c->last_passes = done_pass;
+    if (!bd) {
+        hipLaunchKernelGGL(zone_ye_kernel, dim3(nz), dim3(64), c->sort_lds, s, a);
+        HIP_OK(hipGetLastError());
+    }
     HIP_OK(hipEventRecord(c->ev[3], s));
     if (c->kn.integration_method == 1) {
         // the state the reference would carry into its next call: the last zone's (exact for

@@ -211,13 +211,9 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
         }
         wave_sync();
     }
-    double ye = s.ye_entry;
-    if (a.K.integration_method == 1) {
-        ye = s.ye_cur;
-    } else if (s.solved) {
-        double *work = zrow(Z.work, Z.work_stride, z);
-        ye = calculate_ye(N, yps, work, sb);
-    }
+    // a zone that solved under backward Euler gets the Ye of its final abundances from
+    // zone_final_ye after the last pass (its sorted sum would hold up the whole pass here)
+    const double ye = a.K.integration_method == 1 ? s.ye_cur : s.ye_entry;
     s.phase = PH_DONE;
     s.ierr = ierr;
     if (!a.K.dynamic) s.st.nvar1 = N.n;
@@ -226,6 +222,18 @@ __device__ inline void zone_finish(const KArgs &a, int z, ZoneState &s, int ierr
         Z.ierr[z] = s.lib_err ? s.lib_err : ierr;
         if (Z.stats) Z.stats[z] = s.st;
     }
+}
+
+// Ye of zone z once every pass has ended: the final abundances' Ye for a zone that solved under
+// backward Euler (the Ye of the last step's result, solver.F90:272, equals it: ypsplus leaves
+// every value >= 1e-99, so the final clip changes nothing); zone_finish wrote the entry Ye,
+// which stands for every other zone
+__device__ inline void zone_final_ye(const KArgs &a, int z, NUPPN_LDS double *sb) {
+    const ZoneState s = a.Z.state[z];
+    if (a.K.integration_method == 1 || s.phase != PH_DONE || !s.solved) return;
+    const double ye = calculate_ye(a.N, zrow(a.Z.yps, a.Z.nsp_stride, z),
+                                   zrow(a.Z.work, a.Z.work_stride, z), sb);
+    if (lane_id() == 0) a.Z.ye[z] = ye;
 }
 
 // one pass of the controller for zone z: consume the result of a Newton iteration if there is
